@@ -1,0 +1,254 @@
+/*
+ * ORACLE -- test infrastructure, not product code.
+ *
+ * Flat entry points over the oracle for the Python test suite (ctypes): whole arrays in, whole
+ * arrays out, layouts matching the product's ([k][n] planar).  `nm` selects the Nelder-Mead
+ * routine: NULL for the oracle's restatement, or the reference's own nelmin loaded from
+ * oracle/_ref (same calling convention).
+ */
+#include <math.h>
+#include <string.h>
+
+#include "photic_oracle.h"
+
+typedef struct {
+    double *x;        /* [n_params][n] */
+    float *depth, *rrs_error, *model_error, *k_min, *iod, *albedo, *bottom_type;
+    float *b_pct;     /* [3][n] */
+    int *n_iter, *converged, *n_evals, *n_starts, *status;
+} orc_py_out;
+
+/* Invert n single-date pixels exactly as the product's single-date path is specified. */
+long orc_py_invert(int mode, int n_bottoms, const int *bottom_idx, const double *wl, int nb,
+                   long n, const float *rrs /*[nb][n]*/, const float *sec_sun /*[n] or NULL*/,
+                   double sec_sun0, double reqmin, int konvge, int kcount, int n_hstarts,
+                   const double *hstart, double early_exit, float nodata, orc_nelmin_fn nm,
+                   const float *seed /*[np][n] or NULL*/, const double *seed_rel,
+                   const double *seed_min, orc_py_out *o)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    orc_nm_cfg cfg;
+    orc_default_cfg(&cfg, 1, reqmin, konvge, kcount);
+    cfg.n_hstarts = n_hstarts;
+    for (int k = 0; k < n_hstarts && k < 16; k++)
+        cfg.hstart[k] = hstart[k];
+    if (early_exit > 0.0)
+        cfg.early_exit = early_exit;
+    long done = 0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : done)
+    for (long p = 0; p < n; p++) {
+        double r[ORC_MAXB], ss = sec_sun ? (double)sec_sun[p] : sec_sun0, sv = 1.0;
+        int bad = 0;
+        for (int b = 0; b < nb; b++) {
+            float v = rrs[(long)b * n + p];
+            if (orc_approx_equal(v, nodata, 1.0e-6f) || v < 0.0f)
+                bad = 1;
+            r[b] = (double)v;
+        }
+        orc_pixel px;
+        orc_pixel_init(&px, mode, n_bottoms, bottom_idx, 1, &bands, &ss, &sv, r);
+        int np_ = orc_n_params(&px);
+        if (bad) {
+            if (o->status) o->status[p] = -1;
+            if (o->depth) o->depth[p] = 0.0f;
+            if (o->x)
+                for (int i = 0; i < np_; i++) o->x[(long)i * n + p] = 0.0;
+            continue;
+        }
+        orc_result res;
+        if (seed) {
+            double st[ORC_MAXN], sp[ORC_MAXN];
+            int off = mode == ORC_MODE_MIX ? 2 : 1 + 2 * n_bottoms;
+            for (int i = 0; i < np_; i++) {
+                int cls = i == 0 ? 0 : (i < off ? 1 : 2 + (i - off) % 3);
+                st[i] = (double)seed[(long)i * n + p];
+                double s = fabs(st[i]) * seed_rel[cls];
+                if (s < seed_min[cls]) s = seed_min[cls];
+                sp[i] = s;
+            }
+            orc_optimise_seeded(&px, &cfg, nm, st, sp, &res);
+        } else {
+            orc_optimise(&px, &cfg, nm, &res);
+        }
+        if (o->x)
+            for (int i = 0; i < np_; i++) o->x[(long)i * n + p] = res.x[i];
+        if (o->depth) o->depth[p] = (float)res.depth;
+        if (o->rrs_error) o->rrs_error[p] = (float)res.Rrs_error;
+        if (o->model_error) o->model_error[p] = (float)res.model_error;
+        if (o->k_min) o->k_min[p] = (float)res.K_min;
+        if (o->iod) o->iod[p] = (float)res.iod;
+        if (o->albedo) o->albedo[p] = (float)res.bottom_albedo;
+        if (o->bottom_type) o->bottom_type[p] = (float)res.bottom_type;
+        if (o->b_pct)
+            for (int k = 0; k < 3; k++) o->b_pct[(long)k * n + p] = (float)res.B_percent[k];
+        if (o->n_iter) o->n_iter[p] = res.n_iterations;
+        if (o->converged) o->converged[p] = res.converged;
+        if (o->n_evals) o->n_evals[p] = (int)res.n_evals;
+        if (o->n_starts) o->n_starts[p] = res.n_starts;
+        if (o->status) o->status[p] = res.converged ? 0 : 2;
+        done++;
+    }
+    return done;
+}
+
+/* Forward model: params [n][5] = (H, P, G, X, B mix), sec_sun [n], Y [n] -> Rrs [n][nb]. */
+void orc_py_forward(const double *wl, int nb, long n, const double *params,
+                    const double *sec_sun, const double *Y, double *out)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    for (long i = 0; i < n; i++) {
+        const double *p = params + i * 5;
+        for (int b = 0; b < nb; b++) {
+            double rho = p[4] * bands.bottom[0][b] + (1.0 - p[4]) * bands.bottom[1][b];
+            out[i * nb + b] = orc_rrs_core(p[1], p[2], p[3], rho, p[0], Y[i], &bands, b,
+                                           sec_sun[i], 1.0, NULL);
+        }
+    }
+}
+
+/* Objective of MIX-layout vectors x [n][5] for pixels rrs [n][nb], sec_sun [n]. */
+void orc_py_objective(const double *wl, int nb, long n, const double *x, const float *rrs,
+                      const float *sec_sun, double *out, double *rrs_error)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    for (long i = 0; i < n; i++) {
+        double r[ORC_MAXB], ss = sec_sun[i], sv = 1.0, xv[5];
+        for (int b = 0; b < nb; b++)
+            r[b] = (double)rrs[i * nb + b];
+        orc_pixel px;
+        orc_pixel_init(&px, ORC_MODE_MIX, 2, NULL, 1, &bands, &ss, &sv, r);
+        for (int k = 0; k < 5; k++)
+            xv[k] = x[i * 5 + k];
+        out[i] = orc_objective(xv, &px);
+        if (rrs_error)
+            rrs_error[i] = px.Rrs_error;
+    }
+}
+
+/* Synthetic single-date pixels pix0 .. pix0+n-1: rrs [nb][n], sec_sun [n], truth [6][n]
+ * (H, P, G, X, B, theta), nodata flag [n], Y [n]. */
+void orc_py_gen(uint64_t seed, long pix0, long n, const double *wl, int nb, double sigma,
+                double deep_frac, float *rrs, float *sec_sun, double *truth, int *nodata,
+                double *Y)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        orc_truth t;
+        float r[ORC_MAXB];
+        double y;
+        int nd = orc_gen_pixel(seed, (uint64_t)(pix0 + i), &bands, sigma, deep_frac, &t, r, &y);
+        for (int b = 0; b < nb; b++)
+            rrs[(long)b * n + i] = r[b];
+        sec_sun[i] = (float)orc_sec_deg(t.theta_sun);
+        if (truth) {
+            truth[0 * n + i] = t.H;
+            truth[1 * n + i] = t.P;
+            truth[2 * n + i] = t.G;
+            truth[3 * n + i] = t.X;
+            truth[4 * n + i] = t.B;
+            truth[5 * n + i] = t.theta_sun;
+        }
+        if (nodata) nodata[i] = nd;
+        if (Y) Y[i] = y;
+    }
+}
+
+/* Band tables at the given wavelengths: out [13][nb] = a0, a1, aw, bbw, 8 bottoms, and
+ * aw640 in out2. */
+void orc_py_band_tables(const double *wl, int nb, double *out, double *aw640)
+{
+    orc_bands t;
+    orc_band_tables(wl, nb, &t);
+    for (int b = 0; b < nb; b++) {
+        out[0 * nb + b] = t.a0[b];
+        out[1 * nb + b] = t.a1[b];
+        out[2 * nb + b] = t.aw[b];
+        out[3 * nb + b] = t.bbw[b];
+        for (int k = 0; k < ORC_NBOTTOM_TYPES; k++)
+            out[(4 + k) * nb + b] = t.bottom[k][b];
+    }
+    *aw640 = t.aw640;
+}
+
+/* Test functions for the minimiser: kind 0 = quadratic sum_i w_i (x_i - c_i)^2, kind 1 =
+ * Rosenbrock chain, kind 2 = a quadratic floored at truncation steps (exercises the
+ * (long)(10 f) local-minimum test). */
+typedef struct {
+    int kind, n;
+    double w[32], c[32];
+    long evals;
+} orc_testfn;
+
+static double orc_testfn_eval(double x[], void *ctx)
+{
+    orc_testfn *t = (orc_testfn *)ctx;
+    double f = 0.0;
+    t->evals++;
+    if (t->kind == 1) {
+        for (int i = 0; i + 1 < t->n; i++) {
+            double a = x[i + 1] - x[i] * x[i], b = 1.0 - x[i];
+            f += 100.0 * a * a + b * b;
+        }
+    } else {
+        for (int i = 0; i < t->n; i++) {
+            double d = x[i] - t->c[i];
+            f += t->w[i] * d * d;
+        }
+        if (t->kind == 2)
+            f = floor(f * 10.0) / 10.0;
+    }
+    return f;
+}
+
+void orc_py_nelmin_test(orc_nelmin_fn nm, int kind, int n, const double *w, const double *c,
+                        double *start, const double *step, double reqmin, int konvge,
+                        int kcount, double *xmin, double *ynewlo, int *icount, int *numres,
+                        int *ifault, long *evals)
+{
+    orc_testfn t;
+    memset(&t, 0, sizeof(t));
+    t.kind = kind;
+    t.n = n;
+    for (int i = 0; i < n; i++) {
+        t.w[i] = w ? w[i] : 1.0;
+        t.c[i] = c ? c[i] : 0.0;
+    }
+    double st[32];
+    memcpy(st, step, sizeof(double) * (size_t)n);
+    if (!nm)
+        nm = orc_nelmin;
+    nm(orc_testfn_eval, &t, n, start, xmin, ynewlo, reqmin, st, konvge, kcount, icount,
+       numres, ifault);
+    *evals = t.evals;
+}
+
+/* The oracle's restatement and the reference's helpers side by side. */
+double orc_py_interp(const double *X, const double *Y, int n, double x)
+{
+    return orc_interp_1d(X, Y, n, x);
+}
+
+int orc_py_approx_equal(float a, float b, float eps) { return orc_approx_equal(a, b, eps); }
+
+void orc_py_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out)
+{
+    orc_philox4x32_10(ctr, key, out);
+}
+
+void orc_py_mlp_forward(const double *w, int n, const double *in, double *out)
+{
+    orc_mlp_forward(w, n, in, out);
+}
+
+double orc_py_mlp_loss_grad(const double *w, int n, const double *in, const double *target,
+                            double *grad)
+{
+    return orc_mlp_loss_grad(w, n, in, target, grad);
+}
+
+int orc_py_mlp_nparam(void) { return ORC_MLP_NPARAM; }

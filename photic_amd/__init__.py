@@ -1,0 +1,296 @@
+"""photic_amd -- MI355X inversion of photic's per-pixel semi-analytical model.
+
+Python mirror of the C ABI in include/photic.h (libphotic.so, built in-tree by
+photic_amd/build.py).  The library is the product; this module only marshals numpy arrays to
+device buffers and back.  It raises ImportError when libphotic.so is missing and
+PhoticError when the library reports an error -- there is no CPU fallback.
+"""
+import ctypes as C  # noqa: N812
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphotic.so")
+
+MAX_BANDS = 8
+MAX_HSTARTS = 16
+MODE_NATIVE, MODE_MIX = 0, 1
+FP64, FP32 = 0, 1
+OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM = 0, 1, 2, 3, 4
+
+
+class PhoticError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("photic error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Cfg(C.Structure):
+    _fields_ = [
+        ("n_bands", C.c_int),
+        ("wavelengths", C.c_double * MAX_BANDS),
+        ("mode", C.c_int),
+        ("n_bottoms", C.c_int),
+        ("bottom_idx", C.c_int * 8),
+        ("reqmin", C.c_double),
+        ("konvge", C.c_int),
+        ("kcount", C.c_int),
+        ("n_hstarts", C.c_int),
+        ("hstart", C.c_double * MAX_HSTARTS),
+        ("early_exit", C.c_double),
+        ("nodata", C.c_float),
+        ("sec_sun", C.c_double),
+        ("sec_view", C.c_double),
+        ("precision", C.c_int),
+        ("seed_rel_step", C.c_double * 5),
+        ("seed_min_step", C.c_double * 5),
+        ("block", C.c_int),
+        ("grid", C.c_int),
+        ("chunk", C.c_int),
+        ("watchdog_s", C.c_double),
+        ("check_watchdog", C.c_int),
+    ]
+
+
+class Outputs(C.Structure):
+    _fields_ = [
+        ("depth", C.c_void_p),
+        ("rrs_error", C.c_void_p),
+        ("model_error", C.c_void_p),
+        ("albedo", C.c_void_p),
+        ("k_min", C.c_void_p),
+        ("iod", C.c_void_p),
+        ("b_pct", C.c_void_p),
+        ("bottom_type", C.c_void_p),
+        ("x", C.c_void_p),
+        ("status", C.c_void_p),
+        ("n_evals", C.c_void_p),
+        ("n_iter", C.c_void_p),
+        ("n_starts", C.c_void_p),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libphotic.so is not built: run photic_amd/build.py (or "
+                          "__graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    vp, i, l, d = C.c_void_p, C.c_int, C.c_long, C.c_double
+    sig = {
+        "photic_cfg_default": (None, [C.POINTER(Cfg), i, C.POINTER(C.c_double)]),
+        "photic_n_params": (i, [C.POINTER(Cfg)]),
+        "photic_invert": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, C.POINTER(Outputs), vp]),
+        "photic_gen_scene": (i, [C.POINTER(Cfg), C.c_uint64, l, l, d, d, vp, vp, vp, vp]),
+        "photic_forward": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, i, vp]),
+        "photic_objective": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, vp, i, vp]),
+        "photic_kernel_info": (i, [C.POINTER(Cfg), C.POINTER(i), C.POINTER(i), C.POINTER(i),
+                                   C.POINTER(i)]),
+        "photic_device_count": (i, []),
+        "photic_device_malloc": (i, [C.POINTER(vp), C.c_size_t]),
+        "photic_device_free": (i, [vp]),
+        "photic_memcpy_htod": (i, [vp, vp, C.c_size_t]),
+        "photic_memcpy_dtoh": (i, [vp, vp, C.c_size_t]),
+        "photic_memset_zero": (i, [vp, C.c_size_t]),
+        "photic_synchronize": (i, []),
+        "photic_stream_create": (i, [C.POINTER(vp)]),
+        "photic_stream_destroy": (i, [vp]),
+        "photic_event_create": (i, [C.POINTER(vp)]),
+        "photic_event_destroy": (i, [vp]),
+        "photic_event_record": (i, [vp, vp]),
+        "photic_event_elapsed_ms": (i, [vp, vp, C.POINTER(C.c_float)]),
+        "photic_last_error": (C.c_char_p, []),
+        "ph_build_const": (i, [C.POINTER(Cfg), i, i, vp]),
+        "ph_const_size": (i, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc):
+    if rc != OK:
+        raise PhoticError(rc, lib.photic_last_error().decode())
+
+
+def device_count():
+    return lib.photic_device_count()
+
+
+def cfg_default(wavelengths=(443.0, 482.0, 561.0, 655.0), **kw):
+    """The reference's settings (model/samodel.c:2142-2147) in MIX mode; keyword overrides."""
+    cfg = Cfg()
+    wl = (C.c_double * MAX_BANDS)(*wavelengths)
+    lib.photic_cfg_default(C.byref(cfg), len(wavelengths), wl)
+    for k, v in kw.items():
+        if k == "hstart":
+            cfg.n_hstarts = len(v)
+            for j, h in enumerate(v):
+                cfg.hstart[j] = h
+        elif k == "bottom_idx":
+            for j, b in enumerate(v):
+                cfg.bottom_idx[j] = b
+        elif k in ("seed_rel_step", "seed_min_step"):
+            arr = getattr(cfg, k)
+            for j, s in enumerate(v):
+                arr[j] = s
+        else:
+            setattr(cfg, k, v)
+    return cfg
+
+
+def n_params(cfg):
+    return lib.photic_n_params(C.byref(cfg))
+
+
+class DeviceBuffer:
+    """A device allocation; frees itself."""
+
+    def __init__(self, nbytes):
+        self.ptr = C.c_void_p()
+        self.nbytes = int(nbytes)
+        check(lib.photic_device_malloc(C.byref(self.ptr), max(self.nbytes, 1)))
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes)
+        if a.nbytes:
+            check(lib.photic_memcpy_htod(buf.ptr, a.ctypes.data, a.nbytes))
+        return buf
+
+    def to_array(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes:
+            check(lib.photic_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib.photic_device_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+_OUT_FIELDS = {
+    # name: (dtype, rows)
+    "depth": (np.float32, 1), "rrs_error": (np.float32, 1), "model_error": (np.float32, 1),
+    "albedo": (np.float32, 1), "k_min": (np.float32, 1), "iod": (np.float32, 1),
+    "b_pct": (np.float32, 3), "bottom_type": (np.float32, 1), "x": (np.float64, None),
+    "status": (np.int32, 1), "n_evals": (np.int32, 1), "n_iter": (np.int32, 1),
+    "n_starts": (np.int32, 1),
+}
+
+
+class DeviceOutputs:
+    """Device output buffers for n pixels (all fields, or those named)."""
+
+    def __init__(self, cfg, n, fields=None):
+        self.n = n
+        self.np = n_params(cfg)
+        self.bufs = {}
+        self.struct = Outputs()
+        for name, (dt, rows) in _OUT_FIELDS.items():
+            if fields is not None and name not in fields:
+                continue
+            rows = self.np if rows is None else rows
+            b = DeviceBuffer(np.dtype(dt).itemsize * rows * n)
+            self.bufs[name] = (b, dt, rows)
+            setattr(self.struct, name, b.value)
+
+    def fetch(self):
+        out = {}
+        for name, (b, dt, rows) in self.bufs.items():
+            shape = (self.n,) if rows == 1 else (rows, self.n)
+            out[name] = b.to_array(dt, shape)
+        return out
+
+
+def invert_device(cfg, n, d_rrs, d_sec_sun=None, d_seed=None, outs=None, stream=None,
+                  d_sec_view=None):
+    """photic_invert on device pointers (ints or None); returns the DeviceOutputs."""
+    if outs is None:
+        outs = DeviceOutputs(cfg, n)
+    check(lib.photic_invert(C.byref(cfg), n, d_rrs, d_sec_sun, d_sec_view, d_seed,
+                            C.byref(outs.struct), stream))
+    return outs
+
+
+def invert(cfg, rrs, sec_sun=None, seed=None, fields=None):
+    """Invert host arrays: rrs [n_bands][n] float32, sec_sun [n], seed [n_params][n]."""
+    rrs = np.ascontiguousarray(rrs, dtype=np.float32)
+    n = rrs.shape[1]
+    d_rrs = DeviceBuffer.from_array(rrs)
+    d_ss = DeviceBuffer.from_array(np.asarray(sec_sun, np.float32)) if sec_sun is not None else None
+    d_seed = DeviceBuffer.from_array(np.asarray(seed, np.float32)) if seed is not None else None
+    outs = DeviceOutputs(cfg, n, fields)
+    invert_device(cfg, n, d_rrs.value, d_ss.value if d_ss else None,
+                  d_seed.value if d_seed else None, outs)
+    check(lib.photic_synchronize())
+    return outs.fetch()
+
+
+def gen_scene(cfg, seed, n, pix0=0, sigma=2e-4, deep_frac=0.0, truth=True):
+    """Synthetic scene on the GPU: returns (rrs [nb][n], sec_sun [n], truth [6][n] or None)."""
+    nb = cfg.n_bands
+    d_rrs = DeviceBuffer(4 * nb * n)
+    d_ss = DeviceBuffer(4 * n)
+    d_tr = DeviceBuffer(4 * 6 * n) if truth else None
+    check(lib.photic_gen_scene(C.byref(cfg), seed, pix0, n, sigma, deep_frac, d_rrs.value,
+                               d_ss.value, d_tr.value if d_tr else None, None))
+    check(lib.photic_synchronize())
+    return (d_rrs.to_array(np.float32, (nb, n)), d_ss.to_array(np.float32, (n,)),
+            d_tr.to_array(np.float32, (6, n)) if truth else None)
+
+
+def forward(cfg, params, sec_sun, Y, precision=FP64):
+    """Forward model on the GPU: params [n][5] (H, P, G, X, B) -> Rrs [n][n_bands] (float64)."""
+    params = np.ascontiguousarray(params, np.float64)
+    n = params.shape[0]
+    bufs = [DeviceBuffer.from_array(params), DeviceBuffer.from_array(np.asarray(sec_sun, np.float64)),
+            DeviceBuffer.from_array(np.asarray(Y, np.float64)), DeviceBuffer(8 * n * cfg.n_bands)]
+    check(lib.photic_forward(C.byref(cfg), n, bufs[0].value, bufs[1].value, bufs[2].value,
+                             bufs[3].value, precision, None))
+    check(lib.photic_synchronize())
+    return bufs[3].to_array(np.float64, (n, cfg.n_bands))
+
+
+def objective(cfg, x, rrs, sec_sun, precision=FP64):
+    """Misfit on the GPU: x [n][5] (MIX layout), rrs [n][n_bands] float32, sec_sun [n]."""
+    x = np.ascontiguousarray(x, np.float64)
+    n = x.shape[0]
+    bx = DeviceBuffer.from_array(x)
+    br = DeviceBuffer.from_array(np.ascontiguousarray(rrs, np.float32))
+    bs = DeviceBuffer.from_array(np.asarray(sec_sun, np.float32))
+    bo, be = DeviceBuffer(8 * n), DeviceBuffer(8 * n)
+    check(lib.photic_objective(C.byref(cfg), n, bx.value, br.value, bs.value, bo.value, be.value,
+                               precision, None))
+    check(lib.photic_synchronize())
+    return bo.to_array(np.float64, (n,)), be.to_array(np.float64, (n,))
+
+
+def kernel_info(cfg):
+    v, s, b, n = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    check(lib.photic_kernel_info(C.byref(cfg), C.byref(v), C.byref(s), C.byref(b), C.byref(n)))
+    return {"vgprs": v.value, "scratch_bytes": s.value, "blocks_per_cu": b.value, "n_cu": n.value}
+
+
+def const_bytes(cfg, n_dates=1, has_seed=False):
+    """The launch constants (PhConst) for cfg, as bytes -- used by the host test build."""
+    buf = C.create_string_buffer(lib.ph_const_size())
+    check(lib.ph_build_const(C.byref(cfg), n_dates, int(has_seed), buf))
+    return buf

@@ -1,0 +1,736 @@
+// photic_amd -- per-pixel semi-analytical inversion on gfx950.
+//
+// The lane engine: photic's forward model (model/samodel.c:2846-2949), its misfit
+// (model/samodel.c:2432-2759, one region) and its modified AS 47 Nelder-Mead
+// (model/asa047.c:10-502) driven by the multi-start of model/samodel.c:2119-2427, recast as a
+// per-lane state machine in which every pass of the kernel's loop makes exactly one
+// objective evaluation per lane.  The evaluation -- the expensive, branch-free part -- then
+// runs in lock-step across the 64 lanes of a wavefront, whatever step of the simplex each
+// lane is at; the short bookkeeping between evaluations is the only divergent code.
+//
+// Everything here is __host__ __device__ so the FP64 instantiation can also be run on the
+// CPU by the test suite (tests/hostsim), where it must agree bit for bit with the FP64
+// oracle; the library itself only ever launches it on the GPU.
+//
+// Arithmetic in the FP64 path follows the reference's order of operations; the file is
+// compiled with -ffp-contract=off so no multiply-add is fused behind our back.  The FP32
+// path fuses deliberately (explicit fmaf) and uses the hardware transcendentals.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ph_launch.h"
+
+// Every engine function is inlined into the kernel: the lane state must stay in registers,
+// and a real call would force it through scratch memory.
+#define PH_INLINE __host__ __device__ __attribute__((always_inline)) inline
+
+#include <utility>
+
+// Source-level unrolling.  Loops over the lane's register arrays are written as
+// ph_for<N>([&](auto i) {...}): every index is a constant from the first optimisation pass on,
+// so the arrays become registers before instruction combining could turn a
+// "c ? s.a[i] : s.b[i]" into a load through a selected pointer (which would pin the whole
+// lane state in scratch memory).
+template <typename F, int... I>
+PH_INLINE void ph_for_seq(F&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int M, typename F>
+PH_INLINE void ph_for(F&& f)
+{
+    ph_for_seq(f, std::make_integer_sequence<int, M>{});
+}
+#define PH_BODY __attribute__((always_inline))
+
+// Side quantities of one evaluation that the outputs need (samodel_optimise reads the
+// same fields of model_data after its final re-evaluation).
+template <int ND, int NB>
+struct PhSide {
+    double K[ND][NB];
+    double rrs_bot[ND][NB];
+    double rrs_mod[ND][NB];
+    double Rrs_error, K_error, model_error, albedo;
+};
+
+// Per-pixel constants held by a lane while it inverts the pixel.
+template <typename FT, int ND, int NB>
+struct PhPixel {
+    double rrs[ND][NB];       // measured Rrs, float input widened
+    FT pw[ND][NB];            // (440/lambda)^Y, Y from the measured 440/490 ratio
+    FT ss[ND], sv[ND];        // sec(sun), sec(view), subsurface
+    double tot;               // sum of measured Rrs in (date, band) order
+};
+
+// ---------------------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------------------
+
+// The reference's approx_equal(float, float, float) (model/common.c:392-394).
+PH_INLINE bool ph_approx_equal(float a, float b, float eps)
+{
+    float d = a - b;
+    double fa = fabs((double)a), fb = fabs((double)b);
+    return fabs((double)d) <= ((fa < fb ? fb : fa) * (double)eps);
+}
+
+// Rrs at one of the five reference wavelengths (k: 0 = 440, 1 = 490, 2 = 550, 3 = 640,
+// 4 = 750 nm) by the reference's linear interpolation / extrapolation (model/common.c:
+// 298-335), with the segment and weights resolved on the host.
+// c ? a : b on values.  With two lvalue operands C++'s conditional is itself an lvalue and
+// would be compiled as a select between addresses, which keeps the arrays involved out of
+// registers; taking the operands by value avoids that.
+PH_INLINE double ph_pick(bool c, double a, double b) { return c ? a : b; }
+
+PH_INLINE double ph_rrs_at_ptr(const PhConst& c, const double* r, int k)
+{
+    return r[c.ip_i0[k]] * c.ip_w0[k] + r[c.ip_i1[k]] * c.ip_w1[k];
+}
+
+template <int NB>
+PH_INLINE double ph_rrs_at(const PhConst& c, const double (&r)[NB], int k)
+{
+    double y0 = r[0], y1 = r[0];
+#pragma unroll
+    for (int b = 1; b < NB; b++) {
+        y0 = ph_pick(b == c.ip_i0[k], r[b], y0);
+        y1 = ph_pick(b == c.ip_i1[k], r[b], y1);
+    }
+    return y0 * c.ip_w0[k] + y1 * c.ip_w1[k];
+}
+
+// (long)(10 * v) as x86-64 evaluates it (cvttsd2si: NaN and out-of-range give INT64_MIN),
+// kept in a double -- exact for every value the misfit can take.
+PH_INLINE double ph_trunc10(double v)
+{
+    double t = 10.0 * v;
+    if (!(fabs(t) < 9.2233720368547758e18)) return -9.2233720368547758e18;
+    return trunc(t);
+}
+
+// Dynamic-index access into register arrays without scratch: unrolled selects.
+template <int M>
+PH_INLINE double ph_sel(const double (&a)[M], int j)
+{
+    double v = a[0];
+    ph_for<M>([&](auto k) PH_BODY { v = ph_pick(j == k, a[k], v); });
+    return v;
+}
+
+// ---------------------------------------------------------------------------------------
+// forward model, one date and band
+// ---------------------------------------------------------------------------------------
+
+// FP64: model/samodel.c:2886-2944 in the reference's operation order.
+PH_INLINE double ph_band64(double P, double G, double X, double rho, double H,
+                                            double a0, double a1, double aw, double bbw,
+                                            double eg, double pw, double ss, double sv,
+                                            double* K_out, double* rB_out, double* r_out)
+{
+    double a_phi = (a0 + a1 * log(fabs(P))) * fabs(P);
+    double a_g = fabs(G) * eg;
+    double a = aw + a_phi + a_g;
+    double b_p = X * pw;
+    double bb = bbw + b_p;
+    double u = bb / (a + bb);
+    double K = a + bb;
+    if (K < 0.0) K = 0.0;
+    if (K > 2.5) K = 2.5;
+    double rrs_dp = (0.084 + 0.170 * u) * u;
+    double DuC = 1.03 * sqrt(1.0 + 2.4 * u);
+    double DuB = 1.04 * sqrt(1.0 + 5.4 * u);
+    double M = ss + DuC * sv;
+    double rC = rrs_dp * (1.0 - exp(-M * K * H));
+    M = ss + DuB * sv;
+    double rB = rho / 3.141592653589793 * exp(-M * K * H);
+    double r = rC + rB;
+    *K_out = K;
+    *rB_out = rB;
+    *r_out = r;
+    return 0.5 * r / (1.0 - 1.5 * r) + 0.0;
+}
+
+// FP32: the same model with hardware transcendentals and fused multiply-adds.  ln P is
+// shared by the bands of a date and passed in.  Inputs are clamped to a range where no
+// intermediate overflows single precision.
+__device__ inline float ph_band32(float P, float lnP, float G, float X, float rho, float H,
+                                  float a0, float a1, float aw, float bbw, float eg, float pw,
+                                  float ss, float sv, float* K_out, float* rB_out, float* r_out)
+{
+    float a = fmaf(G, eg, fmaf(fmaf(a1, lnP, a0), P, aw));
+    float bb = fmaf(X, pw, bbw);
+    float apb = a + bb;
+    float u = bb * __builtin_amdgcn_rcpf(apb);
+    float K = fminf(fmaxf(apb, 0.0f), 2.5f);
+    float rrs_dp = fmaf(0.170f, u, 0.084f) * u;
+    float DuC = 1.03f * __builtin_amdgcn_sqrtf(fmaf(2.4f, u, 1.0f));
+    float DuB = 1.04f * __builtin_amdgcn_sqrtf(fmaf(5.4f, u, 1.0f));
+    float KH = K * H;
+    const float L2E = 1.4426950408889634f;
+    float eC = __builtin_amdgcn_exp2f(-fmaf(DuC, sv, ss) * KH * L2E);
+    float eB = __builtin_amdgcn_exp2f(-fmaf(DuB, sv, ss) * KH * L2E);
+    float r = fmaf(rrs_dp, 1.0f - eC, rho * 0.3183098861837907f * eB);
+    *K_out = K;
+    *rB_out = rho * 0.3183098861837907f * eB;
+    *r_out = r;
+    return 0.5f * r * __builtin_amdgcn_rcpf(fmaf(-1.5f, r, 1.0f));
+}
+
+// ---------------------------------------------------------------------------------------
+// objective: model/samodel.c:2432-2759 with one region
+// ---------------------------------------------------------------------------------------
+
+template <int NBOT>
+struct PhLayout {
+    static constexpr int nbot = NBOT == 0 ? 2 : NBOT;
+    static constexpr int off = NBOT == 0 ? 2 : 1 + 2 * NBOT;
+    template <int ND>
+    static constexpr int n() { return off + 3 * ND; }
+};
+
+template <typename FT, int ND, int NB, int NBOT, bool SIDE>
+PH_INLINE double ph_objective(const double* x, const PhConst& c,
+                                               const PhPixel<FT, ND, NB>& px,
+                                               PhSide<ND, NB>* side)
+{
+    using L = PhLayout<NBOT>;
+    constexpr int nbot = L::nbot;
+    double B[nbot], q[nbot], qsum = 0.0;
+    if (NBOT == 0) {
+        B[0] = 0.01 * fabs(100.0);
+        B[nbot - 1] = 0.01 * fabs(100.0);
+        q[0] = fabs(fabs(x[1]));
+        q[nbot - 1] = fabs(fabs(1.0 - x[1]));
+    } else {
+#pragma unroll
+        for (int k = 0; k < nbot; k++) {
+            B[k] = 0.01 * fabs(x[1 + k]);
+            q[k] = fabs(x[1 + nbot + k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < nbot; k++) qsum += q[k];
+#pragma unroll
+    for (int k = 0; k < nbot; k++) q[k] /= qsum;
+
+    // bottom reflectance per band: rho = sum_i (q_i B_i) R_i(lambda)
+    double qb[nbot];
+    double alb = 0.0;
+#pragma unroll
+    for (int k = 0; k < nbot; k++) {
+        qb[k] = q[k] * B[k];
+        alb += qb[k];
+    }
+    const double H = fabs(x[0]);
+    double err = 0.0;
+    double Kd[ND][NB];
+#pragma unroll
+    for (int s = 0; s < ND; s++) {
+        double P = 0.01 * fabs(x[L::off + 3 * s]);
+        double G = 0.01 * fabs(x[L::off + 3 * s + 1]);
+        double X = 0.01 * fabs(x[L::off + 3 * s + 2]);
+        if constexpr (sizeof(FT) == 8) {
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                double rho = 0.0;
+#pragma unroll
+                for (int k = 0; k < nbot; k++) {
+                    int t = NBOT == 0 ? k : c.bottom_idx[k];
+                    rho += qb[k] * c.bot[t][b];
+                }
+                double K, rB, r;
+                double m = ph_band64(P, G, X, rho, H, c.a0[b], c.a1[b], c.aw[b], c.bbw[b],
+                                     c.eg[b], px.pw[s][b], px.ss[s], px.sv[s], &K, &rB, &r);
+                double d = m - px.rrs[s][b];
+                err += d * d;
+                Kd[s][b] = K;
+                if (SIDE) {
+                    side->K[s][b] = K;
+                    side->rrs_bot[s][b] = rB;
+                    side->rrs_mod[s][b] = r;
+                }
+            }
+        } else {
+            const float HUGE_F = 1.0e30f;
+            float Pf = fminf(fmaxf((float)P, 1.0e-30f), HUGE_F);
+            float Gf = fminf((float)G, HUGE_F), Xf = fminf((float)X, HUGE_F);
+            float Hf = fminf((float)H, HUGE_F);
+            float lnP = __builtin_amdgcn_logf(Pf) * 0.6931471805599453f;
+            float qbf[nbot];
+#pragma unroll
+            for (int k = 0; k < nbot; k++) qbf[k] = (float)qb[k];
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                float rho = 0.0f;
+#pragma unroll
+                for (int k = 0; k < nbot; k++) {
+                    int t = NBOT == 0 ? k : c.bottom_idx[k];
+                    rho = fmaf(qbf[k], c.fbot[t][b], rho);
+                }
+                float K, rB, r;
+                float m = ph_band32(Pf, lnP, Gf, Xf, rho, Hf, c.fa0[b], c.fa1[b], c.faw[b],
+                                    c.fbbw[b], c.feg[b], px.pw[s][b], px.ss[s], px.sv[s], &K,
+                                    &rB, &r);
+                double d = (double)m - px.rrs[s][b];
+                err += d * d;
+                Kd[s][b] = (double)K;
+                if (SIDE) {
+                    side->K[s][b] = K;
+                    side->rrs_bot[s][b] = rB;
+                    side->rrs_mod[s][b] = r;
+                }
+            }
+        }
+    }
+    const double nt = (double)(ND * NB);
+    double mean = px.tot / nt;
+    double e_rrs = 100.0 * sqrt(err / nt) / mean;
+
+    // K penalty (model/samodel.c:2696-2732)
+    const double kmm = 0.275, kmn = 0.185;
+    double Kerr = 0.0, Kmin = 1.0e4;
+#pragma unroll
+    for (int s = 0; s < ND; s++) {
+        Kmin = 1.0e4;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            double k = Kd[s][b];
+            if (!ph_approx_equal((float)k, 0.0f, 1.0e-6f) && k < Kmin) Kmin = k;
+        }
+        double thr = -1.0;
+        if (H < 1.0 && Kmin < kmm) thr = 0.01 + kmn;
+        else if (H < 2.0 && Kmin < 0.5 * (1.5 * kmn + 0.5 * kmm)) thr = 0.01 + 0.5 * (1.5 * kmn + 0.5 * kmm);
+        else if (H < 3.0 && Kmin < 0.5 * (1.25 * kmn + 0.75 * kmm)) thr = 0.01 + 0.5 * (1.25 * kmn + 0.75 * kmm);
+        else if (H < 4.0 && Kmin < 0.5 * (1.5 * kmn + 0.5 * kmm)) thr = 0.01 + 0.5 * (1.5 * kmn + 0.5 * kmm);
+        else if (H < 5.0 && Kmin < 0.5 * (1.75 * kmn + 0.25 * kmm)) thr = 0.01 + 0.5 * (1.75 * kmn + 0.25 * kmm);
+        if (thr > 0.0) {
+            double t = 1.0 / (0.01 + Kmin) - 1.0 / thr;
+            Kerr += 100.0 * (t * t);
+        }
+    }
+    if (Kmin > 0.7) {
+        double t = 4.0 * (Kmin - 0.7);
+        Kerr += 100.0 * (t * t);
+    }
+    double e = 80.0 * e_rrs + 15.0 * Kerr;
+    e /= 120.0;
+    if (SIDE) {
+        side->Rrs_error = e_rrs;
+        side->K_error = Kerr;
+        side->model_error = e;
+        side->albedo = alb;
+    }
+    return e;
+}
+
+// ---------------------------------------------------------------------------------------
+// pixel set-up: model/samodel.c:1785-1799 (Rrs at 440/490/550/640 nm), :2898-2903 (Y)
+// ---------------------------------------------------------------------------------------
+
+// rrs_in[s][b]; returns false when the pixel is nodata on any band of any date
+// (model/samodel.c:933-947: within 1e-6 of the nodata value, or negative).
+template <typename FT, int ND, int NB>
+PH_INLINE bool ph_pixel_init(PhPixel<FT, ND, NB>& px, const PhConst& c,
+                                              const float (&rrs_in)[ND][NB], const double* ss,
+                                              const double* sv)
+{
+    bool ok = true;
+    double tot = 0.0;
+#pragma unroll
+    for (int s = 0; s < ND; s++) {
+        double r[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            float v = rrs_in[s][b];
+            if (ph_approx_equal(v, (float)c.nodata, 1.0e-6f) || v < 0.0f) ok = false;
+            r[b] = (double)v;
+            px.rrs[s][b] = r[b];
+            tot += r[b];
+        }
+        double r440 = ph_rrs_at(c, r, 0);
+        double r490 = ph_rrs_at(c, r, 1);
+        if (r440 < 0.0) r440 = 0.0001;
+        double chi = r440 / r490;
+        double Y = 3.44 * (1.0 - 3.17 * exp(-2.01 * chi));
+        if (Y < 0.0) Y = 0.0;
+        if (Y > 2.5) Y = 2.5;
+#pragma unroll
+        for (int b = 0; b < NB; b++) px.pw[s][b] = (FT)pow(440.0 / c.wl[b], Y);
+        px.ss[s] = (FT)ss[s];
+        px.sv[s] = (FT)sv[s];
+    }
+    px.tot = tot;
+    return ok;
+}
+
+// ---------------------------------------------------------------------------------------
+// the lane state machine
+// ---------------------------------------------------------------------------------------
+
+enum PhPhase {
+    PH_IDLE = 0,
+    PH_START,        // set up the next start (depth start, or the seed)      [no eval]
+    PH_BUILD,        // evaluating simplex vertex `sub` (sub == N: the start point)
+    PH_REFLECT,
+    PH_EXPAND,
+    PH_CONTRACT_IN,  // contraction on the worst vertex's side
+    PH_CONTRACT_OUT, // contraction on the reflection's side
+    PH_SHRINK,       // evaluating shrunk vertex `sub`
+    PH_FACT_PLUS,    // local-minimum test, coordinate `sub`, + side
+    PH_FACT_MINUS,
+    PH_FINAL,        // final re-evaluation at the best point
+    PH_DONE          // outputs ready                                          [no eval]
+};
+
+// The lane's large arrays -- simplex vertices, steps, best point -- live in LDS (on the GPU)
+// or in an ordinary array (host test build), interleaved across the lanes of a block so that
+// a wavefront's accesses to one element are contiguous and free of bank conflicts.  The
+// element of lane t at slot e is base[e * stride].
+template <int N>
+struct PhLaneMem {
+    static constexpr int n_slots = (N + 1) * N + 2 * N;
+    double* base;
+    int stride;
+    PH_INLINE double p(int k, int i) const { return base[(k * N + i) * stride]; }
+    PH_INLINE void set_p(int k, int i, double v) const { base[(k * N + i) * stride] = v; }
+    PH_INLINE double step(int i) const { return base[((N + 1) * N + i) * stride]; }
+    PH_INLINE void set_step(int i, double v) const { base[((N + 1) * N + i) * stride] = v; }
+    PH_INLINE double best(int i) const { return base[((N + 1) * N + N + i) * stride]; }
+    PH_INLINE void set_best(int i, double v) const { base[((N + 1) * N + N + i) * stride] = v; }
+};
+
+template <int N>
+struct PhNM {
+    PhLaneMem<N> m;
+    double y[N + 1];     // vertex values
+    double xs[N];        // start point while building; reflected point while iterating
+    double xb[N];        // centroid while iterating; probe point in the local-minimum test
+    double xe[N];        // point being evaluated
+    double ylo, ystar, ynewlo, del, lowest, yr;
+    int ilo, ihi, icount, jcount, numres, ifault, phase, sub, kstart, nstart;
+    int best_icount, best_conv, evals;
+};
+
+// Start point and steps of depth start k (k < 0: the seed), model/samodel.c:2246-2354.
+template <typename FT, int ND, int NB, int NBOT>
+PH_INLINE void ph_start_step(PhNM<PhLayout<NBOT>::template n<ND>()>& S, const PhConst& c,
+                             const PhPixel<FT, ND, NB>& px, int k, const double* seed)
+{
+    using L = PhLayout<NBOT>;
+    constexpr int N = L::template n<ND>();
+    if (k < 0) {
+        ph_for<N>([&](auto i) PH_BODY {
+            constexpr int cls = i == 0 ? 0 : (i < L::off ? 1 : 2 + (i - L::off) % 3);
+            double st = fabs(seed[i]) * c.seed_rel_step[cls];
+            if (st < c.seed_min_step[cls]) st = c.seed_min_step[cls];
+            S.xs[i] = seed[i];
+            S.m.set_step(i, st);
+        });
+        return;
+    }
+    double h0 = c.hstart[0];
+    ph_for<PH_MAXH - 1>([&](auto m0) PH_BODY {
+        constexpr int m = m0 + 1;
+        h0 = ph_pick(k == m, c.hstart[m], h0);
+    });
+    S.xs[0] = h0;
+    S.m.set_step(0, 1.25 * S.xs[0]);
+    if (NBOT == 0) {
+        S.xs[1] = 0.5;
+        S.m.set_step(1, 0.5);
+    } else {
+        double m490 = 0.0;
+        ph_for<ND>([&](auto s) PH_BODY { m490 += ph_rrs_at(c, px.rrs[s], 1); });
+        m490 /= (double)1 * ND;
+        ph_for<L::nbot>([&](auto b) PH_BODY {
+            S.xs[1 + b] = 100.0 * 4.0 * m490;
+            S.m.set_step(1 + b, 1.5 * S.xs[1 + b]);
+        });
+        ph_for<L::nbot>([&](auto b) PH_BODY {
+            S.xs[1 + L::nbot + b] = 1.0;
+            S.m.set_step(1 + L::nbot + b, 0.5 * S.xs[1 + L::nbot + b]);
+        });
+    }
+    ph_for<ND>([&](auto s) PH_BODY {
+        constexpr int o = L::off + 3 * s;
+        double m490 = ph_rrs_at(c, px.rrs[s], 1) / (double)1;
+        double m550 = ph_rrs_at(c, px.rrs[s], 2) / (double)1;
+        double m640 = ph_rrs_at(c, px.rrs[s], 3) / (double)1;
+        S.xs[o] = 100.0 * 0.072 * pow(m490 / m550, -1.7);
+        S.xs[o + 1] = 1.5 * S.xs[o];
+        S.xs[o + 2] = 100.0 * 30.0 * c.aw640 * m640;
+        S.m.set_step(o, 2.0 * S.xs[o]);
+        S.m.set_step(o + 1, 2.0 * S.xs[o + 1]);
+        S.m.set_step(o + 2, 2.0 * S.xs[o + 2]);
+    });
+}
+
+// Begin a Nelder-Mead run from S.xs and the steps (asa047.c:162-170): the first point
+// evaluated is the start itself, stored as vertex N.
+template <int N>
+PH_INLINE void ph_nm_begin(PhNM<N>& S, const PhConst& c)
+{
+    S.icount = 0;
+    S.numres = 0;
+    S.jcount = c.konvge;
+    S.del = 1.0;
+    S.phase = PH_BUILD;
+    S.sub = N;
+    ph_for<N>([&](auto i) PH_BODY { S.xe[i] = S.xs[i]; });
+}
+
+// One step of the lane state machine: consume the value f of the point S.xe evaluated in
+// phase S.phase, and advance to the next point to evaluate (left in S.xe), or to PH_START
+// (the kernel then sets up the next start) or PH_DONE (outputs ready).
+//
+// Written as classify -> at most one simplex-vertex write -> shared follow-up, so that lanes
+// at different steps of the simplex run the same instructions as far as possible.
+template <int N>
+PH_INLINE void ph_nm_consume(PhNM<N>& S, const PhConst& c, double f)
+{
+    const double dn = (double)N, dnn = (double)(N + 1);
+    const double eps = 1.0e-6;
+    enum {
+        A_NONE,
+        A_BUILD_NEXT,  // evaluate simplex vertex S.sub
+        A_AFTER_BUILD, // simplex complete (or shrunk): find the best vertex, then iterate
+        A_MOVE,        // evaluate xb + coef * (src - xb): expansion / contractions
+        A_SHRINK_NEXT, // evaluate shrunk vertex S.sub
+        A_ITER_END,
+        A_FACT_FAIL,
+        A_FACT_MINUS,
+        A_FACT_NEXT,
+        A_RUN_DONE
+    };
+    S.evals++;
+    const int ph = S.phase;
+    if (ph == PH_FINAL) {
+        S.phase = PH_DONE;
+        return;
+    }
+    if (ph < PH_BUILD || ph > PH_FACT_MINUS) return;
+    S.icount++;
+
+    // ---- classify (asa047.c:176-393, 455-479) ----
+    bool wr = false, wr_xs = false;   // vertex write: vertex wj = wr_xs ? xs : xe, value wf
+    int wj = 0;
+    double wf = f;
+    int act = A_NONE;
+    bool mv_phi = false;              // A_MOVE: src is vertex ihi (else xs)
+    double mv_coef = 0.5;
+    int mv_phase = PH_EXPAND;
+    if (ph == PH_BUILD) {
+        wr = true;
+        wj = S.sub;
+        const int next = (S.sub == N) ? 0 : S.sub + 1;
+        S.sub = next;
+        act = (next < N) ? A_BUILD_NEXT : A_AFTER_BUILD;
+    } else if (ph == PH_REFLECT) {
+        S.ystar = f;
+        if (f < S.ylo) {
+            act = A_MOVE; mv_coef = 2.0; mv_phase = PH_EXPAND;
+        } else {
+            int l = 0;
+            ph_for<N + 1>([&](auto i) PH_BODY { l += (f < S.y[i]) ? 1 : 0; });
+            if (l > 1) {
+                wr = true; wj = S.ihi; wr_xs = true; wf = f;
+                act = A_ITER_END;
+            } else if (l == 0) {
+                act = A_MOVE; mv_phi = true; mv_phase = PH_CONTRACT_IN;
+            } else {
+                act = A_MOVE; mv_phase = PH_CONTRACT_OUT;
+            }
+        }
+    } else if (ph == PH_EXPAND) {
+        wr = true; wj = S.ihi;
+        wr_xs = S.ystar < f;
+        wf = ph_pick(wr_xs, S.ystar, f);
+        act = A_ITER_END;
+    } else if (ph == PH_CONTRACT_IN) {
+        if (ph_sel(S.y, S.ihi) < f) {
+            S.sub = 0;
+            act = A_SHRINK_NEXT;
+        } else {
+            wr = true; wj = S.ihi;
+            act = A_ITER_END;
+        }
+    } else if (ph == PH_CONTRACT_OUT) {
+        wr = true; wj = S.ihi;
+        wr_xs = !(f <= S.ystar);
+        wf = ph_pick(wr_xs, S.ystar, f);
+        act = A_ITER_END;
+    } else if (ph == PH_SHRINK) {
+        wr = true; wj = S.sub;
+        S.sub = S.sub + 1;
+        act = (S.sub <= N) ? A_SHRINK_NEXT : A_AFTER_BUILD;
+    } else if (ph == PH_FACT_PLUS) {
+        act = (ph_trunc10(f) < S.yr) ? A_FACT_FAIL : A_FACT_MINUS;
+    } else { // PH_FACT_MINUS
+        act = (ph_trunc10(f) < S.yr) ? A_FACT_FAIL : A_FACT_NEXT;
+    }
+
+    // ---- the vertex write ----
+    if (wr) {
+        ph_for<N>([&](auto i) PH_BODY { S.m.set_p(wj, i, ph_pick(wr_xs, S.xs[i], S.xe[i])); });
+        ph_for<N + 1>([&](auto k) PH_BODY { S.y[k] = ph_pick(wj == (int)k, wf, S.y[k]); });
+    }
+
+    // ---- follow-up ----
+    if (act == A_BUILD_NEXT) {
+        const int j = S.sub;
+        ph_for<N>([&](auto i) PH_BODY {
+            S.xe[i] = (j == (int)i) ? S.xs[i] + S.m.step(i) * S.del : +S.xs[i];
+        });
+        S.phase = PH_BUILD;
+        return;
+    }
+    if (act == A_MOVE) {
+        ph_for<N>([&](auto i) PH_BODY {
+            const double v = mv_phi ? S.m.p(S.ihi, i) : +S.xs[i];
+            S.xe[i] = S.xb[i] + mv_coef * (v - S.xb[i]);
+        });
+        S.phase = mv_phase;
+        return;
+    }
+    if (act == A_SHRINK_NEXT) {
+        const int j = S.sub, lo = S.ilo;
+        ph_for<N>([&](auto i) PH_BODY { S.xe[i] = (S.m.p(j, i) + S.m.p(lo, i)) * 0.5; });
+        S.phase = PH_SHRINK;
+        return;
+    }
+    if (act == A_FACT_FAIL) {
+        // asa047.c:488-493: restart from the probe point with a simplex of relative size
+        // eps; jcount and icount carry over
+        S.ifault = 2;
+        ph_for<N>([&](auto i) PH_BODY {
+            S.xs[i] = S.xb[i];
+            S.xe[i] = S.xs[i];
+        });
+        S.del = eps;
+        S.numres++;
+        S.phase = PH_BUILD;
+        S.sub = N;
+        return;
+    }
+    if (act == A_FACT_MINUS || act == A_FACT_NEXT) {
+        const int i0 = S.sub;
+        const bool minus = (act == A_FACT_MINUS);
+        const int i1 = minus ? -1 : i0 + 1;
+        ph_for<N>([&](auto i) PH_BODY {
+            const double d = S.m.step(i) * eps;
+            double v = S.xb[i];
+            if (i0 == (int)i) v = minus ? v - d - d : v + d;
+            if (i1 == (int)i) v = v + d;
+            S.xb[i] = v;
+            S.xe[i] = v;
+        });
+        if (minus) {
+            S.phase = PH_FACT_MINUS;
+            return;
+        }
+        if (i1 < N) {
+            S.sub = i1;
+            S.phase = PH_FACT_PLUS;
+            return;
+        }
+        S.ifault = 0;
+        act = A_RUN_DONE;
+    }
+
+    bool iter_begin = false, nm_finish = false;
+    if (act == A_AFTER_BUILD) {
+        S.ylo = S.y[0];
+        S.ilo = 0;
+        ph_for<N>([&](auto i0) PH_BODY {
+            constexpr int i = i0 + 1;
+            if (S.y[i] < S.ylo) { S.ylo = S.y[i]; S.ilo = i; }
+        });
+        iter_begin = true;
+    }
+    // ---- end of an iteration: asa047.c:397-434 ----
+    if (act == A_ITER_END) {
+        const double yhi = ph_sel(S.y, S.ihi);
+        if (yhi < S.ylo) { S.ylo = yhi; S.ilo = S.ihi; }
+        S.jcount--;
+        if (S.jcount > 0) {
+            iter_begin = true;
+        } else if (S.icount <= c.kcount) {
+            S.jcount = c.konvge;
+            double z = 0.0;
+            ph_for<N + 1>([&](auto i) PH_BODY { z = z + S.y[i]; });
+            const double xm = z / dnn;
+            z = 0.0;
+            ph_for<N + 1>([&](auto i) PH_BODY { z = z + (S.y[i] - xm) * (S.y[i] - xm); });
+            if (z <= c.rq) nm_finish = true;
+            else iter_begin = true;
+        } else {
+            iter_begin = true;
+        }
+    }
+    // ---- start of an iteration: asa047.c:215-253 ----
+    if (iter_begin) {
+        if (c.kcount <= S.icount) {
+            nm_finish = true;
+        } else {
+            S.ynewlo = S.y[0];
+            S.ihi = 0;
+            ph_for<N>([&](auto i0) PH_BODY {
+                constexpr int i = i0 + 1;
+                if (S.ynewlo < S.y[i]) { S.ynewlo = S.y[i]; S.ihi = i; }
+            });
+            const int hi = S.ihi;
+            ph_for<N>([&](auto i) PH_BODY {
+                double z = 0.0;
+                ph_for<N + 1>([&](auto k) PH_BODY { z = z + S.m.p(k, i); });
+                const double phi = S.m.p(hi, i);
+                z = z - phi;
+                S.xb[i] = z / dn;
+                S.xs[i] = S.xb[i] + 1.0 * (S.xb[i] - phi);
+                S.xe[i] = S.xs[i];
+            });
+            S.phase = PH_REFLECT;
+            return;
+        }
+    }
+    // ---- convergence or evaluations exhausted: asa047.c:440-454 ----
+    if (nm_finish) {
+        const int lo = S.ilo;
+        ph_for<N>([&](auto i) PH_BODY { S.xb[i] = S.m.p(lo, i); });
+        S.ynewlo = ph_sel(S.y, lo);
+        S.yr = ph_trunc10(S.ynewlo);
+        if (c.kcount < S.icount) {
+            S.ifault = 2;
+            act = A_RUN_DONE;
+        } else {
+            S.ifault = 0;
+            S.sub = 0;
+            ph_for<N>([&](auto i) PH_BODY {
+                if (i == 0) S.xb[i] = S.xb[i] + S.m.step(i) * eps;
+                S.xe[i] = S.xb[i];
+            });
+            S.phase = PH_FACT_PLUS;
+            return;
+        }
+    }
+    // ---- a Nelder-Mead run ended; its result is (xb, ynewlo, icount, ifault) ----
+    // multi-start bookkeeping, model/samodel.c:2385-2405
+    if (act == A_RUN_DONE) {
+        S.nstart++;
+        if (S.ynewlo < S.lowest) {
+            S.lowest = S.ynewlo;
+            ph_for<N>([&](auto i) PH_BODY { S.m.set_best(i, S.xb[i]); });
+            S.best_icount = S.icount;
+            S.best_conv = (S.ifault == 0) ? 1 : 0;
+            if (S.lowest < c.early_exit) S.kstart = c.n_hstarts;
+        }
+        S.kstart++;
+        if (S.kstart < c.n_hstarts) {
+            S.phase = PH_START;
+            return;
+        }
+        ph_for<N>([&](auto i) PH_BODY { S.xe[i] = S.m.best(i); });
+        S.phase = PH_FINAL;
+    }
+}

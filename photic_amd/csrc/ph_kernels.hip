@@ -1,0 +1,462 @@
+// photic_amd -- gfx950 kernels and their C-ABI launchers.
+//
+//   ph_invert_kernel   persistent work-queue inversion: one pixel per lane, the lane engine
+//                      of ph_engine.h, wave-level refill from a global chunk counter
+//   ph_fwd_kernel      forward model only (parity of the FP32/FP64 forward model)
+//   ph_objective_kernel misfit of given parameter vectors (parity of the objective)
+//   ph_gen_kernel      Philox synthetic scene
+//
+// Launchers return hipError_t as int; photic_capi.c turns them into the public API.
+#include <hip/hip_runtime.h>
+
+#include "ph_engine.h"
+#include "ph_gen.h"
+#include "ph_launch.h"
+
+// ---------------------------------------------------------------------------------------
+// inversion kernel
+// ---------------------------------------------------------------------------------------
+
+template <typename FT, int NB, int NBOT>
+__device__ __attribute__((always_inline)) inline void ph_write_outputs(const PhInvertArgs& a, const PhConst& c, long pix,
+                                        const PhNM<PhLayout<NBOT>::template n<1>()>& S,
+                                        const PhSide<1, NB>& side)
+{
+    using L = PhLayout<NBOT>;
+    constexpr int N = L::template n<1>();
+    double x[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = fabs(S.m.best(i));
+    if (a.x_out)
+#pragma unroll
+        for (int i = 0; i < N; i++) a.x_out[(long)i * a.n_pix + pix] = x[i];
+    // one region: origin weight sqrt(1), depth = H (model/samodel.c:2000-2017)
+    if (a.depth) a.depth[pix] = (float)((0.0 + 1.0 * x[0]) / (1.0 + 1.0 - 1.0));
+    if (a.rrs_error) a.rrs_error[pix] = (float)side.Rrs_error;
+    if (a.model_error) a.model_error[pix] = (float)side.model_error;
+    if (a.albedo) a.albedo[pix] = (float)side.albedo;
+    if (a.k_min) {
+        double kmin = 1.0e10;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            double k = side.K[0][b];
+            if (!ph_approx_equal((float)k, 0.0f, 1.0e-4f) && k < kmin) kmin = k;
+        }
+        a.k_min[pix] = (float)(kmin == 1.0e10 ? 0.0 : kmin);
+    }
+    if (a.iod) {
+        double iod = 0.0, nobs = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            iod += side.rrs_bot[0][b] / side.rrs_mod[0][b];
+            nobs += 1.0;
+        }
+        a.iod[pix] = (float)(100.0 * iod / nobs);
+    }
+    if (a.b_pct || a.bottom_type) {
+        constexpr int nb = L::nbot;
+        double qv[nb], qsum = 0.0, pct[PH_NBOT];
+        if (NBOT == 0) {
+            qv[0] = fabs(S.m.best(1));
+            qv[nb - 1] = fabs(1.0 - S.m.best(1));
+        } else {
+#pragma unroll
+            for (int k = 0; k < nb; k++) qv[k] = x[1 + nb + k];
+        }
+#pragma unroll
+        for (int k = 0; k < nb; k++) qsum += qv[k];
+#pragma unroll
+        for (int k = 0; k < PH_NBOT; k++) pct[k] = 0.0;
+        double largest = 0.0, btype = 0.0;
+#pragma unroll
+        for (int k = 0; k < nb; k++) {
+            int t = NBOT == 0 ? k : c.bottom_idx[k];
+            double v = 100.0 * qv[k] / qsum;
+#pragma unroll
+            for (int m = 0; m < PH_NBOT; m++) pct[m] = ph_pick(m == t, v, pct[m]);
+        }
+#pragma unroll
+        for (int k = 0; k < nb; k++) {
+            int t = NBOT == 0 ? k : c.bottom_idx[k];
+            double v = 0.0;
+#pragma unroll
+            for (int m = 0; m < PH_NBOT; m++) v = ph_pick(m == t, pct[m], v);
+            if (v > largest) {
+                largest = v;
+                btype = 1.0 + t;
+            }
+        }
+        if (a.b_pct)
+#pragma unroll
+            for (int m = 0; m < 3; m++) a.b_pct[(long)m * a.n_pix + pix] = (float)pct[m];
+        if (a.bottom_type) a.bottom_type[pix] = (float)btype;
+    }
+    if (a.status) a.status[pix] = S.best_conv ? 0 : 2;
+    if (a.n_evals) a.n_evals[pix] = S.evals;
+    if (a.n_iter) a.n_iter[pix] = S.best_icount;
+    if (a.n_starts) a.n_starts[pix] = S.nstart;
+}
+
+__device__ inline void ph_write_nodata(const PhInvertArgs& a, long pix, int n_params)
+{
+    // the reference's defaults for pixels it does not model (model/samodel.c:819-841)
+    if (a.depth) a.depth[pix] = 0.0f;
+    if (a.rrs_error) a.rrs_error[pix] = 0.0f;
+    if (a.model_error) a.model_error[pix] = 0.0f;
+    if (a.albedo) a.albedo[pix] = 0.0f;
+    if (a.k_min) a.k_min[pix] = 0.0f;
+    if (a.iod) a.iod[pix] = 0.0f;
+    if (a.b_pct)
+        for (int m = 0; m < 3; m++) a.b_pct[(long)m * a.n_pix + pix] = -9999.0f;
+    if (a.bottom_type) a.bottom_type[pix] = -9999.0f;
+    if (a.x_out)
+        for (int i = 0; i < n_params; i++) a.x_out[(long)i * a.n_pix + pix] = 0.0;
+    if (a.status) a.status[pix] = -1;
+    if (a.n_evals) a.n_evals[pix] = 0;
+    if (a.n_iter) a.n_iter[pix] = 0;
+    if (a.n_starts) a.n_starts[pix] = 0;
+}
+
+template <typename FT, int NB, int NBOT>
+__global__ void __launch_bounds__(256) ph_invert_kernel(PhInvertArgs a)
+{
+    using L = PhLayout<NBOT>;
+    constexpr int N = L::template n<1>();
+    const PhConst& c = a.c;
+    const int lane = (int)__lane_id();
+
+    extern __shared__ double ph_lds[];
+    PhNM<N> S;
+    S.m.base = ph_lds + threadIdx.x;
+    S.m.stride = (int)blockDim.x;
+    PhPixel<FT, 1, NB> px;
+#pragma unroll
+    for (int k = 0; k <= N; k++) S.y[k] = 0.0;
+    S.phase = PH_IDLE;
+    long pix = -1;
+
+    // wave-local slice of the global work queue: [q_next, q_end)
+    long q_next = 0, q_end = 0;
+    bool exhausted = false;
+    const unsigned long long t0 = wall_clock64();
+
+    for (;;) {
+        // watchdog: a wave still busy after max_ticks stops and raises the flag
+        if (wall_clock64() - t0 > a.max_ticks) {
+            if (lane == 0) atomicOr(a.queue + 1, 1ull);
+            break;
+        }
+        // ---- refill idle lanes ----
+        unsigned long long idle = __ballot(S.phase == PH_IDLE);
+        if (idle != 0ull && !exhausted) {
+            if (q_next >= q_end) {
+                unsigned long long base = 0;
+                if (lane == __ffsll((long long)idle) - 1)
+                    base = atomicAdd(a.queue, (unsigned long long)a.chunk);
+                base = __shfl(base, __ffsll((long long)idle) - 1);
+                if ((long)base >= a.n_pix) {
+                    exhausted = true;
+                } else {
+                    q_next = (long)base;
+                    q_end = q_next + a.chunk;
+                    if (q_end > a.n_pix) q_end = a.n_pix;
+                }
+            }
+            if (!exhausted || q_next < q_end) {
+                int rank = __popcll(idle & ((1ull << lane) - 1ull));
+                long avail = q_end - q_next;
+                if (S.phase == PH_IDLE && rank < avail) {
+                    pix = q_next + rank;
+                    float r[1][NB];
+#pragma unroll
+                    for (int b = 0; b < NB; b++) r[0][b] = a.rrs[(long)b * a.n_pix + pix];
+                    double ss = a.sec_sun ? (double)a.sec_sun[pix] : c.sec_sun0;
+                    double sv = a.sec_view ? (double)a.sec_view[pix] : c.sec_view0;
+                    bool ok = ph_pixel_init<FT, 1, NB>(px, c, r, &ss, &sv);
+                    if (ok) {
+                        S.kstart = c.has_seed ? -1 : 0;
+                        S.lowest = 1.0e4;
+                        S.nstart = 0;
+                        S.evals = 0;
+                        S.best_icount = 0;
+                        S.best_conv = 0;
+#pragma unroll
+                        for (int i = 0; i < N; i++) S.m.set_best(i, 0.0);
+                        S.phase = PH_START;
+                    } else {
+                        ph_write_nodata(a, pix, N);
+                    }
+                }
+                int taken = __popcll(idle);
+                q_next += (taken < avail) ? taken : avail;
+            }
+        }
+        if (__ballot(S.phase != PH_IDLE) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+
+        // ---- set up a new start ----
+        if (S.phase == PH_START) {
+            double sd[N];
+            if (S.kstart < 0) {
+#pragma unroll
+                for (int i = 0; i < N; i++) sd[i] = (double)a.seed[(long)i * a.n_pix + pix];
+            }
+            ph_start_step<FT, 1, NB, NBOT>(S, c, px, S.kstart, sd);
+            ph_nm_begin(S, c);
+        }
+
+        // ---- one evaluation per lane, in lock-step ----
+        // A pass in which some lane makes its final re-evaluation also keeps the side
+        // quantities the outputs need; every other pass evaluates the misfit alone.
+        if (__ballot(S.phase == PH_FINAL) != 0ull) {
+            PhSide<1, NB> side;
+            double f = 0.0;
+            if (S.phase != PH_IDLE) f = ph_objective<FT, 1, NB, NBOT, true>(S.xe, c, px, &side);
+            if (S.phase != PH_IDLE) {
+                ph_nm_consume(S, c, f);
+                if (S.phase == PH_DONE) {
+                    ph_write_outputs<FT, NB, NBOT>(a, c, pix, S, side);
+                    S.phase = PH_IDLE;
+                }
+            }
+        } else {
+            double f = 0.0;
+            if (S.phase != PH_IDLE)
+                f = ph_objective<FT, 1, NB, NBOT, false>(S.xe, c, px, nullptr);
+            if (S.phase != PH_IDLE) ph_nm_consume(S, c, f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// forward model / objective kernels (parity tests)
+// ---------------------------------------------------------------------------------------
+
+// params [n][5] = (H, P, G, X, B) per sample; sec_sun [n]; Y [n]; out Rrs [n][NB].
+template <typename FT>
+__global__ void ph_fwd_kernel(const PhConst c, long n, const double* params,
+                              const double* sec_sun, const double* Yv, double* out)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double H = params[i * 5 + 0], P = params[i * 5 + 1], G = params[i * 5 + 2];
+    double X = params[i * 5 + 3], B = params[i * 5 + 4];
+    double ss = sec_sun[i], Y = Yv[i];
+    for (int b = 0; b < c.n_bands; b++) {
+        double rho = B * c.bot[0][b] + (1.0 - B) * c.bot[1][b];
+        double pw = pow(440.0 / c.wl[b], Y);
+        if constexpr (sizeof(FT) == 8) {
+            double K, rB, r;
+            out[i * c.n_bands + b] = ph_band64(P, G, X, rho, H, c.a0[b], c.a1[b], c.aw[b],
+                                               c.bbw[b], c.eg[b], pw, ss, 1.0, &K, &rB, &r);
+        } else {
+            float K, rB, r;
+            float Pf = fminf(fmaxf((float)P, 1.0e-30f), 1.0e30f);
+            float lnP = __builtin_amdgcn_logf(Pf) * 0.6931471805599453f;
+            out[i * c.n_bands + b] = (double)ph_band32(
+                Pf, lnP, (float)G, (float)X, (float)rho, (float)H, c.fa0[b], c.fa1[b], c.faw[b],
+                c.fbbw[b], c.feg[b], (float)pw, (float)ss, 1.0f, &K, &rB, &r);
+        }
+    }
+}
+
+// x [n][N] parameter vectors (mix layout); rrs [n][NB] float measured; sec_sun [n].
+template <typename FT, int NB>
+__global__ void ph_objective_kernel(const PhConst c, long n, const double* x,
+                                    const float* rrs, const float* sec_sun, double* out,
+                                    double* rrs_error)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PhPixel<FT, 1, NB> px;
+    float r[1][NB];
+    for (int b = 0; b < NB; b++) r[0][b] = rrs[i * NB + b];
+    double ss = sec_sun[i], sv = 1.0;
+    ph_pixel_init<FT, 1, NB>(px, c, r, &ss, &sv);
+    PhSide<1, NB> side;
+    double xv[5];
+    for (int k = 0; k < 5; k++) xv[k] = x[i * 5 + k];
+    out[i] = ph_objective<FT, 1, NB, 0, true>(xv, c, px, &side);
+    if (rrs_error) rrs_error[i] = side.Rrs_error;
+}
+
+// ---------------------------------------------------------------------------------------
+// synthetic scene
+// ---------------------------------------------------------------------------------------
+
+__global__ void ph_gen_kernel(const PhConst c, uint64_t seed, long pix0, long n,
+                              double sigma, double deep_frac, float* rrs /*[NB][n]*/,
+                              float* sec_sun, float* truth /*[6][n] or null*/)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PhTruth t;
+    float r[PH_MAXB];
+    ph_gen_pixel(seed, (uint64_t)(pix0 + i), c, c.n_bands, sigma, deep_frac, &t, r);
+    for (int b = 0; b < c.n_bands; b++) rrs[(long)b * n + i] = r[b];
+    double th = t.theta;
+    th *= 3.141592653589793 / 180.0;
+    sec_sun[i] = (float)(1.0 / cos(th));
+    if (truth) {
+        truth[0 * n + i] = (float)t.H;
+        truth[1 * n + i] = (float)t.P;
+        truth[2 * n + i] = (float)t.G;
+        truth[3 * n + i] = (float)t.X;
+        truth[4 * n + i] = (float)t.B;
+        truth[5 * n + i] = (float)t.theta;
+    }
+}
+
+__global__ void ph_zero_u64_kernel(unsigned long long* p, int n)
+{
+    int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------
+
+extern "C" int ph_launch_zero_u64(unsigned long long* p, int n, void* stream)
+{
+    hipLaunchKernelGGL(ph_zero_u64_kernel, dim3((n + 63) / 64), dim3(64), 0,
+                       (hipStream_t)stream, p, n);
+    return (int)hipGetLastError();
+}
+
+template <typename FT, int NB, int NBOT>
+static int ph_attrs_t(int* vgprs, int* sgprs, int* scratch, int* maxt)
+{
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, (const void*)ph_invert_kernel<FT, NB, NBOT>);
+    if (e != hipSuccess) return (int)e;
+    *vgprs = 0;
+    *sgprs = 0;
+    *scratch = (int)fa.localSizeBytes;
+    *maxt = fa.maxThreadsPerBlock;
+    *vgprs = fa.numRegs;
+    return 0;
+}
+
+template <typename FT, int NB, int NBOT>
+static int ph_occ_t(int block, int* bpc)
+{
+    constexpr int N = PhLayout<NBOT>::template n<1>();
+    size_t lds = sizeof(double) * (size_t)PhLaneMem<N>::n_slots * (size_t)block;
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        bpc, (const void*)ph_invert_kernel<FT, NB, NBOT>, block, lds);
+}
+
+extern "C" int ph_lds_bytes(int n_bottoms, int block)
+{
+    switch (n_bottoms) {
+    case 0: return (int)(sizeof(double) * PhLaneMem<PhLayout<0>::n<1>()>::n_slots * block);
+    case 1: return (int)(sizeof(double) * PhLaneMem<PhLayout<1>::n<1>()>::n_slots * block);
+    case 2: return (int)(sizeof(double) * PhLaneMem<PhLayout<2>::n<1>()>::n_slots * block);
+    case 3: return (int)(sizeof(double) * PhLaneMem<PhLayout<3>::n<1>()>::n_slots * block);
+    default: return -1;
+    }
+}
+
+extern "C" int ph_kernel_occupancy(int fp32, int n_bands, int n_bottoms, int block, int* bpc)
+{
+    if (n_bands != 4) return PH_LAUNCH_UNSUPPORTED;
+    if (fp32) return n_bottoms == 0 ? ph_occ_t<float, 4, 0>(block, bpc) : PH_LAUNCH_UNSUPPORTED;
+    switch (n_bottoms) {
+    case 0: return ph_occ_t<double, 4, 0>(block, bpc);
+    case 1: return ph_occ_t<double, 4, 1>(block, bpc);
+    case 2: return ph_occ_t<double, 4, 2>(block, bpc);
+    case 3: return ph_occ_t<double, 4, 3>(block, bpc);
+    default: return PH_LAUNCH_UNSUPPORTED;
+    }
+}
+
+extern "C" int ph_kernel_attrs(int fp32, int n_bands, int n_bottoms, int* vgprs, int* sgprs,
+                               int* scratch, int* maxt)
+{
+    if (n_bands != 4) return PH_LAUNCH_UNSUPPORTED;
+    if (fp32) return n_bottoms == 0 ? ph_attrs_t<float, 4, 0>(vgprs, sgprs, scratch, maxt)
+                                    : PH_LAUNCH_UNSUPPORTED;
+    switch (n_bottoms) {
+    case 0: return ph_attrs_t<double, 4, 0>(vgprs, sgprs, scratch, maxt);
+    case 1: return ph_attrs_t<double, 4, 1>(vgprs, sgprs, scratch, maxt);
+    case 2: return ph_attrs_t<double, 4, 2>(vgprs, sgprs, scratch, maxt);
+    case 3: return ph_attrs_t<double, 4, 3>(vgprs, sgprs, scratch, maxt);
+    default: return PH_LAUNCH_UNSUPPORTED;
+    }
+}
+
+template <typename FT, int NB, int NBOT>
+static int ph_launch_invert_t(const PhInvertArgs& a, int grid, int block, hipStream_t st)
+{
+    constexpr int N = PhLayout<NBOT>::template n<1>();
+    size_t lds = sizeof(double) * (size_t)PhLaneMem<N>::n_slots * (size_t)block;
+    hipError_t e = hipFuncSetAttribute((const void*)ph_invert_kernel<FT, NB, NBOT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL((ph_invert_kernel<FT, NB, NBOT>), dim3(grid), dim3(block), lds, st, a);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ph_launch_invert(const PhInvertArgs* a, int fp32, int n_bands, int n_bottoms,
+                                int grid, int block, void* stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (n_bands != 4) return PH_LAUNCH_UNSUPPORTED;
+    if (fp32) {
+        if (n_bottoms == 0) return ph_launch_invert_t<float, 4, 0>(*a, grid, block, st);
+        return PH_LAUNCH_UNSUPPORTED;
+    }
+    switch (n_bottoms) {
+    case 0: return ph_launch_invert_t<double, 4, 0>(*a, grid, block, st);
+    case 1: return ph_launch_invert_t<double, 4, 1>(*a, grid, block, st);
+    case 2: return ph_launch_invert_t<double, 4, 2>(*a, grid, block, st);
+    case 3: return ph_launch_invert_t<double, 4, 3>(*a, grid, block, st);
+    default: return PH_LAUNCH_UNSUPPORTED;
+    }
+}
+
+extern "C" int ph_launch_fwd(const PhConst* c, long n, const double* params, const double* sec_sun,
+                             const double* Y, double* out, int fp32, void* stream)
+{
+    int blocks = (int)((n + 255) / 256);
+    if (blocks == 0) return 0;
+    if (fp32)
+        hipLaunchKernelGGL(ph_fwd_kernel<float>, dim3(blocks), dim3(256), 0,
+                           (hipStream_t)stream, *c, n, params, sec_sun, Y, out);
+    else
+        hipLaunchKernelGGL(ph_fwd_kernel<double>, dim3(blocks), dim3(256), 0,
+                           (hipStream_t)stream, *c, n, params, sec_sun, Y, out);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ph_launch_objective(const PhConst* c, long n, const double* x, const float* rrs,
+                                   const float* sec_sun, double* out, double* rrs_error,
+                                   int fp32, int n_bands, void* stream)
+{
+    int blocks = (int)((n + 255) / 256);
+    if (n_bands != 4) return PH_LAUNCH_UNSUPPORTED;
+    if (blocks == 0) return 0;
+    if (fp32)
+        hipLaunchKernelGGL((ph_objective_kernel<float, 4>), dim3(blocks), dim3(256), 0,
+                           (hipStream_t)stream, *c, n, x, rrs, sec_sun, out,
+                           rrs_error);
+    else
+        hipLaunchKernelGGL((ph_objective_kernel<double, 4>), dim3(blocks), dim3(256), 0,
+                           (hipStream_t)stream, *c, n, x, rrs, sec_sun, out,
+                           rrs_error);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ph_launch_gen(const PhConst* c, uint64_t seed, long pix0, long n, double sigma,
+                             double deep_frac, float* rrs, float* sec_sun, float* truth,
+                             void* stream)
+{
+    int blocks = (int)((n + 255) / 256);
+    if (blocks == 0) return 0;
+    hipLaunchKernelGGL(ph_gen_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                       *c, seed, pix0, n, sigma, deep_frac, rrs, sec_sun, truth);
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,92 @@
+/* photic_amd -- the internal boundary between the C host code and the HIP kernels.
+ *
+ * Plain C structs: the host fills them, the launchers pass them to the kernels by value, so
+ * every uniform constant of a call lives in the kernel-argument segment and is read through
+ * the scalar cache.  Internal to libphotic; the public API is include/photic.h. */
+#ifndef PH_LAUNCH_H
+#define PH_LAUNCH_H
+
+#include <stdint.h>
+
+#define PH_MAXB 8  /* bands per date */
+#define PH_NBOT 8  /* bottom types in the table */
+#define PH_MAXH 16 /* depth starts */
+#define PH_MAXD 8  /* dates per pixel (stacks) */
+
+#define PH_LAUNCH_UNSUPPORTED 100001
+
+/* Uniform constants of one inversion call. */
+typedef struct PhConst {
+    int n_bands;
+    int n_dates;
+    int mode;                /* 0 = native bottoms, 1 = sand/seagrass mix */
+    int n_bottoms;           /* native mode */
+    int bottom_idx[PH_NBOT]; /* native mode */
+    int konvge, kcount, n_hstarts, has_seed;
+    double reqmin, rq, early_exit, nodata;
+    double sec_sun0, sec_view0; /* geometry where the per-pixel arrays are NULL */
+    double hstart[PH_MAXH];
+    double wl[PH_MAXB], a0[PH_MAXB], a1[PH_MAXB], aw[PH_MAXB], bbw[PH_MAXB];
+    double eg[PH_MAXB]; /* exp(-0.015 (lambda - 440)), evaluated on the host with libm */
+    double bot[PH_NBOT][PH_MAXB];
+    double aw640;
+    /* Rrs at 440/490/550/640/750 nm from the band values: the reference's interp_1d segment
+     * and weights depend only on the wavelengths, so the host resolves them once:
+     * value = R[i0] * w0 + R[i1] * w1, with w0 = 1 - alpha, w1 = alpha (model/common.c:298). */
+    int ip_i0[5], ip_i1[5];
+    double ip_w0[5], ip_w1[5];
+    double seed_rel_step[8]; /* seeded start: simplex step as a fraction of the seed value, */
+    double seed_min_step[8]; /* and its floor, per class H, bottom, P, G, X              */
+    /* FP32 copies for the fast path */
+    float fa0[PH_MAXB], fa1[PH_MAXB], faw[PH_MAXB], fbbw[PH_MAXB], feg[PH_MAXB];
+    float fbot[PH_NBOT][PH_MAXB];
+} PhConst;
+
+/* One single-date inversion launch.  Arrays are device pointers, planar per band / per
+ * parameter ([k][n_pix]); any output may be NULL. */
+typedef struct PhInvertArgs {
+    PhConst c;
+    long n_pix;
+    const float *rrs;      /* [n_bands][n_pix] */
+    const float *sec_sun;  /* [n_pix] or NULL (nadir sun) */
+    const float *sec_view; /* [n_pix] or NULL (nadir view) */
+    const float *seed;     /* [n_params][n_pix] when c.has_seed */
+    unsigned long long *queue; /* [0] work counter, [1] watchdog flag; zero at launch */
+    int chunk;
+    unsigned long long max_ticks; /* watchdog: wall-clock ticks before every wave gives up */
+    float *depth, *rrs_error, *model_error, *albedo, *k_min, *iod;
+    float *b_pct; /* [3][n_pix]: sand, seagrass, coral percent */
+    float *bottom_type;
+    double *x_out; /* [n_params][n_pix], |best| */
+    int *status, *n_evals, *n_iter, *n_starts;
+} PhInvertArgs;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int ph_launch_invert(const PhInvertArgs *a, int fp32, int n_bands, int n_bottoms, int grid,
+                     int block, void *stream);
+int ph_launch_zero_u64(unsigned long long *p, int n, void *stream);
+int ph_launch_fwd(const PhConst *c, long n, const double *params, const double *sec_sun,
+                  const double *Y, double *out, int fp32, void *stream);
+int ph_launch_objective(const PhConst *c, long n, const double *x, const float *rrs,
+                        const float *sec_sun, double *out, double *rrs_error, int fp32,
+                        int n_bands, void *stream);
+int ph_launch_gen(const PhConst *c, uint64_t seed, long pix0, long n, double sigma,
+                  double deep_frac, float *rrs, float *sec_sun, float *truth, void *stream);
+int ph_kernel_attrs(int fp32, int n_bands, int n_bottoms, int *vgprs, int *sgprs,
+                    int *scratch_bytes, int *max_threads);
+int ph_kernel_occupancy(int fp32, int n_bands, int n_bottoms, int block, int *blocks_per_cu);
+int ph_lds_bytes(int n_bottoms, int block);
+
+/* Host side (photic_capi.c): the launch constants for a photic_cfg; exported for the test
+ * suite's host build of the lane engine. */
+struct photic_cfg;
+int ph_build_const(const struct photic_cfg *cfg, int n_dates, int has_seed, PhConst *out);
+int ph_const_size(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,485 @@
+/* photic_amd -- C host code: the public C ABI of include/photic.h.
+ *
+ * Turns a photic_cfg into the uniform constants of a launch (band tables interpolated at the
+ * band wavelengths as model/samodel.c:505-618 does, Nelder-Mead settings, the interpolation
+ * segments of model/samodel.c:1785-1799), checks arguments, launches the gfx950 kernels of
+ * ph_kernels.hip on the caller's stream, and wraps the HIP runtime for callers that have no
+ * allocator of their own.  No computation happens here on the CPU apart from those
+ * per-call constants; without a GPU every computing entry point returns PHOTIC_EDEVICE. */
+#include "photic.h"
+
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "ph_launch.h"
+#include "ph_tables.h"
+
+/* ------------------------------------------------------------------------------------ */
+/* errors                                                                               */
+/* ------------------------------------------------------------------------------------ */
+
+static _Thread_local char ph_err[512];
+
+const char *photic_last_error(void) { return ph_err; }
+
+static int ph_fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ph_err, sizeof(ph_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+static int ph_hip(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return PHOTIC_OK;
+    if (e == hipErrorOutOfMemory)
+        return ph_fail(PHOTIC_ENOMEM, "%s: %s", what, hipGetErrorString(e));
+    return ph_fail(PHOTIC_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+static int ph_launch_status(int rc, const char *what)
+{
+    if (rc == PH_LAUNCH_UNSUPPORTED)
+        return ph_fail(PHOTIC_EUNSUPPORTED, "%s: configuration not compiled", what);
+    return ph_hip((hipError_t)rc, what);
+}
+
+static int ph_have_device(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+        return ph_fail(PHOTIC_EDEVICE, "no GPU device available");
+    return PHOTIC_OK;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* the reference's helpers, model/common.c:298-335 and :392-406                          */
+/* ------------------------------------------------------------------------------------ */
+
+static int ph_feq(float a, float b, float eps)
+{
+    float d = a - b;
+    double fa = fabs((double)a), fb = fabs((double)b);
+    return fabs((double)d) <= ((fa < fb ? fb : fa) * (double)eps);
+}
+
+/* Segment and weights of the reference's interp_1d for query x: value = Y[i0] w0 + Y[i1] w1
+ * (node hits return the node value: i1 = i0, w0 = 1, w1 = 0). */
+static void ph_interp_weights(const double *X, int n, double x, int *i0, int *i1, double *w0,
+                              double *w1)
+{
+    int a = 0, b = 1;
+    float xf = (float)x;
+    if (ph_feq(xf, (float)X[0], 1.0e-5f)) {
+        *i0 = *i1 = 0; *w0 = 1.0; *w1 = 0.0;
+        return;
+    }
+    if (ph_feq(xf, (float)X[n - 1], 1.0e-5f)) {
+        *i0 = *i1 = n - 1; *w0 = 1.0; *w1 = 0.0;
+        return;
+    }
+    if (X[0] < X[n - 1] && x < X[0]) {
+        a = 0; b = 1;
+    } else if (X[n - 1] > X[0] && x > X[n - 1]) {
+        a = n - 2; b = n - 1;
+    } else {
+        for (int i = 0; i < n - 1; i++) {
+            float xi = (float)X[i], xj = (float)X[i + 1];
+            int le_i = xi < xf || ph_feq(xi, xf, 1.0e-5f);
+            int ge_j = xj > xf || ph_feq(xj, xf, 1.0e-5f);
+            int ge_i = xi > xf || ph_feq(xi, xf, 1.0e-5f);
+            int le_j = xj < xf || ph_feq(xj, xf, 1.0e-5f);
+            if ((le_i && ge_j) || (ge_i && le_j)) {
+                a = i; b = i + 1;
+                break;
+            }
+        }
+    }
+    double alpha = (x - X[a]) / (X[b] - X[a]);
+    *i0 = a; *i1 = b; *w0 = 1.0 - alpha; *w1 = alpha;
+}
+
+static double ph_interp(const double *X, const double *Y, int n, double x)
+{
+    int i0, i1;
+    double w0, w1;
+    ph_interp_weights(X, n, x, &i0, &i1, &w0, &w1);
+    return Y[i0] * w0 + Y[i1] * w1;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* configuration                                                                          */
+/* ------------------------------------------------------------------------------------ */
+
+void photic_cfg_default(photic_cfg *cfg, int n_bands, const double *wavelengths)
+{
+    static const double hs[8] = {40.0, 30.0, 20.0, 15.0, 10.0, 7.5, 2.5, 1.0};
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->n_bands = n_bands;
+    for (int b = 0; b < n_bands && b < PHOTIC_MAX_BANDS; b++)
+        cfg->wavelengths[b] = wavelengths[b];
+    cfg->mode = PHOTIC_MODE_MIX;
+    cfg->n_bottoms = 2;
+    for (int k = 0; k < PHOTIC_N_BOTTOM_TYPES; k++)
+        cfg->bottom_idx[k] = k;
+    cfg->reqmin = 1.0e-2;
+    cfg->konvge = 100;
+    cfg->kcount = 5000;
+    cfg->n_hstarts = 8;
+    for (int i = 0; i < 8; i++)
+        cfg->hstart[i] = hs[i];
+    cfg->early_exit = 0.0;
+    cfg->nodata = -9999.0f;
+    cfg->sec_sun = 1.0;
+    cfg->sec_view = 1.0;
+    cfg->precision = PHOTIC_FP64;
+    /* seeded starts: H +-30 %, B +-0.25, P/G/X a factor 2 either side */
+    const double rel[5] = {0.3, 0.0, 1.0, 1.0, 1.0};
+    const double mn[5] = {0.5, 0.25, 0.1, 0.1, 0.02};
+    for (int k = 0; k < 5; k++) {
+        cfg->seed_rel_step[k] = rel[k];
+        cfg->seed_min_step[k] = mn[k];
+    }
+}
+
+int photic_n_params(const photic_cfg *cfg)
+{
+    if (cfg->mode == PHOTIC_MODE_MIX) return 5;
+    return 1 + 2 * cfg->n_bottoms + 3;
+}
+
+static int ph_make_const(const photic_cfg *cfg, int n_dates, int has_seed, PhConst *c)
+{
+    if (!cfg) return ph_fail(PHOTIC_EINVAL, "cfg is NULL");
+    if (cfg->n_bands < 2 || cfg->n_bands > PH_MAXB)
+        return ph_fail(PHOTIC_EINVAL, "n_bands %d outside 2..%d", cfg->n_bands, PH_MAXB);
+    for (int b = 1; b < cfg->n_bands; b++)
+        if (!(cfg->wavelengths[b] > cfg->wavelengths[b - 1]))
+            return ph_fail(PHOTIC_EINVAL, "wavelengths must ascend");
+    if (cfg->mode != PHOTIC_MODE_MIX && cfg->mode != PHOTIC_MODE_NATIVE)
+        return ph_fail(PHOTIC_EINVAL, "mode %d unknown", cfg->mode);
+    if (cfg->mode == PHOTIC_MODE_NATIVE) {
+        if (cfg->n_bottoms < 1 || cfg->n_bottoms > 3)
+            return ph_fail(PHOTIC_EUNSUPPORTED, "native mode supports 1..3 bottoms, not %d",
+                           cfg->n_bottoms);
+        for (int k = 0; k < cfg->n_bottoms; k++)
+            if (cfg->bottom_idx[k] < 0 || cfg->bottom_idx[k] >= PH_NBOT)
+                return ph_fail(PHOTIC_EINVAL, "bottom_idx[%d] = %d", k, cfg->bottom_idx[k]);
+    }
+    if (!(cfg->reqmin > 0.0) || cfg->konvge < 1 || cfg->kcount < 1)
+        return ph_fail(PHOTIC_EINVAL, "reqmin must be > 0, konvge and kcount >= 1");
+    if (cfg->n_hstarts < 0 || cfg->n_hstarts > PH_MAXH || (cfg->n_hstarts == 0 && !has_seed))
+        return ph_fail(PHOTIC_EINVAL, "n_hstarts %d outside 1..%d", cfg->n_hstarts, PH_MAXH);
+    if (cfg->precision != PHOTIC_FP64 && cfg->precision != PHOTIC_FP32)
+        return ph_fail(PHOTIC_EINVAL, "precision %d unknown", cfg->precision);
+
+    memset(c, 0, sizeof(*c));
+    const int nb = cfg->n_bands;
+    c->n_bands = nb;
+    c->n_dates = n_dates;
+    c->mode = cfg->mode == PHOTIC_MODE_MIX ? 1 : 0;
+    c->n_bottoms = cfg->mode == PHOTIC_MODE_MIX ? 2 : cfg->n_bottoms;
+    for (int k = 0; k < PH_NBOT; k++)
+        c->bottom_idx[k] = cfg->mode == PHOTIC_MODE_MIX ? k : cfg->bottom_idx[k];
+    c->konvge = cfg->konvge;
+    c->kcount = cfg->kcount;
+    c->n_hstarts = cfg->n_hstarts;
+    c->has_seed = has_seed;
+    c->reqmin = cfg->reqmin;
+    int n = cfg->mode == PHOTIC_MODE_MIX ? 2 + 3 * n_dates : 1 + 2 * cfg->n_bottoms + 3 * n_dates;
+    c->rq = cfg->reqmin * (double)n; /* asa047.c:170 */
+    c->early_exit = cfg->early_exit > 0.0 ? cfg->early_exit : 2.5 * (double)(float)n_dates;
+    c->nodata = (double)cfg->nodata;
+    c->sec_sun0 = cfg->sec_sun;
+    c->sec_view0 = cfg->sec_view;
+    for (int i = 0; i < PH_MAXH; i++)
+        c->hstart[i] = i < cfg->n_hstarts ? cfg->hstart[i] : 0.0;
+    for (int b = 0; b < nb; b++) {
+        double wl = cfg->wavelengths[b];
+        c->wl[b] = wl;
+        c->a0[b] = ph_interp(ph_ref_wl, ph_tab_a0, PH_NREF, wl);
+        c->a1[b] = ph_interp(ph_ref_wl, ph_tab_a1, PH_NREF, wl);
+        c->bbw[b] = ph_interp(ph_ref_wl, ph_tab_bbw, PH_NREF, wl);
+        c->aw[b] = ph_interp(ph_ref_wl, ph_tab_aw, PH_NREF, wl);
+        c->eg[b] = exp(-0.015 * (wl - 440.0));
+        for (int k = 0; k < PH_NBOT; k++)
+            c->bot[k][b] = ph_interp(ph_ref_wl, ph_tab_bottom[k], PH_NREF, wl);
+        c->fa0[b] = (float)c->a0[b];
+        c->fa1[b] = (float)c->a1[b];
+        c->faw[b] = (float)c->aw[b];
+        c->fbbw[b] = (float)c->bbw[b];
+        c->feg[b] = (float)c->eg[b];
+        for (int k = 0; k < PH_NBOT; k++)
+            c->fbot[k][b] = (float)c->bot[k][b];
+    }
+    c->aw640 = ph_interp(ph_ref_wl, ph_tab_aw, PH_NREF, 640.0);
+    const double q[5] = {440.0, 490.0, 550.0, 640.0, 750.0};
+    for (int k = 0; k < 5; k++)
+        ph_interp_weights(cfg->wavelengths, nb, q[k], &c->ip_i0[k], &c->ip_i1[k], &c->ip_w0[k],
+                          &c->ip_w1[k]);
+    for (int k = 0; k < 5; k++) {
+        c->seed_rel_step[k] = cfg->seed_rel_step[k];
+        c->seed_min_step[k] = cfg->seed_min_step[k];
+    }
+    return PHOTIC_OK;
+}
+
+int ph_build_const(const struct photic_cfg *cfg, int n_dates, int has_seed, PhConst *out)
+{
+    return ph_make_const(cfg, n_dates, has_seed, out);
+}
+
+int ph_const_size(void) { return (int)sizeof(PhConst); }
+
+/* ------------------------------------------------------------------------------------ */
+/* inversion                                                                              */
+/* ------------------------------------------------------------------------------------ */
+
+static int ph_n_cu(int *n_cu)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+    return ph_hip(e, "hipDeviceGetAttribute");
+}
+
+int photic_kernel_info(const photic_cfg *cfg, int *vgprs, int *scratch_bytes,
+                       int *blocks_per_cu, int *n_cu)
+{
+    PhConst c;
+    int rc = ph_have_device();
+    if (rc) return rc;
+    rc = ph_make_const(cfg, 1, 0, &c);
+    if (rc) return rc;
+    int fp32 = cfg->precision == PHOTIC_FP32, nbot = c.mode ? 0 : c.n_bottoms;
+    int sg = 0, mt = 0, block = cfg->block > 0 ? cfg->block : 256;
+    rc = ph_kernel_attrs(fp32, c.n_bands, nbot, vgprs, &sg, scratch_bytes, &mt);
+    if (rc) return ph_launch_status(rc, "kernel attributes");
+    rc = ph_kernel_occupancy(fp32, c.n_bands, nbot, block, blocks_per_cu);
+    if (rc) return ph_launch_status(rc, "occupancy");
+    return ph_n_cu(n_cu);
+}
+
+int photic_invert(const photic_cfg *cfg, long n_pix, const float *rrs, const float *sec_sun,
+                  const float *sec_view, const float *seed, const photic_outputs *out,
+                  void *stream)
+{
+    PhInvertArgs a;
+    int rc = ph_have_device();
+    if (rc) return rc;
+    if (n_pix < 0) return ph_fail(PHOTIC_EINVAL, "n_pix < 0");
+    if (!out) return ph_fail(PHOTIC_EINVAL, "out is NULL");
+    if (n_pix > 0 && !rrs) return ph_fail(PHOTIC_EINVAL, "rrs is NULL");
+    memset(&a, 0, sizeof(a));
+    rc = ph_make_const(cfg, 1, seed != NULL, &a.c);
+    if (rc) return rc;
+    if (n_pix == 0) return PHOTIC_OK;
+
+    const int fp32 = cfg->precision == PHOTIC_FP32;
+    const int nbot = a.c.mode ? 0 : a.c.n_bottoms;
+    if (fp32 && nbot != 0)
+        return ph_fail(PHOTIC_EUNSUPPORTED, "FP32 forward model is compiled for MIX mode only");
+    /* block size: the lanes' simplexes live in LDS, so the largest multiple of 64 lanes
+     * (up to 256) whose LDS fits one workgroup */
+    int dev = 0, max_lds = 0;
+    rc = ph_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    rc = ph_hip(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+                "hipDeviceGetAttribute");
+    if (rc) return rc;
+    const int lds_lane = ph_lds_bytes(nbot, 1);
+    int block = cfg->block > 0 ? cfg->block : (max_lds / lds_lane) / 64 * 64;
+    if (cfg->block <= 0 && block > 256) block = 256;
+    if (block < 64 || block % 64 != 0 || block > 256)
+        return ph_fail(PHOTIC_EINVAL, "block %d must be a multiple of 64 in 64..256", block);
+    if ((long)lds_lane * block > max_lds)
+        return ph_fail(PHOTIC_EUNSUPPORTED, "block %d needs %d bytes of LDS, the device has %d",
+                       block, lds_lane * block, max_lds);
+    int n_cu = 0, bpc = 0;
+    rc = ph_n_cu(&n_cu);
+    if (rc) return rc;
+    rc = ph_kernel_occupancy(fp32, a.c.n_bands, nbot, block, &bpc);
+    if (rc) return ph_launch_status(rc, "occupancy");
+    if (bpc < 1) return ph_fail(PHOTIC_EUNSUPPORTED, "kernel does not fit a CU at block %d", block);
+    long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
+    long waves_needed = (n_pix + 63) / 64;
+    long blocks_needed = (waves_needed * 64 + block - 1) / block;
+    if (grid > blocks_needed) grid = blocks_needed;
+    if (grid < 1) grid = 1;
+
+    a.n_pix = n_pix;
+    a.rrs = rrs;
+    a.sec_sun = sec_sun;
+    a.sec_view = sec_view;
+    a.seed = seed;
+    a.chunk = cfg->chunk > 0 ? cfg->chunk : 64;
+    a.depth = out->depth;
+    a.rrs_error = out->rrs_error;
+    a.model_error = out->model_error;
+    a.albedo = out->albedo;
+    a.k_min = out->k_min;
+    a.iod = out->iod;
+    a.b_pct = out->b_pct;
+    a.bottom_type = out->bottom_type;
+    a.x_out = out->x;
+    a.status = out->status;
+    a.n_evals = out->n_evals;
+    a.n_iter = out->n_iter;
+    a.n_starts = out->n_starts;
+
+    /* watchdog: an inversion kernel still running after this long gives up (every wave
+     * exits, leaving its pixels unwritten); with cfg->check_watchdog the call waits for the
+     * kernel and reports PHOTIC_EDEVICE when that happened */
+    int wrate = 0;
+    if (hipDeviceGetAttribute(&wrate, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        wrate <= 0)
+        wrate = 100000; /* kHz */
+    double limit_s = cfg->watchdog_s > 0.0 ? cfg->watchdog_s : 600.0;
+    a.max_ticks = (unsigned long long)(limit_s * 1000.0 * (double)wrate);
+
+    hipStream_t st = (hipStream_t)stream;
+    unsigned long long *queue = NULL;
+    rc = ph_hip(hipMallocAsync((void **)&queue, 2 * sizeof(unsigned long long), st),
+                "hipMallocAsync");
+    if (rc) return rc;
+    rc = ph_launch_status(ph_launch_zero_u64(queue, 2, st), "zero work counter");
+    if (rc == PHOTIC_OK) {
+        a.queue = queue;
+        rc = ph_launch_status(ph_launch_invert(&a, fp32, a.c.n_bands, nbot, (int)grid, block, st),
+                              "inversion kernel");
+    }
+    if (rc == PHOTIC_OK && cfg->check_watchdog) {
+        unsigned long long flag = 0;
+        rc = ph_hip(hipMemcpyAsync(&flag, queue + 1, sizeof(flag), hipMemcpyDeviceToHost, st),
+                    "watchdog read");
+        if (rc == PHOTIC_OK) rc = ph_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        if (rc == PHOTIC_OK && flag)
+            rc = ph_fail(PHOTIC_EDEVICE, "inversion kernel stopped by its %.0f s watchdog",
+                         limit_s);
+    }
+    int rc2 = ph_hip(hipFreeAsync(queue, st), "hipFreeAsync");
+    return rc ? rc : rc2;
+}
+
+int photic_gen_scene(const photic_cfg *cfg, uint64_t seed, long pix0, long n, double sigma,
+                     double deep_frac, float *rrs, float *sec_sun, float *truth, void *stream)
+{
+    PhConst c;
+    int rc = ph_have_device();
+    if (rc) return rc;
+    rc = ph_make_const(cfg, 1, 0, &c);
+    if (rc) return rc;
+    if (n < 0 || pix0 < 0) return ph_fail(PHOTIC_EINVAL, "negative pixel range");
+    if (n > 0 && (!rrs || !sec_sun)) return ph_fail(PHOTIC_EINVAL, "rrs and sec_sun are required");
+    return ph_launch_status(
+        ph_launch_gen(&c, seed, pix0, n, sigma, deep_frac, rrs, sec_sun, truth, stream),
+        "scene generator");
+}
+
+int photic_forward(const photic_cfg *cfg, long n, const double *params, const double *sec_sun,
+                   const double *Y, double *out, int precision, void *stream)
+{
+    PhConst c;
+    int rc = ph_have_device();
+    if (rc) return rc;
+    rc = ph_make_const(cfg, 1, 0, &c);
+    if (rc) return rc;
+    if (n < 0) return ph_fail(PHOTIC_EINVAL, "n < 0");
+    return ph_launch_status(ph_launch_fwd(&c, n, params, sec_sun, Y, out,
+                                          precision == PHOTIC_FP32, stream),
+                            "forward kernel");
+}
+
+int photic_objective(const photic_cfg *cfg, long n, const double *x, const float *rrs,
+                     const float *sec_sun, double *out, double *rrs_error, int precision,
+                     void *stream)
+{
+    PhConst c;
+    int rc = ph_have_device();
+    if (rc) return rc;
+    rc = ph_make_const(cfg, 1, 0, &c);
+    if (rc) return rc;
+    if (n < 0) return ph_fail(PHOTIC_EINVAL, "n < 0");
+    return ph_launch_status(ph_launch_objective(&c, n, x, rrs, sec_sun, out, rrs_error,
+                                                precision == PHOTIC_FP32, c.n_bands, stream),
+                            "objective kernel");
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* device plumbing                                                                        */
+/* ------------------------------------------------------------------------------------ */
+
+int photic_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int photic_device_malloc(void **ptr, size_t bytes)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    return ph_hip(hipMalloc(ptr, bytes ? bytes : 1), "hipMalloc");
+}
+
+int photic_device_free(void *ptr) { return ph_hip(hipFree(ptr), "hipFree"); }
+
+int photic_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+    return ph_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+int photic_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+    return ph_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+int photic_memset_zero(void *dst, size_t bytes)
+{
+    return ph_hip(hipMemset(dst, 0, bytes), "hipMemset");
+}
+
+int photic_synchronize(void) { return ph_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+int photic_stream_create(void **stream)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    return ph_hip(hipStreamCreateWithFlags((hipStream_t *)stream, hipStreamNonBlocking),
+                  "hipStreamCreate");
+}
+
+int photic_stream_destroy(void *stream)
+{
+    return ph_hip(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
+}
+
+int photic_event_create(void **event)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    return ph_hip(hipEventCreate((hipEvent_t *)event), "hipEventCreate");
+}
+
+int photic_event_destroy(void *event)
+{
+    return ph_hip(hipEventDestroy((hipEvent_t)event), "hipEventDestroy");
+}
+
+int photic_event_record(void *event, void *stream)
+{
+    return ph_hip(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "hipEventRecord");
+}
+
+int photic_event_elapsed_ms(void *start, void *stop, float *ms)
+{
+    hipError_t e = hipEventSynchronize((hipEvent_t)stop);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop);
+    return ph_hip(e, "hipEventElapsedTime");
+}

@@ -1,0 +1,83 @@
+// Test infrastructure: the product's lane engine (photic_amd/csrc/ph_engine.h) compiled for
+// the host and run one lane at a time, so the CPU test suite can check the exact state
+// machine that the GPU kernel runs -- in its FP64 instantiation, with the host's libm, where
+// it must match the FP64 oracle bit for bit.  Never loaded by the library.
+#include <hip/hip_runtime.h>
+#include <vector>
+
+#include "ph_engine.h"
+
+extern "C" int phs_invert(const PhConst* c, int n_bottoms, long n_pix, const float* rrs,
+                          const float* sec_sun, const float* seed, double* x_out, float* depth,
+                          float* rrs_error, float* model_error, int* status, int* n_evals,
+                          int* n_iter, int* n_starts);
+
+template <int NBOT>
+static int run(const PhConst& c, long n_pix, const float* rrs, const float* sec_sun,
+               const float* seed, double* x_out, float* depth, float* rrs_error,
+               float* model_error, int* status, int* n_evals, int* n_iter, int* n_starts)
+{
+    constexpr int NB = 4;
+    constexpr int N = PhLayout<NBOT>::template n<1>();
+    std::vector<double> mem(PhLaneMem<N>::n_slots);
+    for (long p = 0; p < n_pix; p++) {
+        PhNM<N> S;
+        S.m.base = mem.data();
+        S.m.stride = 1;
+        for (int k = 0; k <= N; k++) S.y[k] = 0.0;
+        PhPixel<double, 1, NB> px;
+        float r[1][NB];
+        for (int b = 0; b < NB; b++) r[0][b] = rrs[(long)b * n_pix + p];
+        double ss = sec_sun ? (double)sec_sun[p] : c.sec_sun0, sv = c.sec_view0;
+        if (!ph_pixel_init<double, 1, NB>(px, c, r, &ss, &sv)) {
+            status[p] = -1;
+            depth[p] = 0.0f;
+            continue;
+        }
+        S.kstart = c.has_seed ? -1 : 0;
+        S.lowest = 1.0e4;
+        S.nstart = 0;
+        S.evals = 0;
+        S.best_icount = 0;
+        S.best_conv = 0;
+        for (int i = 0; i < N; i++) S.m.set_best(i, 0.0);
+        S.phase = PH_START;
+        PhSide<1, NB> side;
+        for (;;) {
+            if (S.phase == PH_START) {
+                double sd[N];
+                if (S.kstart < 0)
+                    for (int i = 0; i < N; i++) sd[i] = (double)seed[(long)i * n_pix + p];
+                ph_start_step<double, 1, NB, NBOT>(S, c, px, S.kstart, sd);
+                ph_nm_begin(S, c);
+            }
+            double f = ph_objective<double, 1, NB, NBOT, true>(S.xe, c, px, &side);
+            ph_nm_consume(S, c, f);
+            if (S.phase == PH_DONE) break;
+        }
+        for (int i = 0; i < N; i++) x_out[(long)i * n_pix + p] = fabs(S.m.best(i));
+        depth[p] = (float)fabs(S.m.best(0));
+        rrs_error[p] = (float)side.Rrs_error;
+        model_error[p] = (float)side.model_error;
+        status[p] = S.best_conv ? 0 : 2;
+        n_evals[p] = S.evals;
+        n_iter[p] = S.best_icount;
+        n_starts[p] = S.nstart;
+    }
+    return 0;
+}
+
+extern "C" int phs_invert(const PhConst* c, int n_bottoms, long n_pix, const float* rrs,
+                          const float* sec_sun, const float* seed, double* x_out, float* depth,
+                          float* rrs_error, float* model_error, int* status, int* n_evals,
+                          int* n_iter, int* n_starts)
+{
+    if (c->n_bands != 4) return 2;
+    switch (n_bottoms) {
+    case 0: return run<0>(*c, n_pix, rrs, sec_sun, seed, x_out, depth, rrs_error, model_error, status, n_evals, n_iter, n_starts);
+    case 1: return run<1>(*c, n_pix, rrs, sec_sun, seed, x_out, depth, rrs_error, model_error, status, n_evals, n_iter, n_starts);
+    case 2: return run<2>(*c, n_pix, rrs, sec_sun, seed, x_out, depth, rrs_error, model_error, status, n_evals, n_iter, n_starts);
+    case 3: return run<3>(*c, n_pix, rrs, sec_sun, seed, x_out, depth, rrs_error, model_error, status, n_evals, n_iter, n_starts);
+    default: return 2;
+    }
+}

@@ -1,0 +1,69 @@
+"""The product's lane engine (photic_amd/csrc/ph_engine.h, the code the GPU kernel runs),
+compiled for the host by tests/hostsim and run one lane at a time, against the FP64 oracle:
+with the host's libm on both sides the two must agree bit for bit -- every parameter, every
+evaluation count, every convergence flag.  This pins the state-machine recasting of the
+reference's Nelder-Mead and multi-start, independent of GPU math-library rounding."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import photic_amd as P
+
+
+def _cfg(**kw):
+    base = dict(reqmin=1e-10, konvge=100, kcount=2000)
+    base.update(kw)
+    return P.cfg_default(**base)
+
+
+@pytest.mark.parametrize("mode,nbot", [(1, 0), (0, 1), (0, 2), (0, 3)])
+def test_engine_matches_oracle(mode, nbot):
+    rrs, ss, truth, nodata, _ = O.gen(1, 160)
+    cfg = _cfg(mode=mode, n_bottoms=max(nbot, 1) if mode == 0 else 2)
+    const = P.const_bytes(cfg)
+    h = O.hostsim_invert(const, nbot, rrs, ss)
+    o = O.invert(rrs, ss, mode=mode, n_bottoms=max(nbot, 1), reqmin=1e-10, konvge=100,
+                 kcount=2000)
+    valid = o["status"] >= 0
+    assert np.array_equal(h["status"], o["status"])
+    assert np.array_equal(h["x"][:, valid], o["x"][:, valid])
+    assert np.array_equal(h["n_iter"][valid], o["n_iter"][valid])
+    assert np.array_equal(h["n_starts"][valid], o["n_starts"][valid])
+    assert np.array_equal(h["rrs_error"][valid], o["rrs_error"][valid])
+    assert np.array_equal(h["model_error"][valid], o["model_error"][valid])
+    # the oracle evaluates each depth start once before handing it to nelmin, as the
+    # reference does (model/samodel.c:2365); the engine skips that discarded evaluation
+    assert np.array_equal(h["n_evals"][valid] + o["n_starts"][valid], o["n_evals"][valid])
+
+
+def test_engine_matches_oracle_reference_settings():
+    # the reference's own settings: reqmin 1e-2, kcount 5000
+    rrs, ss, _, _, _ = O.gen(2, 128)
+    cfg = P.cfg_default()
+    h = O.hostsim_invert(P.const_bytes(cfg), 0, rrs, ss)
+    o = O.invert(rrs, ss, reqmin=1e-2, konvge=100, kcount=5000)
+    assert np.array_equal(h["x"], o["x"])
+    assert np.array_equal(h["n_iter"], o["n_iter"])
+
+
+def test_engine_matches_oracle_seeded():
+    rrs, ss, truth, nodata, _ = O.gen(3, 128)
+    seed = np.stack([truth[0] * 1.2, truth[4], 100 * truth[1], 100 * truth[2], 100 * truth[3]])
+    cfg = _cfg()
+    const = P.const_bytes(cfg, has_seed=True)
+    h = O.hostsim_invert(const, 0, rrs, ss, seed=seed.astype(np.float32))
+    o = O.invert(rrs, ss, seed=seed.astype(np.float32))
+    assert np.array_equal(h["x"], o["x"])
+    assert np.array_equal(h["n_iter"], o["n_iter"])
+    assert np.array_equal(h["n_starts"], o["n_starts"])
+
+
+def test_engine_small_kcount_and_restarts():
+    # tiny kcount: runs end on the evaluation limit (ifault 2) and the restart branch
+    rrs, ss, _, _, _ = O.gen(4, 96)
+    for kcount, reqmin in ((60, 1e-10), (400, 1e-1), (3000, 1e-1)):
+        cfg = _cfg(kcount=kcount, reqmin=reqmin, konvge=7)
+        h = O.hostsim_invert(P.const_bytes(cfg), 0, rrs, ss)
+        o = O.invert(rrs, ss, reqmin=reqmin, konvge=7, kcount=kcount)
+        assert np.array_equal(h["x"], o["x"]), kcount
+        assert np.array_equal(h["status"], o["status"]), kcount
