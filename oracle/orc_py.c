@@ -252,3 +252,19 @@ double orc_py_mlp_loss_grad(const double *w, int n, const double *in, const doub
 }
 
 int orc_py_mlp_nparam(void) { return ORC_MLP_NPARAM; }
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the oracle's OpenMP loops (<= 0: leave the runtime's default). */
+int orc_py_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}

@@ -93,6 +93,7 @@ def _load():
         "photic_memcpy_dtoh": (i, [vp, vp, C.c_size_t]),
         "photic_memset_zero": (i, [vp, C.c_size_t]),
         "photic_synchronize": (i, []),
+        "photic_device_copy": (i, [vp, vp, C.c_size_t, vp]),
         "photic_stream_create": (i, [C.POINTER(vp)]),
         "photic_stream_destroy": (i, [vp]),
         "photic_event_create": (i, [C.POINTER(vp)]),

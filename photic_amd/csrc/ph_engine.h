@@ -59,8 +59,8 @@ struct PhSide {
 template <typename FT, int ND, int NB>
 struct PhPixel {
     double rrs[ND][NB];       // measured Rrs, float input widened
-    FT pw[ND][NB];            // (440/lambda)^Y, Y from the measured 440/490 ratio
-    FT ss[ND], sv[ND];        // sec(sun), sec(view), subsurface
+    double pw[ND][NB];        // (440/lambda)^Y, Y from the measured 440/490 ratio
+    double ss[ND], sv[ND];    // sec(sun), sec(view), subsurface
     double tot;               // sum of measured Rrs in (date, band) order
 };
 
@@ -152,30 +152,58 @@ PH_INLINE double ph_band64(double P, double G, double X, double rho, double H,
     return 0.5 * r / (1.0 - 1.5 * r) + 0.0;
 }
 
-// FP32: the same model with hardware transcendentals and fused multiply-adds.  ln P is
-// shared by the bands of a date and passed in.  Inputs are clamped to a range where no
-// intermediate overflows single precision.
-__device__ inline float ph_band32(float P, float lnP, float G, float X, float rho, float H,
-                                  float a0, float a1, float aw, float bbw, float eg, float pw,
-                                  float ss, float sv, float* K_out, float* rB_out, float* r_out)
+// Fast path: the same model with the arithmetic in FP64 (full rate on gfx950) and the
+// transcendentals from the single-precision hardware units, each made accurate to a few
+// 1e-8 relative: exp with an exact FP64 range reduction (2^n * 2^f, |f| <= 1/2), and the
+// double-precision reciprocal / reciprocal square root refined by one Newton step.
+__device__ PH_INLINE double ph_rcp_fast(double x)
 {
-    float a = fmaf(G, eg, fmaf(fmaf(a1, lnP, a0), P, aw));
-    float bb = fmaf(X, pw, bbw);
-    float apb = a + bb;
-    float u = bb * __builtin_amdgcn_rcpf(apb);
-    float K = fminf(fmaxf(apb, 0.0f), 2.5f);
-    float rrs_dp = fmaf(0.170f, u, 0.084f) * u;
-    float DuC = 1.03f * __builtin_amdgcn_sqrtf(fmaf(2.4f, u, 1.0f));
-    float DuB = 1.04f * __builtin_amdgcn_sqrtf(fmaf(5.4f, u, 1.0f));
-    float KH = K * H;
-    const float L2E = 1.4426950408889634f;
-    float eC = __builtin_amdgcn_exp2f(-fmaf(DuC, sv, ss) * KH * L2E);
-    float eB = __builtin_amdgcn_exp2f(-fmaf(DuB, sv, ss) * KH * L2E);
-    float r = fmaf(rrs_dp, 1.0f - eC, rho * 0.3183098861837907f * eB);
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ PH_INLINE double ph_sqrt_fast(double x)
+{
+    double r = __builtin_amdgcn_rsq(x); // ~ 1/sqrt(x)
+    double s = x * r;
+    double h = 0.5 * r;
+    double e = fma(-s, h, 0.5);
+    return fma(s, e, s);
+}
+
+__device__ PH_INLINE double ph_exp_fast(double x)
+{
+    x = fmax(x, -700.0);
+    double y = x * 1.4426950408889634;
+    double n = rint(y);
+    float f = (float)(y - n);
+    return __builtin_amdgcn_ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);
+}
+
+__device__ PH_INLINE double ph_band_fast(double P, double lnP, double G, double X,
+                                         double rho_pi, double H, double a0, double a1,
+                                         double aw, double bbw, double eg, double pw,
+                                         double ss, double sv, double* K_out, double* rB_out,
+                                         double* r_out)
+{
+    double a = fma(G, eg, fma(fma(a1, lnP, a0), P, aw));
+    double bb = fma(X, pw, bbw);
+    double apb = a + bb;
+    double u = bb * ph_rcp_fast(apb);
+    double K = fmin(fmax(apb, 0.0), 2.5);
+    double rrs_dp = fma(0.170, u, 0.084) * u;
+    double DuC = 1.03 * ph_sqrt_fast(fma(2.4, u, 1.0));
+    double DuB = 1.04 * ph_sqrt_fast(fma(5.4, u, 1.0));
+    double KH = K * H;
+    double eC = ph_exp_fast(-fma(DuC, sv, ss) * KH);
+    double eB = ph_exp_fast(-fma(DuB, sv, ss) * KH);
+    double rB = rho_pi * eB;
+    double r = fma(rrs_dp, 1.0 - eC, rB);
     *K_out = K;
-    *rB_out = rho * 0.3183098861837907f * eB;
+    *rB_out = rB;
     *r_out = r;
-    return 0.5f * r * __builtin_amdgcn_rcpf(fmaf(-1.5f, r, 1.0f));
+    return 0.5 * r * ph_rcp_fast(fma(-1.5, r, 1.0));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -253,29 +281,23 @@ PH_INLINE double ph_objective(const double* x, const PhConst& c,
                 }
             }
         } else {
-            const float HUGE_F = 1.0e30f;
-            float Pf = fminf(fmaxf((float)P, 1.0e-30f), HUGE_F);
-            float Gf = fminf((float)G, HUGE_F), Xf = fminf((float)X, HUGE_F);
-            float Hf = fminf((float)H, HUGE_F);
-            float lnP = __builtin_amdgcn_logf(Pf) * 0.6931471805599453f;
-            float qbf[nbot];
-#pragma unroll
-            for (int k = 0; k < nbot; k++) qbf[k] = (float)qb[k];
+            const double lnP = (double)__builtin_amdgcn_logf((float)fmax(P, 1.0e-30)) *
+                               0.6931471805599453;
 #pragma unroll
             for (int b = 0; b < NB; b++) {
-                float rho = 0.0f;
+                double rho = 0.0;
 #pragma unroll
                 for (int k = 0; k < nbot; k++) {
                     int t = NBOT == 0 ? k : c.bottom_idx[k];
-                    rho = fmaf(qbf[k], c.fbot[t][b], rho);
+                    rho = fma(qb[k], c.bot[t][b], rho);
                 }
-                float K, rB, r;
-                float m = ph_band32(Pf, lnP, Gf, Xf, rho, Hf, c.fa0[b], c.fa1[b], c.faw[b],
-                                    c.fbbw[b], c.feg[b], px.pw[s][b], px.ss[s], px.sv[s], &K,
-                                    &rB, &r);
-                double d = (double)m - px.rrs[s][b];
-                err += d * d;
-                Kd[s][b] = (double)K;
+                double K, rB, r;
+                double m = ph_band_fast(P, lnP, G, X, rho * 0.3183098861837907, H, c.a0[b],
+                                        c.a1[b], c.aw[b], c.bbw[b], c.eg[b], px.pw[s][b],
+                                        px.ss[s], px.sv[s], &K, &rB, &r);
+                double d = m - px.rrs[s][b];
+                err = fma(d, d, err);
+                Kd[s][b] = K;
                 if (SIDE) {
                     side->K[s][b] = K;
                     side->rrs_bot[s][b] = rB;
@@ -357,9 +379,9 @@ PH_INLINE bool ph_pixel_init(PhPixel<FT, ND, NB>& px, const PhConst& c,
         if (Y < 0.0) Y = 0.0;
         if (Y > 2.5) Y = 2.5;
 #pragma unroll
-        for (int b = 0; b < NB; b++) px.pw[s][b] = (FT)pow(440.0 / c.wl[b], Y);
-        px.ss[s] = (FT)ss[s];
-        px.sv[s] = (FT)sv[s];
+        for (int b = 0; b < NB; b++) px.pw[s][b] = pow(440.0 / c.wl[b], Y);
+        px.ss[s] = ss[s];
+        px.sv[s] = sv[s];
     }
     px.tot = tot;
     return ok;

@@ -37,9 +37,6 @@ typedef struct PhConst {
     double ip_w0[5], ip_w1[5];
     double seed_rel_step[8]; /* seeded start: simplex step as a fraction of the seed value, */
     double seed_min_step[8]; /* and its floor, per class H, bottom, P, G, X              */
-    /* FP32 copies for the fast path */
-    float fa0[PH_MAXB], fa1[PH_MAXB], faw[PH_MAXB], fbbw[PH_MAXB], feg[PH_MAXB];
-    float fbot[PH_NBOT][PH_MAXB];
 } PhConst;
 
 /* One single-date inversion launch.  Arrays are device pointers, planar per band / per
@@ -68,6 +65,7 @@ extern "C" {
 int ph_launch_invert(const PhInvertArgs *a, int fp32, int n_bands, int n_bottoms, int grid,
                      int block, void *stream);
 int ph_launch_zero_u64(unsigned long long *p, int n, void *stream);
+int ph_launch_copy16(const void *src, void *dst, long n16, void *stream);
 int ph_launch_fwd(const PhConst *c, long n, const double *params, const double *sec_sun,
                   const double *Y, double *out, int fp32, void *stream);
 int ph_launch_objective(const PhConst *c, long n, const double *x, const float *rrs,

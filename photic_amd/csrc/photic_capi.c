@@ -209,13 +209,6 @@ static int ph_make_const(const photic_cfg *cfg, int n_dates, int has_seed, PhCon
         c->eg[b] = exp(-0.015 * (wl - 440.0));
         for (int k = 0; k < PH_NBOT; k++)
             c->bot[k][b] = ph_interp(ph_ref_wl, ph_tab_bottom[k], PH_NREF, wl);
-        c->fa0[b] = (float)c->a0[b];
-        c->fa1[b] = (float)c->a1[b];
-        c->faw[b] = (float)c->aw[b];
-        c->fbbw[b] = (float)c->bbw[b];
-        c->feg[b] = (float)c->eg[b];
-        for (int k = 0; k < PH_NBOT; k++)
-            c->fbot[k][b] = (float)c->bot[k][b];
     }
     c->aw640 = ph_interp(ph_ref_wl, ph_tab_aw, PH_NREF, 640.0);
     const double q[5] = {440.0, 490.0, 550.0, 640.0, 750.0};
@@ -443,6 +436,13 @@ int photic_memcpy_dtoh(void *dst, const void *src, size_t bytes)
 int photic_memset_zero(void *dst, size_t bytes)
 {
     return ph_hip(hipMemset(dst, 0, bytes), "hipMemset");
+}
+
+int photic_device_copy(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (bytes % 16 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 16 != 0)
+        return ph_fail(PHOTIC_EINVAL, "device copy needs 16-byte sizes and alignment");
+    return ph_launch_status(ph_launch_copy16(src, dst, (long)(bytes / 16), stream), "copy kernel");
 }
 
 int photic_synchronize(void) { return ph_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"); }

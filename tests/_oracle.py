@@ -1,195 +1,8 @@
-"""ctypes bindings of the oracle (oracle/liboracle.so) and of the reference's own compiled
-nelmin / interp_1d / approx_equal (oracle/_ref/libphotic_ref.so) -- test infrastructure."""
-import ctypes as C
+"""The oracle's ctypes bindings (oracle/oracle_bindings.py), for the test modules."""
 import os
+import sys
 
-import numpy as np
-
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
-REF = os.path.join(ROOT, "oracle", "_ref", "libphotic_ref.so")
-HOSTSIM = os.path.join(ROOT, "tests", "hostsim", "libph_hostsim.so")
-
-WL4 = (443.0, 482.0, 561.0, 655.0)
-
-
-def _p(a):
-    return None if a is None else a.ctypes.data
-
-
-class PyOut(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ("x", "depth", "rrs_error", "model_error", "k_min", "iod",
-                                          "albedo", "bottom_type", "b_pct", "n_iter",
-                                          "converged", "n_evals", "n_starts", "status")]
-
-
-_lib = None
-_ref = None
-
-
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE):
-            raise RuntimeError("oracle not built: make -C oracle")
-        _lib = C.CDLL(ORACLE)
-        v, i, l, d = C.c_void_p, C.c_int, C.c_long, C.c_double
-        _lib.orc_py_invert.restype = l
-        _lib.orc_py_invert.argtypes = [i, i, v, v, i, l, v, v, d, d, i, i, i, v, d, C.c_float, v, v,
-                                       v, v, C.POINTER(PyOut)]
-        _lib.orc_py_forward.argtypes = [v, i, l, v, v, v, v]
-        _lib.orc_py_objective.argtypes = [v, i, l, v, v, v, v, v]
-        _lib.orc_py_gen.argtypes = [C.c_uint64, l, l, v, i, d, d, v, v, v, v, v]
-        _lib.orc_py_band_tables.argtypes = [v, i, v, v]
-        _lib.orc_py_nelmin_test.argtypes = [v, i, i, v, v, v, v, d, i, i, v, v, v, v, v, v]
-        _lib.orc_py_interp.restype = d
-        _lib.orc_py_interp.argtypes = [v, v, i, d]
-        _lib.orc_py_approx_equal.restype = i
-        _lib.orc_py_approx_equal.argtypes = [C.c_float, C.c_float, C.c_float]
-        _lib.orc_py_philox.argtypes = [v, v, v]
-        _lib.orc_py_mlp_forward.argtypes = [v, i, v, v]
-        _lib.orc_py_mlp_loss_grad.restype = d
-        _lib.orc_py_mlp_loss_grad.argtypes = [v, i, v, v, v]
-        _lib.orc_py_mlp_nparam.restype = i
-    return _lib
-
-
-def ref():
-    """The reference's compiled asa047.c + common.c, or None when not built."""
-    global _ref
-    if _ref is None and os.path.exists(REF):
-        _ref = C.CDLL(REF)
-        _ref.interp_1d.restype = C.c_double
-        _ref.interp_1d.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double]
-        _ref.approx_equal.restype = C.c_int
-        _ref.approx_equal.argtypes = [C.c_float, C.c_float, C.c_float]
-    return _ref
-
-
-def ref_nelmin_ptr():
-    r = ref()
-    if r is None:
-        return None
-    return C.cast(r.nelmin, C.c_void_p).value
-
-
-def invert(rrs, sec_sun=None, mode=1, n_bottoms=2, bottom_idx=(0, 1, 2), wl=WL4, reqmin=1e-10,
-           konvge=100, kcount=2000, hstart=(40.0, 30.0, 20.0, 15.0, 10.0, 7.5, 2.5, 1.0),
-           early_exit=0.0, nodata=-9999.0, use_ref_nm=False, seed=None,
-           seed_rel=(0.3, 0.0, 1.0, 1.0, 1.0), seed_min=(0.5, 0.25, 0.1, 0.1, 0.02),
-           sec_sun0=1.0):
-    """Oracle inversion of single-date pixels; rrs [nb][n] float32."""
-    L = lib()
-    rrs = np.ascontiguousarray(rrs, np.float32)
-    nb, n = rrs.shape
-    npar = 5 if mode == 1 else 1 + 2 * n_bottoms + 3
-    out = {
-        "x": np.zeros((npar, n)), "depth": np.zeros(n, np.float32),
-        "rrs_error": np.zeros(n, np.float32), "model_error": np.zeros(n, np.float32),
-        "k_min": np.zeros(n, np.float32), "iod": np.zeros(n, np.float32),
-        "albedo": np.zeros(n, np.float32), "bottom_type": np.zeros(n, np.float32),
-        "b_pct": np.zeros((3, n), np.float32), "n_iter": np.zeros(n, np.int32),
-        "converged": np.zeros(n, np.int32), "n_evals": np.zeros(n, np.int32),
-        "n_starts": np.zeros(n, np.int32), "status": np.zeros(n, np.int32),
-    }
-    o = PyOut(**{k: v.ctypes.data for k, v in out.items()})
-    wl = np.asarray(wl, np.float64)
-    hs = np.asarray(hstart, np.float64)
-    bi = np.asarray(bottom_idx, np.int32)
-    ss = None if sec_sun is None else np.ascontiguousarray(sec_sun, np.float32)
-    sd = None if seed is None else np.ascontiguousarray(seed, np.float32)
-    sr = np.asarray(seed_rel, np.float64)
-    sm = np.asarray(seed_min, np.float64)
-    nm = ref_nelmin_ptr() if use_ref_nm else None
-    if use_ref_nm and nm is None:
-        raise RuntimeError("oracle/_ref not built")
-    L.orc_py_invert(mode, n_bottoms, bi.ctypes.data, wl.ctypes.data, nb, n, rrs.ctypes.data,
-                    _p(ss), sec_sun0, reqmin, konvge, kcount, len(hs), hs.ctypes.data, early_exit,
-                    nodata, nm, _p(sd), sr.ctypes.data, sm.ctypes.data, C.byref(o))
-    return out
-
-
-def forward(params, sec_sun, Y, wl=WL4):
-    params = np.ascontiguousarray(params, np.float64)
-    n = params.shape[0]
-    out = np.zeros((n, len(wl)))
-    w = np.asarray(wl, np.float64)
-    lib().orc_py_forward(w.ctypes.data, len(wl), n, params.ctypes.data,
-                         np.ascontiguousarray(sec_sun, np.float64).ctypes.data,
-                         np.ascontiguousarray(Y, np.float64).ctypes.data, out.ctypes.data)
-    return out
-
-
-def objective(x, rrs, sec_sun, wl=WL4):
-    x = np.ascontiguousarray(x, np.float64)
-    n = x.shape[0]
-    out, err = np.zeros(n), np.zeros(n)
-    w = np.asarray(wl, np.float64)
-    r = np.ascontiguousarray(rrs, np.float32)
-    s = np.ascontiguousarray(sec_sun, np.float32)
-    lib().orc_py_objective(w.ctypes.data, len(wl), n, x.ctypes.data, r.ctypes.data, s.ctypes.data,
-                           out.ctypes.data, err.ctypes.data)
-    return out, err
-
-
-def gen(seed, n, pix0=0, sigma=2e-4, deep_frac=0.0, wl=WL4):
-    nb = len(wl)
-    rrs = np.zeros((nb, n), np.float32)
-    ss = np.zeros(n, np.float32)
-    truth = np.zeros((6, n))
-    nodata = np.zeros(n, np.int32)
-    Y = np.zeros(n)
-    w = np.asarray(wl, np.float64)
-    lib().orc_py_gen(seed, pix0, n, w.ctypes.data, nb, sigma, deep_frac, rrs.ctypes.data,
-                     ss.ctypes.data, truth.ctypes.data, nodata.ctypes.data, Y.ctypes.data)
-    return rrs, ss, truth, nodata, Y
-
-
-def band_tables(wl=WL4):
-    nb = len(wl)
-    out = np.zeros((12, nb))
-    aw640 = C.c_double()
-    w = np.asarray(wl, np.float64)
-    lib().orc_py_band_tables(w.ctypes.data, nb, out.ctypes.data, C.byref(aw640))
-    return out, aw640.value
-
-
-def nelmin_test(kind, start, step, w=None, c=None, reqmin=1e-10, konvge=10, kcount=5000,
-                use_ref=False):
-    n = len(start)
-    st = np.array(start, np.float64)
-    sp = np.array(step, np.float64)
-    xmin = np.zeros(n)
-    yn = C.c_double()
-    ic, nr, ifa = C.c_int(), C.c_int(), C.c_int()
-    ev = C.c_long()
-    wv = None if w is None else np.asarray(w, np.float64)
-    cv = None if c is None else np.asarray(c, np.float64)
-    nm = ref_nelmin_ptr() if use_ref else None
-    lib().orc_py_nelmin_test(nm, kind, n, _p(wv), _p(cv), st.ctypes.data, sp.ctypes.data, reqmin,
-                             konvge, kcount, xmin.ctypes.data, C.byref(yn), C.byref(ic),
-                             C.byref(nr), C.byref(ifa), C.byref(ev))
-    return dict(xmin=xmin, ynewlo=yn.value, icount=ic.value, numres=nr.value, ifault=ifa.value,
-                evals=ev.value, start_after=st)
-
-
-def hostsim_invert(cfg_const, n_bottoms, rrs, sec_sun=None, seed=None):
-    """The product's lane engine on the host (tests/hostsim)."""
-    if not os.path.exists(HOSTSIM):
-        raise RuntimeError("host build of the engine missing: tests/hostsim/build.sh")
-    h = C.CDLL(HOSTSIM)
-    rrs = np.ascontiguousarray(rrs, np.float32)
-    nb, n = rrs.shape
-    npar = 5 if n_bottoms == 0 else 1 + 2 * n_bottoms + 3
-    out = {"x": np.zeros((npar, n)), "depth": np.zeros(n, np.float32),
-           "rrs_error": np.zeros(n, np.float32), "model_error": np.zeros(n, np.float32),
-           "status": np.zeros(n, np.int32), "n_evals": np.zeros(n, np.int32),
-           "n_iter": np.zeros(n, np.int32), "n_starts": np.zeros(n, np.int32)}
-    ss = None if sec_sun is None else np.ascontiguousarray(sec_sun, np.float32)
-    sd = None if seed is None else np.ascontiguousarray(seed, np.float32)
-    h.phs_invert.argtypes = [C.c_void_p, C.c_int, C.c_long] + [C.c_void_p] * 11
-    rc = h.phs_invert(C.cast(cfg_const, C.c_void_p), n_bottoms, n, rrs.ctypes.data, _p(ss), _p(sd),
-                      *[out[k].ctypes.data for k in ("x", "depth", "rrs_error", "model_error",
-                                                      "status", "n_evals", "n_iter", "n_starts")])
-    assert rc == 0
-    return out
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                "oracle"))
+from oracle_bindings import *  # noqa: E402,F401,F403
+from oracle_bindings import lib, ref, ref_nelmin_ptr  # noqa: E402,F401
