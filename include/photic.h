@@ -122,6 +122,39 @@ int photic_objective(const photic_cfg *cfg, long n, const double *x, const float
                      const float *sec_sun, double *out, double *rrs_error, int precision,
                      void *stream);
 
+/* ---- the learned warm start --------------------------------------------------------
+ * A 5-128-128-128-5 GELU network (BF16 operands, FP32 accumulation on the matrix cores)
+ * mapping ln Rrs at the four bands and the sun zenith to a starting point (H, B, P, G, X)
+ * for the simplex.  Inputs x = ((ln max(Rrs, 1e-4) + 4) / 1.5 per band, (theta_deg - 35) / 15);
+ * targets t = ((ln H - 1.25), (ln P + 3.45) / 1.85, (ln G + 3.91) / 2.30, (ln X + 4.95) / 1.96,
+ * 2 B - 1).  Parameters are FP32 masters in the layout W1[128][5] b1 W2[128][128] b2
+ * W3[128][128] b3 W4[5][128] b4 (34,437 values) with a packed BF16 copy for the kernels. */
+typedef struct photic_mlp photic_mlp;
+
+int photic_mlp_create(photic_mlp **mlp, uint64_t init_seed);
+void photic_mlp_destroy(photic_mlp *mlp);
+int photic_mlp_n_params(void);
+int photic_mlp_set_weights(photic_mlp *mlp, const float *host_w);
+int photic_mlp_get_weights(photic_mlp *mlp, float *host_w);
+/* Training pairs (device x, t: [n][5]) drawn from the scene generator's distributions
+ * (single date, configured bands), pixels pix0 .. pix0 + n - 1 of seed. */
+int photic_mlp_trainset(const photic_cfg *cfg, uint64_t seed, long pix0, long n, double sigma,
+                        float *x, float *t, void *stream);
+/* Mean-squared-error loss and its gradient (host_grad: photic_mlp_n_params() floats) over
+ * n samples (a multiple of 16). */
+int photic_mlp_gradient(photic_mlp *mlp, const float *x, const float *t, long n,
+                        float *host_grad, float *loss);
+/* Adam (lr decaying geometrically to lr_final) over `epochs` passes of n samples in
+ * mini-batches of `batch` (a multiple of 16); *final_loss = the last epoch's mean loss. */
+int photic_mlp_train(photic_mlp *mlp, const float *x, const float *t, long n, long batch,
+                     int epochs, float lr, float lr_final, float *final_loss, void *stream);
+/* Raw network outputs y [n][5] for inputs x [n][5]. */
+int photic_mlp_forward(photic_mlp *mlp, const float *x, long n, float *y, void *stream);
+/* Simplex seeds [5][n] in the MIX layout (H, B, 100 P, 100 G, 100 X) for a single-date scene
+ * (rrs [4][n], sec_sun [n] or NULL for cfg->sec_sun). */
+int photic_mlp_seed(photic_mlp *mlp, const photic_cfg *cfg, long n, const float *rrs,
+                    const float *sec_sun, float *seed, void *stream);
+
 /* Launch-shape facts of the compiled inversion kernel for cfg. */
 int photic_kernel_info(const photic_cfg *cfg, int *vgprs, int *scratch_bytes,
                        int *blocks_per_cu, int *n_cu);

@@ -101,6 +101,17 @@ def _load():
         "photic_event_record": (i, [vp, vp]),
         "photic_event_elapsed_ms": (i, [vp, vp, C.POINTER(C.c_float)]),
         "photic_last_error": (C.c_char_p, []),
+        "photic_mlp_create": (i, [C.POINTER(vp), C.c_uint64]),
+        "photic_mlp_destroy": (None, [vp]),
+        "photic_mlp_n_params": (i, []),
+        "photic_mlp_set_weights": (i, [vp, vp]),
+        "photic_mlp_get_weights": (i, [vp, vp]),
+        "photic_mlp_trainset": (i, [C.POINTER(Cfg), C.c_uint64, l, l, d, vp, vp, vp]),
+        "photic_mlp_gradient": (i, [vp, vp, vp, l, vp, C.POINTER(C.c_float)]),
+        "photic_mlp_train": (i, [vp, vp, vp, l, l, i, C.c_float, C.c_float,
+                                 C.POINTER(C.c_float), vp]),
+        "photic_mlp_forward": (i, [vp, vp, l, vp, vp]),
+        "photic_mlp_seed": (i, [vp, C.POINTER(Cfg), l, vp, vp, vp, vp]),
         "ph_build_const": (i, [C.POINTER(Cfg), i, i, vp]),
         "ph_const_size": (i, []),
     }
@@ -295,3 +306,69 @@ def const_bytes(cfg, n_dates=1, has_seed=False):
     buf = C.create_string_buffer(lib.ph_const_size())
     check(lib.ph_build_const(C.byref(cfg), n_dates, int(has_seed), buf))
     return buf
+
+
+class Mlp:
+    """The warm-start network on the device (photic_mlp_* of include/photic.h)."""
+
+    def __init__(self, init_seed=1):
+        self.h = C.c_void_p()
+        check(lib.photic_mlp_create(C.byref(self.h), init_seed))
+        self.n_params = lib.photic_mlp_n_params()
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib.photic_mlp_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.n_params
+        check(lib.photic_mlp_set_weights(self.h, w.ctypes.data))
+
+    def get_weights(self):
+        w = np.empty(self.n_params, np.float32)
+        check(lib.photic_mlp_get_weights(self.h, w.ctypes.data))
+        return w
+
+    def gradient(self, x, t):
+        """Loss and gradient over host samples x, t [n][5] (n a multiple of 16)."""
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        bx, bt = DeviceBuffer.from_array(x), DeviceBuffer.from_array(t)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        check(lib.photic_mlp_gradient(self.h, bx.value, bt.value, x.shape[0], g.ctypes.data,
+                                      C.byref(loss)))
+        return loss.value, g
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        bx, by = DeviceBuffer.from_array(x), DeviceBuffer(x.nbytes)
+        check(lib.photic_mlp_forward(self.h, bx.value, x.shape[0], by.value, None))
+        check(lib.photic_synchronize())
+        return by.to_array(np.float32, x.shape)
+
+    def train_device(self, d_x, d_t, n, batch=8192, epochs=1, lr=1e-3, lr_final=1e-4,
+                     stream=None):
+        loss = C.c_float()
+        check(lib.photic_mlp_train(self.h, d_x, d_t, n, batch, epochs, lr, lr_final,
+                                   C.byref(loss), stream))
+        return loss.value
+
+    def seed_device(self, cfg, n, d_rrs, d_sec_sun, d_seed, stream=None):
+        check(lib.photic_mlp_seed(self.h, C.byref(cfg), n, d_rrs, d_sec_sun, d_seed, stream))
+
+
+def trainset_device(cfg, seed, n, pix0=0, sigma=2e-4):
+    """Normalised training pairs on the device: (DeviceBuffer x [n][5], DeviceBuffer t [n][5])."""
+    dx, dt = DeviceBuffer(4 * 5 * n), DeviceBuffer(4 * 5 * n)
+    check(lib.photic_mlp_trainset(C.byref(cfg), seed, pix0, n, sigma, dx.value, dt.value, None))
+    check(lib.photic_synchronize())
+    return dx, dt

@@ -26,7 +26,7 @@ CC_FLAGS = [
     "-std=c11", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-D__HIP_PLATFORM_AMD__",
     "-I" + os.path.join(ROCM, "include"), "-I" + CSRC, "-I" + os.path.join(ROOT, "include"),
 ]
-HIP_SOURCES = ["ph_kernels.hip"]
+HIP_SOURCES = ["ph_kernels.hip", "ph_mlp.hip"]
 C_SOURCES = ["photic_capi.c"]
 
 

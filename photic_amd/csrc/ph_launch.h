@@ -78,6 +78,24 @@ int ph_kernel_attrs(int fp32, int n_bands, int n_bottoms, int *vgprs, int *sgprs
 int ph_kernel_occupancy(int fp32, int n_bands, int n_bottoms, int block, int *blocks_per_cu);
 int ph_lds_bytes(int n_bottoms, int block);
 
+/* the warm-start network (ph_mlp.hip) */
+int ph_mlp_nparam(void);
+int ph_mlp_packed_sizes(int *n_bf16, int *n_f32);
+long ph_mlp_gbuf_elems(long batch);
+int ph_launch_mlp_init(uint64_t seed, float *w, void *stream);
+int ph_launch_mlp_adam(float *w, float *m, float *v, const float *g, float lr, float b1,
+                       float b2, float eps, int step, unsigned short *pk, float *pb,
+                       void *stream);
+int ph_launch_mlp_fwdbwd(const unsigned short *pk, const float *pb, const float *x,
+                         const float *t, float *y, long n, float gscale, unsigned short *gbuf,
+                         float *loss, int grid, void *stream);
+int ph_launch_mlp_wgrad(const unsigned short *gbuf, long n, float *grad, void *stream);
+int ph_launch_mlp_trainset(const PhConst *c, uint64_t seed, long pix0, long n, double sigma,
+                           float *x, float *t, void *stream);
+int ph_launch_mlp_prep(const float *rrs, const float *ss, float ss0, long n, float *x,
+                       void *stream);
+int ph_launch_mlp_seed(const float *y, long n, float *seed, void *stream);
+
 /* Host side (photic_capi.c): the launch constants for a photic_cfg; exported for the test
  * suite's host build of the lane engine. */
 struct photic_cfg;

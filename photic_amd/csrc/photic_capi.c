@@ -11,6 +11,7 @@
 #include <hip/hip_runtime_api.h>
 #include <math.h>
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -401,6 +402,202 @@ int photic_objective(const photic_cfg *cfg, long n, const double *x, const float
     return ph_launch_status(ph_launch_objective(&c, n, x, rrs, sec_sun, out, rrs_error,
                                                 precision == PHOTIC_FP32, c.n_bands, stream),
                             "objective kernel");
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* the learned warm start                                                                 */
+/* ------------------------------------------------------------------------------------ */
+
+struct photic_mlp {
+    float *w, *m, *v, *g; /* FP32 masters, Adam moments, gradient */
+    unsigned short *pk;   /* packed BF16 weights */
+    float *pb;            /* packed FP32 biases */
+    float *loss;
+    int step;
+};
+
+int photic_mlp_n_params(void) { return ph_mlp_nparam(); }
+
+static int ph_mlp_pack(photic_mlp *m, void *stream)
+{
+    return ph_launch_status(ph_launch_mlp_adam(m->w, m->m, m->v, NULL, 0.f, 0.9f, 0.999f, 1e-8f, 1,
+                                               m->pk, m->pb, stream), "pack weights");
+}
+
+void photic_mlp_destroy(photic_mlp *m)
+{
+    if (!m) return;
+    hipFree(m->w);
+    hipFree(m->m);
+    hipFree(m->v);
+    hipFree(m->g);
+    hipFree(m->pk);
+    hipFree(m->pb);
+    hipFree(m->loss);
+    free(m);
+}
+
+int photic_mlp_create(photic_mlp **out, uint64_t init_seed)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    if (!out) return ph_fail(PHOTIC_EINVAL, "out is NULL");
+    photic_mlp *m = (photic_mlp *)calloc(1, sizeof(*m));
+    if (!m) return ph_fail(PHOTIC_ENOMEM, "host allocation");
+    int np = ph_mlp_nparam(), nbf, nf;
+    ph_mlp_packed_sizes(&nbf, &nf);
+    rc = ph_hip(hipMalloc((void **)&m->w, sizeof(float) * np), "hipMalloc");
+    if (!rc) rc = ph_hip(hipMalloc((void **)&m->m, sizeof(float) * np), "hipMalloc");
+    if (!rc) rc = ph_hip(hipMalloc((void **)&m->v, sizeof(float) * np), "hipMalloc");
+    if (!rc) rc = ph_hip(hipMalloc((void **)&m->g, sizeof(float) * np), "hipMalloc");
+    if (!rc) rc = ph_hip(hipMalloc((void **)&m->pk, sizeof(unsigned short) * nbf), "hipMalloc");
+    if (!rc) rc = ph_hip(hipMalloc((void **)&m->pb, sizeof(float) * nf), "hipMalloc");
+    if (!rc) rc = ph_hip(hipMalloc((void **)&m->loss, sizeof(float)), "hipMalloc");
+    if (!rc) rc = ph_hip(hipMemset(m->m, 0, sizeof(float) * np), "hipMemset");
+    if (!rc) rc = ph_hip(hipMemset(m->v, 0, sizeof(float) * np), "hipMemset");
+    if (!rc) rc = ph_hip(hipMemset(m->pk, 0, sizeof(unsigned short) * nbf), "hipMemset");
+    if (!rc) rc = ph_hip(hipMemset(m->pb, 0, sizeof(float) * nf), "hipMemset");
+    if (!rc) rc = ph_launch_status(ph_launch_mlp_init(init_seed, m->w, NULL), "init weights");
+    if (!rc) rc = ph_mlp_pack(m, NULL);
+    if (!rc) rc = ph_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (rc) {
+        photic_mlp_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return PHOTIC_OK;
+}
+
+int photic_mlp_set_weights(photic_mlp *m, const float *host_w)
+{
+    if (!m || !host_w) return ph_fail(PHOTIC_EINVAL, "NULL argument");
+    int rc = ph_hip(hipMemcpy(m->w, host_w, sizeof(float) * ph_mlp_nparam(), hipMemcpyHostToDevice),
+                    "hipMemcpy");
+    if (!rc) rc = ph_mlp_pack(m, NULL);
+    if (!rc) rc = ph_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return rc;
+}
+
+int photic_mlp_get_weights(photic_mlp *m, float *host_w)
+{
+    if (!m || !host_w) return ph_fail(PHOTIC_EINVAL, "NULL argument");
+    return ph_hip(hipMemcpy(host_w, m->w, sizeof(float) * ph_mlp_nparam(), hipMemcpyDeviceToHost),
+                  "hipMemcpy");
+}
+
+int photic_mlp_trainset(const photic_cfg *cfg, uint64_t seed, long pix0, long n, double sigma,
+                        float *x, float *t, void *stream)
+{
+    PhConst c;
+    int rc = ph_have_device();
+    if (rc) return rc;
+    rc = ph_make_const(cfg, 1, 0, &c);
+    if (rc) return rc;
+    if (cfg->n_bands != 4) return ph_fail(PHOTIC_EUNSUPPORTED, "the network takes 4 bands");
+    return ph_launch_status(ph_launch_mlp_trainset(&c, seed, pix0, n, sigma, x, t, stream),
+                            "training set");
+}
+
+/* one step on samples [x, t) of n: gradient into m->g, loss into m->loss (accumulated) */
+static int ph_mlp_grad_step(photic_mlp *m, const float *x, const float *t, long n,
+                            unsigned short *gbuf, void *stream)
+{
+    float gscale = 2.0f / ((float)n * 5.0f);
+    int rc = ph_launch_status(ph_launch_mlp_fwdbwd(m->pk, m->pb, x, t, NULL, n, gscale, gbuf,
+                                                   m->loss, 0, stream), "network forward/backward");
+    if (!rc) rc = ph_launch_status(ph_launch_mlp_wgrad(gbuf, n, m->g, stream), "weight gradients");
+    return rc;
+}
+
+int photic_mlp_gradient(photic_mlp *m, const float *x, const float *t, long n, float *host_grad,
+                        float *loss)
+{
+    if (!m || n <= 0 || n % 16) return ph_fail(PHOTIC_EINVAL, "n must be a positive multiple of 16");
+    unsigned short *gbuf = NULL;
+    int rc = ph_hip(hipMalloc((void **)&gbuf, sizeof(unsigned short) * ph_mlp_gbuf_elems(n)), "hipMalloc");
+    if (rc) return rc;
+    rc = ph_hip(hipMemset(m->loss, 0, sizeof(float)), "hipMemset");
+    if (!rc) rc = ph_hip(hipMemset(m->g, 0, sizeof(float) * ph_mlp_nparam()), "hipMemset");
+    if (!rc) rc = ph_mlp_grad_step(m, x, t, n, gbuf, NULL);
+    if (!rc) rc = ph_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (!rc && host_grad)
+        rc = ph_hip(hipMemcpy(host_grad, m->g, sizeof(float) * ph_mlp_nparam(), hipMemcpyDeviceToHost),
+                    "hipMemcpy");
+    if (!rc && loss) {
+        float l = 0.f;
+        rc = ph_hip(hipMemcpy(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+        *loss = l / ((float)n * 5.0f);
+    }
+    hipFree(gbuf);
+    return rc;
+}
+
+int photic_mlp_train(photic_mlp *m, const float *x, const float *t, long n, long batch,
+                     int epochs, float lr, float lr_final, float *final_loss, void *stream)
+{
+    if (!m || batch <= 0 || batch % 16 || n < batch || epochs < 1)
+        return ph_fail(PHOTIC_EINVAL, "need 16 | batch <= n and epochs >= 1");
+    unsigned short *gbuf = NULL;
+    int rc = ph_hip(hipMalloc((void **)&gbuf, sizeof(unsigned short) * ph_mlp_gbuf_elems(batch)),
+                    "hipMalloc");
+    if (rc) return rc;
+    const long steps_per_epoch = n / batch;
+    const long total = steps_per_epoch * epochs;
+    const double decay = (lr_final > 0.f && lr > 0.f) ? pow((double)lr_final / lr, 1.0 / (double)total) : 1.0;
+    double cur = lr;
+    float last = 0.f;
+    for (int e = 0; e < epochs && !rc; e++) {
+        rc = ph_hip(hipMemsetAsync(m->loss, 0, sizeof(float), (hipStream_t)stream), "hipMemsetAsync");
+        for (long k = 0; k < steps_per_epoch && !rc; k++) {
+            const float *xb = x + k * batch * 5, *tb = t + k * batch * 5;
+            rc = ph_mlp_grad_step(m, xb, tb, batch, gbuf, stream);
+            if (!rc) {
+                m->step++;
+                rc = ph_launch_status(ph_launch_mlp_adam(m->w, m->m, m->v, m->g, (float)cur, 0.9f,
+                                                         0.999f, 1e-8f, m->step, m->pk, m->pb,
+                                                         stream), "adam");
+            }
+            cur *= decay;
+        }
+        if (!rc) {
+            float l = 0.f;
+            rc = ph_hip(hipMemcpyAsync(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost,
+                                       (hipStream_t)stream), "hipMemcpyAsync");
+            if (!rc) rc = ph_hip(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+            last = l / ((float)steps_per_epoch * (float)batch * 5.0f);
+        }
+    }
+    if (final_loss) *final_loss = last;
+    hipFree(gbuf);
+    return rc;
+}
+
+int photic_mlp_forward(photic_mlp *m, const float *x, long n, float *y, void *stream)
+{
+    if (!m || n < 0) return ph_fail(PHOTIC_EINVAL, "bad arguments");
+    return ph_launch_status(ph_launch_mlp_fwdbwd(m->pk, m->pb, x, NULL, y, n, 0.f, NULL, NULL, 0,
+                                                 stream), "network forward");
+}
+
+int photic_mlp_seed(photic_mlp *m, const photic_cfg *cfg, long n, const float *rrs,
+                    const float *sec_sun, float *seed, void *stream)
+{
+    if (!m || !cfg || n < 0) return ph_fail(PHOTIC_EINVAL, "bad arguments");
+    if (cfg->n_bands != 4 || cfg->mode != PHOTIC_MODE_MIX)
+        return ph_fail(PHOTIC_EUNSUPPORTED, "seeds are made for 4 bands in MIX mode");
+    if (n == 0) return PHOTIC_OK;
+    hipStream_t st = (hipStream_t)stream;
+    float *xb = NULL, *yb = NULL;
+    int rc = ph_hip(hipMallocAsync((void **)&xb, sizeof(float) * 5 * n, st), "hipMallocAsync");
+    if (rc) return rc;
+    rc = ph_hip(hipMallocAsync((void **)&yb, sizeof(float) * 5 * n, st), "hipMallocAsync");
+    if (!rc) rc = ph_launch_status(ph_launch_mlp_prep(rrs, sec_sun, (float)cfg->sec_sun, n, xb, st),
+                                   "network inputs");
+    if (!rc) rc = photic_mlp_forward(m, xb, n, yb, st);
+    if (!rc) rc = ph_launch_status(ph_launch_mlp_seed(yb, n, seed, st), "seeds");
+    hipFreeAsync(xb, st);
+    if (yb) hipFreeAsync(yb, st);
+    return rc;
 }
 
 /* ------------------------------------------------------------------------------------ */
