@@ -75,6 +75,11 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-only", action="store_true", help="skip the warm-start network")
+    ap.add_argument("--no-equal-misfit", action="store_true")
+    ap.add_argument("--equal-misfit-pixels", type=int, default=1 << 24)
+    ap.add_argument("--train-samples", type=int, default=1 << 24)
+    ap.add_argument("--train-epochs", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X and the "
@@ -98,31 +103,102 @@ def main():
     P.check(P.lib.photic_gen_scene(C.byref(cfg), 4, 0, n, 2e-4, 0.0, d_rrs.value, d_ss.value,
                                    d_truth.value, stream))
     P.check(P.lib.photic_synchronize())
-    outs = P.DeviceOutputs(cfg, n, fields=("depth", "rrs_error", "status", "n_evals"))
+    truth_H = d_truth.to_array(np.float32, (6, n))[0]
+    del d_truth
+    fields = ("depth", "rrs_error", "status", "n_evals", "n_starts")
+
+    # ---- the warm-start network, trained once on forward-model samples (not timed) ----
+    net = None
+    train_info = None
+    if not args.plain_only:
+        t = time.perf_counter()
+        n_tr = args.train_samples
+        dx, dt = P.trainset_device(cfg, 101, n_tr)
+        net = P.Mlp(init_seed=1)
+        loss = net.train_device(dx.value, dt.value, n_tr, batch=8192, epochs=args.train_epochs,
+                                lr=2e-3, lr_final=1e-4, stream=stream)
+        P.check(P.lib.photic_synchronize())
+        train_info = {"samples": n_tr, "epochs": args.train_epochs, "final_loss": loss,
+                      "seconds": time.perf_counter() - t}
+        del dx, dt
+    d_seed = P.DeviceBuffer(4 * 5 * n) if net is not None else None
+
+    def scene_slice(n_sub):
+        """The first n_sub pixels as their own planar arrays ([4][n_sub], [n_sub])."""
+        if n_sub == n:
+            return d_rrs, d_ss
+        r = P.DeviceBuffer(4 * 4 * n_sub)
+        sv = P.DeviceBuffer(4 * n_sub)
+        for b in range(4):
+            P.check(P.lib.photic_device_copy(r.value + 4 * b * n_sub, d_rrs.value + 4 * b * n,
+                                             4 * n_sub, stream))
+        P.check(P.lib.photic_device_copy(sv.value, d_ss.value, 4 * n_sub, stream))
+        P.check(P.lib.photic_synchronize())
+        return r, sv
+
+    def run(kind, cfg_run, n_run):
+        """One step of `kind` over the first n_run pixels; returns (wall ms, kernel ms,
+        outputs)."""
+        r, sv = scene_slice(n_run)
+        outs = P.DeviceOutputs(cfg_run, n_run, fields=fields)
+        P.check(P.lib.photic_synchronize())
+        t0 = time.perf_counter()
+        if kind == "seeded":
+            net.seed_device(cfg_run, n_run, r.value, sv.value, d_seed.value, stream=stream)
+        k_ms = timed_invert(P, cfg_run, n_run, r.value, sv.value,
+                            d_seed.value if kind == "seeded" else None, outs, stream)
+        P.check(P.lib.photic_synchronize())
+        return 1e3 * (time.perf_counter() - t0), k_ms, outs.fetch()
+
+    def summary(ms, res, n_run):
+        v = res["status"] >= 0
+        dH = res["depth"][v].astype(np.float64) - truth_H[:n_run][v]
+        return {
+            "pixels_per_s": n_run / (ms / 1e3),
+            "ms": ms,
+            "evals_per_pixel": float(res["n_evals"][v].mean()),
+            "starts_per_pixel": float(res["n_starts"][v].mean()),
+            "converged_frac": float((res["status"][v] == 0).mean()),
+            "median_rrs_error_pct": float(np.median(res["rrs_error"][v])),
+            "depth_rmse_m": float(np.sqrt(np.mean(dH * dH))),
+            "valid_pixels": int(v.sum()),
+        }
 
     # ---- warm-up on a slice ----
-    n_w = min(n, 1 << 20)
-    outs_w = P.DeviceOutputs(cfg, n_w, fields=("depth", "status"))
+    n_w = min(n, 1 << 20)  # a multiple of 4, so the slices stay 16-byte aligned
     for _ in range(args.warmup):
-        timed_invert(P, cfg, n_w, d_rrs.value, d_ss.value, None, outs_w, stream)
+        run("plain", cfg, n_w)
+        if net is not None:
+            run("seeded", cfg, n_w)
 
-    # ---- timed: K whole-scene plain inversions ----
-    P.check(P.lib.photic_synchronize())
-    t0 = time.perf_counter()
-    kernel_ms = []
+    # ---- timed: K whole-scene steps of the headline method ----
+    head = "plain" if net is None else "seeded"
+    walls, kms = [], []
+    res = None
     for _ in range(args.steps):
-        kernel_ms.append(timed_invert(P, cfg, n, d_rrs.value, d_ss.value, None, outs, stream))
-    P.check(P.lib.photic_synchronize())
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
-
-    res = outs.fetch()
-    valid = res["status"] >= 0
-    evals = float(res["n_evals"][valid].astype(np.int64).sum())
-    truth_h = d_truth.to_array(np.float32, (6, n))
-    dH = res["depth"][valid].astype(np.float64) - truth_h[0][valid]
+        w_ms, k_ms, res = run(head, cfg, n)
+        walls.append(w_ms)
+        kms.append(k_ms)
+    ms_per_step = float(np.mean(walls))
+    head_sum = summary(ms_per_step, res, n)
+    evals = float(res["n_evals"][res["status"] >= 0].astype(np.int64).sum())
+    k_ms = float(np.mean(kms))
     px_s = n / (ms_per_step / 1e3)
-    k_ms = float(np.mean(kernel_ms))
+
+    # ---- the plain GPU Nelder-Mead on the same scene, beside it ----
+    comparisons = {}
+    if net is not None:
+        w_ms, _, pres = run("plain", cfg, n)
+        comparisons["plain"] = summary(w_ms, pres, n)
+        if not args.no_equal_misfit:
+            # plain at equal misfit: the reference's starts without its early exit (every
+            # start run, best kept) -- what the plain method needs to reach the seeded misfit
+            cfg_eq = P.cfg_default(WL, reqmin=1e-10, konvge=100, kcount=2000, precision=prec,
+                                   watchdog_s=900.0, early_exit=1e-300)
+            n_eq = min(n, args.equal_misfit_pixels)
+            w_ms, _, eres = run("plain", cfg_eq, n_eq)
+            comparisons["plain_all_starts"] = summary(w_ms, eres, n_eq)
+            comparisons["plain_all_starts"]["pixels"] = n_eq
     achieved = evals * FLOP_PER_EVAL / (k_ms / 1e3) / 1e12
     # HBM bytes per launch: the per-pixel FETCH_SIZE / WRITE_SIZE of the same kernel from
     # rocprofv3 --pmc, each calibrated by a copy kernel of known size (profiles/)
@@ -132,6 +208,8 @@ def main():
         with open(tj) as f:
             t = json.load(f)
         traffic = (t["fetch_bytes_per_pixel"] + t["write_bytes_per_pixel"]) * n
+        if head == "seeded":
+            traffic += 20.0 * n  # the five seed floats each pixel reads
 
     line = {
         "metric": METRIC,
@@ -145,22 +223,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if prec == P.FP64 else "f64+f32-transcendentals",
-        "data": "synthetic (Philox-seeded scene, no real imagery)",
+        "data": "synthetic (Philox-seeded scene and training samples, no real imagery)",
         "config": {
-            "workload": "plain GPU Nelder-Mead, %dx%d single-date scene, 4 bands, 5 params, "
-                        "reqmin 1e-10 kcount 2000" % (args.side, args.side),
+            "workload": "%s inversion, %dx%d single-date scene, 4 bands, 5 params (H, B, P, G, X), "
+                        "reqmin 1e-10 kcount 2000 konvge 100" % (
+                            "MLP-seeded Nelder-Mead with plain fallback" if head == "seeded"
+                            else "plain GPU Nelder-Mead", args.side, args.side),
             "pixels": n, "seed": 4, "noise_sigma": 2e-4,
             "parallelism": "1 GPU, persistent work-queue kernel",
-            "mlp_seeded": None,
         },
-        "plain": {
-            "pixels_per_s": px_s,
-            "evals_per_pixel": evals / max(valid.sum(), 1),
-            "converged_frac": float((res["status"][valid] == 0).mean()),
-            "median_rrs_error_pct": float(np.median(res["rrs_error"][valid])),
-            "depth_rmse_m": float(np.sqrt(np.mean(dH * dH))),
-            "valid_pixels": int(valid.sum()),
-        },
+        head: head_sum,
         "roofline": {
             "bound": "valu",
             "achieved": achieved,
@@ -175,6 +247,13 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if train_info:
+        line["mlp_training"] = train_info
+    line.update(comparisons)
+    if "plain" in comparisons:
+        line["seeded_over_plain"] = px_s / comparisons["plain"]["pixels_per_s"]
+    if "plain_all_starts" in comparisons:
+        line["seeded_over_plain_equal_misfit"] = px_s / comparisons["plain_all_starts"]["pixels_per_s"]
 
     if not args.no_cpu:
         ns = min(args.cpu_sample, n)
@@ -187,7 +266,11 @@ def main():
                       "schedule, %.1f s" % (ns, dt),
         }
         line["gpu_over_cpu"] = px_s / cpx
+        if "plain" in line:
+            line["plain_gpu_over_cpu"] = line["plain"]["pixels_per_s"] / cpx
 
+    if net is not None:
+        net.close()
     P.lib.photic_stream_destroy(stream)
     print(json.dumps(line))
 

@@ -105,6 +105,13 @@ int photic_invert(const photic_cfg *cfg, long n_pix, const float *rrs, const flo
                   const float *sec_view, const float *seed, const photic_outputs *out,
                   void *stream);
 
+/* As photic_invert, with one depth start per pixel (hstart [n_pix], metres) in place of the
+ * configured list: the reference's mode when an empirical depth grid is given
+ * (model/samodel.c:2208-2231). */
+int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const float *sec_sun,
+                    const float *sec_view, const float *seed, const float *hstart,
+                    const photic_outputs *out, void *stream);
+
 /* Synthetic single-date scene (Philox4x32-10 keyed by seed, counter = pixel index): draws
  * H ~ U(0.5, 25) (deep_frac of pixels U(30, 60)), log-normal P, G, X, B ~ U(0, 1), sun zenith
  * ~ U(20, 50) deg, FP64 forward model with self-consistent Y, N(0, sigma) noise.  rrs:

@@ -179,11 +179,14 @@ def nelmin_test(kind, start, step, w=None, c=None, reqmin=1e-10, konvge=10, kcou
                 evals=ev.value, start_after=st)
 
 
-def hostsim_invert(cfg_const, n_bottoms, rrs, sec_sun=None, seed=None):
-    """The product's lane engine on the host (tests/hostsim)."""
-    if not os.path.exists(HOSTSIM):
+def hostsim_invert(cfg_const, n_bottoms, rrs, sec_sun=None, seed=None, math="libm"):
+    """The product's lane engine on the host (tests/hostsim).  math="libm": exp/log/pow from
+    the host C library, as in the oracle; math="cr": the correctly rounded functions the GPU
+    kernels use."""
+    path = HOSTSIM if math == "libm" else HOSTSIM.replace("libph_hostsim", "libph_hostsim_cr")
+    if not os.path.exists(path):
         raise RuntimeError("host build of the engine missing: tests/hostsim/build.sh")
-    h = C.CDLL(HOSTSIM)
+    h = C.CDLL(path)
     rrs = np.ascontiguousarray(rrs, np.float32)
     nb, n = rrs.shape
     npar = 5 if n_bottoms == 0 else 1 + 2 * n_bottoms + 3

@@ -27,7 +27,7 @@ CC_FLAGS = [
     "-I" + os.path.join(ROCM, "include"), "-I" + CSRC, "-I" + os.path.join(ROOT, "include"),
 ]
 HIP_SOURCES = ["ph_kernels.hip", "ph_mlp.hip"]
-C_SOURCES = ["photic_capi.c"]
+C_SOURCES = ["photic_capi.c", "photic_samodel.c"]
 
 
 def _run(cmd):

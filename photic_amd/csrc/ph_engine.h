@@ -20,7 +20,21 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ph_crmath.h"
 #include "ph_launch.h"
+
+// Transcendentals of the exact (FP64) path: correctly rounded, ph_crmath.h.  The host test
+// build can substitute the C library's (PH_HOST_LIBM) to check the state machine against the
+// oracle bit for bit with identical arithmetic on both sides.
+#if defined(PH_HOST_LIBM) && !defined(__HIP_DEVICE_COMPILE__)
+#define PH_EXP exp
+#define PH_LOG log
+#define PH_POW pow
+#else
+#define PH_EXP ph_exp_cr
+#define PH_LOG ph_log_cr
+#define PH_POW ph_pow_cr
+#endif
 
 // Every engine function is inlined into the kernel: the lane state must stay in registers,
 // and a real call would force it through scratch memory.
@@ -129,7 +143,7 @@ PH_INLINE double ph_band64(double P, double G, double X, double rho, double H,
                                             double eg, double pw, double ss, double sv,
                                             double* K_out, double* rB_out, double* r_out)
 {
-    double a_phi = (a0 + a1 * log(fabs(P))) * fabs(P);
+    double a_phi = (a0 + a1 * PH_LOG(fabs(P))) * fabs(P);
     double a_g = fabs(G) * eg;
     double a = aw + a_phi + a_g;
     double b_p = X * pw;
@@ -142,9 +156,9 @@ PH_INLINE double ph_band64(double P, double G, double X, double rho, double H,
     double DuC = 1.03 * sqrt(1.0 + 2.4 * u);
     double DuB = 1.04 * sqrt(1.0 + 5.4 * u);
     double M = ss + DuC * sv;
-    double rC = rrs_dp * (1.0 - exp(-M * K * H));
+    double rC = rrs_dp * (1.0 - PH_EXP(-M * K * H));
     M = ss + DuB * sv;
-    double rB = rho / 3.141592653589793 * exp(-M * K * H);
+    double rB = rho / 3.141592653589793 * PH_EXP(-M * K * H);
     double r = rC + rB;
     *K_out = K;
     *rB_out = rB;
@@ -375,11 +389,11 @@ PH_INLINE bool ph_pixel_init(PhPixel<FT, ND, NB>& px, const PhConst& c,
         double r490 = ph_rrs_at(c, r, 1);
         if (r440 < 0.0) r440 = 0.0001;
         double chi = r440 / r490;
-        double Y = 3.44 * (1.0 - 3.17 * exp(-2.01 * chi));
+        double Y = 3.44 * (1.0 - 3.17 * PH_EXP(-2.01 * chi));
         if (Y < 0.0) Y = 0.0;
         if (Y > 2.5) Y = 2.5;
 #pragma unroll
-        for (int b = 0; b < NB; b++) px.pw[s][b] = pow(440.0 / c.wl[b], Y);
+        for (int b = 0; b < NB; b++) px.pw[s][b] = PH_POW(440.0 / c.wl[b], Y);
         px.ss[s] = ss[s];
         px.sv[s] = sv[s];
     }
@@ -438,7 +452,8 @@ struct PhNM {
 // Start point and steps of depth start k (k < 0: the seed), model/samodel.c:2246-2354.
 template <typename FT, int ND, int NB, int NBOT>
 PH_INLINE void ph_start_step(PhNM<PhLayout<NBOT>::template n<ND>()>& S, const PhConst& c,
-                             const PhPixel<FT, ND, NB>& px, int k, const double* seed)
+                             const PhPixel<FT, ND, NB>& px, int k, const double* seed,
+                             double h_override = -1.0)
 {
     using L = PhLayout<NBOT>;
     constexpr int N = L::template n<ND>();
@@ -457,6 +472,7 @@ PH_INLINE void ph_start_step(PhNM<PhLayout<NBOT>::template n<ND>()>& S, const Ph
         constexpr int m = m0 + 1;
         h0 = ph_pick(k == m, c.hstart[m], h0);
     });
+    if (h_override >= 0.0) h0 = h_override; // the pixel's own depth start
     S.xs[0] = h0;
     S.m.set_step(0, 1.25 * S.xs[0]);
     if (NBOT == 0) {
@@ -480,7 +496,7 @@ PH_INLINE void ph_start_step(PhNM<PhLayout<NBOT>::template n<ND>()>& S, const Ph
         double m490 = ph_rrs_at(c, px.rrs[s], 1) / (double)1;
         double m550 = ph_rrs_at(c, px.rrs[s], 2) / (double)1;
         double m640 = ph_rrs_at(c, px.rrs[s], 3) / (double)1;
-        S.xs[o] = 100.0 * 0.072 * pow(m490 / m550, -1.7);
+        S.xs[o] = 100.0 * 0.072 * PH_POW(m490 / m550, -1.7);
         S.xs[o + 1] = 1.5 * S.xs[o];
         S.xs[o + 2] = 100.0 * 30.0 * c.aw640 * m640;
         S.m.set_step(o, 2.0 * S.xs[o]);

@@ -203,7 +203,8 @@ __global__ void __launch_bounds__(256) ph_invert_kernel(PhInvertArgs a)
 #pragma unroll
                 for (int i = 0; i < N; i++) sd[i] = (double)a.seed[(long)i * a.n_pix + pix];
             }
-            ph_start_step<FT, 1, NB, NBOT>(S, c, px, S.kstart, sd);
+            const double hov = a.hstart ? (double)a.hstart[pix] : -1.0;
+            ph_start_step<FT, 1, NB, NBOT>(S, c, px, S.kstart, sd, hov);
             ph_nm_begin(S, c);
         }
 
@@ -246,7 +247,7 @@ __global__ void ph_fwd_kernel(const PhConst c, long n, const double* params,
     double ss = sec_sun[i], Y = Yv[i];
     for (int b = 0; b < c.n_bands; b++) {
         double rho = B * c.bot[0][b] + (1.0 - B) * c.bot[1][b];
-        double pw = pow(440.0 / c.wl[b], Y);
+        double pw = PH_POW(440.0 / c.wl[b], Y);
         if constexpr (sizeof(FT) == 8) {
             double K, rB, r;
             out[i * c.n_bands + b] = ph_band64(P, G, X, rho, H, c.a0[b], c.a1[b], c.aw[b],

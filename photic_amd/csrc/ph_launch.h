@@ -48,6 +48,7 @@ typedef struct PhInvertArgs {
     const float *sec_sun;  /* [n_pix] or NULL (nadir sun) */
     const float *sec_view; /* [n_pix] or NULL (nadir view) */
     const float *seed;     /* [n_params][n_pix] when c.has_seed */
+    const float *hstart;   /* [n_pix]: one depth start per pixel (c.n_hstarts == 1), or NULL */
     unsigned long long *queue; /* [0] work counter, [1] watchdog flag; zero at launch */
     int chunk;
     unsigned long long max_ticks; /* watchdog: wall-clock ticks before every wave gives up */

@@ -263,6 +263,13 @@ int photic_invert(const photic_cfg *cfg, long n_pix, const float *rrs, const flo
                   const float *sec_view, const float *seed, const photic_outputs *out,
                   void *stream)
 {
+    return photic_invert_h(cfg, n_pix, rrs, sec_sun, sec_view, seed, NULL, out, stream);
+}
+
+int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const float *sec_sun,
+                    const float *sec_view, const float *seed, const float *hstart,
+                    const photic_outputs *out, void *stream)
+{
     PhInvertArgs a;
     int rc = ph_have_device();
     if (rc) return rc;
@@ -270,7 +277,14 @@ int photic_invert(const photic_cfg *cfg, long n_pix, const float *rrs, const flo
     if (!out) return ph_fail(PHOTIC_EINVAL, "out is NULL");
     if (n_pix > 0 && !rrs) return ph_fail(PHOTIC_EINVAL, "rrs is NULL");
     memset(&a, 0, sizeof(a));
-    rc = ph_make_const(cfg, 1, seed != NULL, &a.c);
+    if (hstart) {
+        /* one depth start per pixel: the reference's empirical-depth mode (samodel.c:2223) */
+        photic_cfg one = *cfg;
+        one.n_hstarts = 1;
+        rc = ph_make_const(&one, 1, seed != NULL, &a.c);
+    } else {
+        rc = ph_make_const(cfg, 1, seed != NULL, &a.c);
+    }
     if (rc) return rc;
     if (n_pix == 0) return PHOTIC_OK;
 
@@ -311,6 +325,7 @@ int photic_invert(const photic_cfg *cfg, long n_pix, const float *rrs, const flo
     a.sec_sun = sec_sun;
     a.sec_view = sec_view;
     a.seed = seed;
+    a.hstart = hstart;
     a.chunk = cfg->chunk > 0 ? cfg->chunk : 64;
     a.depth = out->depth;
     a.rrs_error = out->rrs_error;

@@ -1,0 +1,237 @@
+/* photic_amd -- C host code: the drop-in samodel() of include/photic_samodel.h.
+ *
+ * Grid in, grid out, like model/samodel.c:289-1763: the caller's row-pointer band grids are
+ * screened for no-data (model/samodel.c:933-947), smoothed as model/common.c:232-276 does
+ * when n_smoothing_radius > 1, packed into planar pixel arrays per depth-start group, inverted
+ * on the GPU by photic_invert / photic_invert_h, and the outputs scattered back into the
+ * caller's grids with the reference's defaults and sign conventions. */
+#include "photic_samodel.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "photic.h"
+
+static int sm_feq(float a, float b, float eps)
+{
+    float d = a - b;
+    double fa = fabs((double)a), fb = fabs((double)b);
+    return fabs((double)d) <= ((fa < fb ? fb : fa) * (double)eps);
+}
+
+/* smoothed_index_2d (model/common.c:232-276): (2r-1)^2 box mean with clamp-to-edge indexing,
+ * skipping no-data; all no-data gives the no-data value. */
+static float sm_smoothed(float **a, int i, int j, int nrows, int ncols, int r, float spval)
+{
+    float sum = 0.0f, nobs = 0.0f;
+    for (int ii = 1 - r; ii < r; ii++) {
+        int iii = i + ii < 0 ? 0 : (i + ii > nrows - 1 ? nrows - 1 : i + ii);
+        for (int jj = 1 - r; jj < r; jj++) {
+            int jjj = j + jj < 0 ? 0 : (j + jj > ncols - 1 ? ncols - 1 : j + jj);
+            float v = a[iii][jjj];
+            if (!sm_feq(v, spval, 1.0e-6f)) {
+                sum += v;
+                nobs += 1.0f;
+            }
+        }
+    }
+    return nobs < 0.5f ? spval : sum / nobs;
+}
+
+typedef struct {
+    long n;
+    long *idx;      /* pixel index i * ncols + j */
+    float *rrs;     /* [4][n] */
+    float *hstart;  /* [n] or NULL */
+} sm_group;
+
+static int sm_run_group(const photic_cfg *cfg, sm_group *g, int ncols, float **depth,
+                        float **model_error, float **bottom_albedo, float **bottom_sand,
+                        float **bottom_seagrass, float **bottom_coral, float **K_min,
+                        float **bottom_type, float **index_optical_depth)
+{
+    if (g->n == 0) return PHOTIC_OK;
+    const long n = g->n;
+    float *d_rrs = NULL, *d_h = NULL, *d_out = NULL;
+    int rc = photic_device_malloc((void **)&d_rrs, sizeof(float) * 4 * n);
+    if (!rc && g->hstart) rc = photic_device_malloc((void **)&d_h, sizeof(float) * n);
+    /* outputs: depth, rrs_error, albedo, k_min, iod, bottom_type, b_pct[3] */
+    if (!rc) rc = photic_device_malloc((void **)&d_out, sizeof(float) * 9 * n);
+    if (!rc) rc = photic_memcpy_htod(d_rrs, g->rrs, sizeof(float) * 4 * n);
+    if (!rc && d_h) rc = photic_memcpy_htod(d_h, g->hstart, sizeof(float) * n);
+    if (!rc) {
+        photic_outputs o;
+        memset(&o, 0, sizeof(o));
+        o.depth = d_out;
+        o.rrs_error = d_out + n;
+        o.albedo = d_out + 2 * n;
+        o.k_min = d_out + 3 * n;
+        o.iod = d_out + 4 * n;
+        o.bottom_type = d_out + 5 * n;
+        o.b_pct = d_out + 6 * n;
+        rc = photic_invert_h(cfg, n, d_rrs, NULL, NULL, NULL, d_h, &o, NULL);
+        if (!rc) rc = photic_synchronize();
+    }
+    float *h = NULL;
+    if (!rc) {
+        h = (float *)malloc(sizeof(float) * 9 * n);
+        if (!h) rc = PHOTIC_ENOMEM;
+    }
+    if (!rc) rc = photic_memcpy_dtoh(h, d_out, sizeof(float) * 9 * n);
+    if (!rc) {
+        for (long p = 0; p < n; p++) {
+            long k = g->idx[p];
+            int i = (int)(k / ncols), j = (int)(k % ncols);
+            depth[i][j] = -h[p]; /* negative below the surface, model/samodel.c:1486-1490 */
+            model_error[i][j] = h[n + p];
+            bottom_albedo[i][j] = h[2 * n + p];
+            K_min[i][j] = h[3 * n + p];
+            index_optical_depth[i][j] = h[4 * n + p];
+            bottom_type[i][j] = h[5 * n + p];
+            bottom_sand[i][j] = h[6 * n + p];
+            bottom_seagrass[i][j] = h[7 * n + p];
+            bottom_coral[i][j] = h[8 * n + p];
+        }
+    }
+    free(h);
+    photic_device_free(d_rrs);
+    if (d_h) photic_device_free(d_h);
+    if (d_out) photic_device_free(d_out);
+    return rc;
+}
+
+int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridded_data[],
+                   int *scene_indexes, int nscenes, int empirical_depth_present,
+                   struct photic_geogrid empirical_depths, int n_smoothing_radius,
+                   int n_spatial, int n_bottoms, float **depth, float **depth_sigma,
+                   float **model_error, float **bottom_albedo, float **bottom_sand,
+                   float **bottom_seagrass, float **bottom_coral, float **K_min,
+                   float **bottom_type, float **index_optical_depth)
+{
+    if (!scene_data || !gridded_data || !scene_indexes || nscenes < 1) return PHOTIC_EINVAL;
+    const struct photic_scene *sc = &scene_data[scene_indexes[0]];
+    const int nrows = sc->nrows, ncols = sc->ncols, nb = sc->n_bands;
+
+    /* the reference's defaults for every pixel (model/samodel.c:819-841) */
+    for (int i = 0; i < nrows; i++)
+        for (int j = 0; j < ncols; j++) {
+            depth[i][j] = 0.0f;
+            if (depth_sigma) depth_sigma[i][j] = 0.0f;
+            model_error[i][j] = 0.0f;
+            bottom_albedo[i][j] = 0.0f;
+            bottom_sand[i][j] = -9999.0f;
+            bottom_seagrass[i][j] = -9999.0f;
+            bottom_coral[i][j] = -9999.0f;
+            K_min[i][j] = 0.0f;
+            bottom_type[i][j] = -9999.0f;
+            index_optical_depth[i][j] = 0.0f;
+        }
+    if (nscenes != 1 || n_spatial > 1 || nb != 4 || n_bottoms < 1 || n_bottoms > 3) {
+        fprintf(stderr, "photic samodel: supported are one scene, one region (n_spatial <= 1), "
+                        "4 bands and 1..3 bottoms (got %d scenes, n_spatial %d, %d bands, %d "
+                        "bottoms); outputs left at their defaults\n",
+                nscenes, n_spatial, nb, n_bottoms);
+        return PHOTIC_EUNSUPPORTED;
+    }
+    const int r = n_smoothing_radius < 1 ? 1 : n_smoothing_radius;
+
+    double wl[4];
+    for (int b = 0; b < 4; b++) wl[b] = (double)sc->wavelengths[b];
+    photic_cfg cfg;
+    photic_cfg_default(&cfg, 4, wl); /* the reference's Nelder-Mead settings */
+    cfg.mode = PHOTIC_MODE_NATIVE;
+    cfg.n_bottoms = n_bottoms;
+    cfg.nodata = -1.0f; /* screened below: any no-data band arrives negative */
+    {
+        double t = sc->theta_w;
+        t *= 3.141592653589793 / 180.0;
+        cfg.sec_sun = 1.0 / cos(t);
+        t = sc->theta_v;
+        t *= 3.141592653589793 / 180.0;
+        cfg.sec_view = 1.0 / cos(t);
+    }
+
+    /* groups: 0 = no empirical depth (all depth starts), 1 = empirical depth <= 8 m, 2 =
+     * empirical depth > 8 m (sand only), model/samodel.c:1826, :2208-2231, :960-976 */
+    long npix = (long)nrows * ncols;
+    sm_group gr[3];
+    memset(gr, 0, sizeof(gr));
+    int rc = PHOTIC_OK;
+    for (int k = 0; k < 3 && !rc; k++) {
+        gr[k].idx = (long *)malloc(sizeof(long) * (size_t)(npix ? npix : 1));
+        gr[k].rrs = (float *)malloc(sizeof(float) * 4 * (size_t)(npix ? npix : 1));
+        gr[k].hstart = k ? (float *)malloc(sizeof(float) * (size_t)(npix ? npix : 1)) : NULL;
+        if (!gr[k].idx || !gr[k].rrs || (k && !gr[k].hstart)) rc = PHOTIC_ENOMEM;
+    }
+    /* pass 1: pixel screening and gather, per group (compacted later per group) */
+    for (int i = 0; i < nrows && !rc; i++)
+        for (int j = 0; j < ncols; j++) {
+            int bad = 0;
+            float v[4];
+            for (int b = 0; b < 4; b++) {
+                const struct photic_geogrid *g = &gridded_data[sc->band_indexes[b]];
+                float refl = g->array[i][j];
+                if (sm_feq(refl, g->nodata_value, 1.0e-6f) || refl < 0.0f) bad = 1;
+                float s = r > 1 ? sm_smoothed(g->array, i, j, nrows, ncols, r, g->nodata_value) : refl;
+                if (sm_feq(s, g->nodata_value, 1.0e-6f)) bad = 1; /* region dropped */
+                v[b] = s;
+            }
+            if (bad) continue;
+            int k = 0;
+            float h = 0.0f;
+            if (empirical_depth_present &&
+                !sm_feq(empirical_depths.array[i][j], empirical_depths.nodata_value, 1.0e-6f)) {
+                float e = empirical_depths.array[i][j];
+                h = e > -1.0f ? 1.0f : fabsf(e);
+                k = h > 8.0 ? 2 : 1;
+            }
+            sm_group *g = &gr[k];
+            g->idx[g->n] = (long)i * ncols + j;
+            for (int b = 0; b < 4; b++) g->rrs[(long)b * npix + g->n] = v[b];
+            if (k) g->hstart[g->n] = h;
+            g->n++;
+        }
+    /* compact the planar band rows to n per group */
+    for (int k = 0; k < 3 && !rc; k++)
+        for (int b = 1; b < 4; b++)
+            memmove(gr[k].rrs + (long)b * gr[k].n, gr[k].rrs + (long)b * npix,
+                    sizeof(float) * (size_t)gr[k].n);
+
+    for (int k = 0; k < 3 && !rc; k++) {
+        photic_cfg c = cfg;
+        if (k == 2) {
+            c.n_bottoms = 1;
+            c.bottom_idx[0] = 0; /* sand */
+        }
+        rc = sm_run_group(&c, &gr[k], ncols, depth, model_error, bottom_albedo, bottom_sand,
+                          bottom_seagrass, bottom_coral, K_min, bottom_type,
+                          index_optical_depth);
+    }
+    for (int k = 0; k < 3; k++) {
+        free(gr[k].idx);
+        free(gr[k].rrs);
+        free(gr[k].hstart);
+    }
+    if (rc)
+        fprintf(stderr, "photic samodel: inversion failed (%d): %s\n", rc, photic_last_error());
+    return rc;
+}
+
+void samodel(struct photic_scene scene_data[], struct photic_geogrid gridded_data[],
+             int *scene_indexes, int nscenes, int empirical_depth_present,
+             struct photic_geogrid empirical_depths, int n_smoothing_radius, int n_spatial,
+             int n_bottoms, float **depth, float **depth_sigma, float **model_error,
+             float **bottom_albedo, float **bottom_sand, float **bottom_seagrass,
+             float **bottom_coral, float **K_min, float **bottom_type,
+             float **index_optical_depth, float pagesize, int background, int linewidth)
+{
+    (void)pagesize;
+    (void)background;
+    (void)linewidth;
+    photic_samodel(scene_data, gridded_data, scene_indexes, nscenes, empirical_depth_present,
+                   empirical_depths, n_smoothing_radius, n_spatial, n_bottoms, depth, depth_sigma,
+                   model_error, bottom_albedo, bottom_sand, bottom_seagrass, bottom_coral, K_min,
+                   bottom_type, index_optical_depth);
+}

@@ -1,0 +1,119 @@
+"""The drop-in samodel() (include/photic_samodel.h) on the GPU, called with the reference's
+structures and row-pointer grids, against the FP64 oracle's native-mode inversion with the
+reference's Nelder-Mead settings, pixel by pixel."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import photic_amd as P
+from samodel_types import Geogrid, RowGrid, Scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _call(bands, nodata, theta_w, n_bottoms, emp=None, emp_nodata=-99999.0, smoothing=1):
+    nb, nrows, ncols = bands.shape
+    grids = [RowGrid(bands[b]) for b in range(nb)]
+    gd = (Geogrid * (nb + 1))()
+    for b in range(nb):
+        gd[b].nrows, gd[b].ncols, gd[b].nodata_value = nrows, ncols, nodata
+        gd[b].array = grids[b].pp
+    scenes = (Scene * 1)()
+    sc = scenes[0]
+    sc.n_bands, sc.nrows, sc.ncols = nb, nrows, ncols
+    for b, wl in enumerate((443, 482, 561, 655)):
+        sc.band_indexes[b] = b
+        sc.wavelengths[b] = wl
+    sc.theta_w, sc.theta_v = theta_w, 0.0
+    idx = (C.c_int * 1)(0)
+    empg = Geogrid()
+    keep = None
+    if emp is not None:
+        keep = RowGrid(emp)
+        empg.nrows, empg.ncols, empg.nodata_value, empg.array = nrows, ncols, emp_nodata, keep.pp
+    outs = {k: RowGrid(np.zeros((nrows, ncols))) for k in
+            ("depth", "sigma", "err", "alb", "sand", "grass", "coral", "kmin", "btype", "iod")}
+    P.lib.photic_samodel.restype = C.c_int
+    rc = P.lib.photic_samodel(scenes, gd, idx, 1, int(emp is not None), empg, smoothing, 1,
+                              n_bottoms, *[outs[k].pp for k in ("depth", "sigma", "err", "alb",
+                                                                 "sand", "grass", "coral", "kmin",
+                                                                 "btype", "iod")])
+    return rc, {k: v.array() for k, v in outs.items()}
+
+
+def _scene(seed, nrows, ncols):
+    rrs, ss, truth, nodata, _ = O.gen(seed, nrows * ncols)
+    return rrs.reshape(4, nrows, ncols), truth
+
+
+@pytest.mark.parametrize("nbot", [1, 2, 3])
+def test_samodel_dropin_matches_oracle(nbot):
+    nrows, ncols = 24, 40
+    bands, truth = _scene(21 + nbot, nrows, ncols)
+    bands[:, 3, 5] = -9999.0           # nodata on every band
+    bands[2, 7, 7] = -9999.0           # nodata on one band
+    bands[1, 9, 11] = -0.0005          # negative reflectance
+    rc, out = _call(bands, -9999.0, 35.0, nbot)
+    assert rc == 0
+    ss = 1.0 / np.cos(35.0 * np.pi / 180.0)
+    o = O.invert(bands.reshape(4, -1), None, mode=0, n_bottoms=nbot, reqmin=1e-2, konvge=100,
+                 kcount=5000, sec_sun0=ss)
+    depth = -out["depth"].reshape(-1)
+    valid = o["status"] >= 0
+    assert np.array_equal(out["btype"].reshape(-1) != -9999.0, valid)
+    for (i, j) in ((3, 5), (7, 7), (9, 11)):
+        assert out["depth"][i, j] == 0 and out["sand"][i, j] == -9999.0
+    ok = np.abs(depth[valid] - o["depth"][valid]) < 0.01
+    err = out["err"].reshape(-1)[valid]
+    flat = np.abs(err - o["rrs_error"][valid]) <= 1e-6 * np.abs(o["rrs_error"][valid])
+    print("nbot %d: %d valid pixels, %.4f with depth within 1 cm of the oracle, identical %.4f" %
+          (nbot, valid.sum(), ok.mean(), (depth[valid] == o["depth"][valid]).mean()))
+    # outside 1 cm only where the misfit is flat in depth (both end points fit equally well)
+    assert np.all(ok | flat) and ok.mean() > 0.99
+    same = depth[valid] == o["depth"][valid]
+    for k_out, k_or in (("err", "rrs_error"), ("kmin", "k_min"), ("iod", "iod"),
+                        ("alb", "albedo"), ("btype", "bottom_type")):
+        a = out[k_out].reshape(-1)[valid][same]
+        b = o[k_or][valid][same]
+        assert np.allclose(a, b, rtol=1e-5, atol=1e-6), k_out
+    assert np.allclose(out["sand"].reshape(-1)[valid][same], o["b_pct"][0][valid][same], rtol=1e-5)
+
+
+def test_samodel_dropin_empirical_depths():
+    nrows, ncols = 16, 32
+    bands, truth = _scene(31, nrows, ncols)
+    rng = np.random.default_rng(5)
+    emp = -rng.uniform(0.2, 20.0, (nrows, ncols)).astype(np.float32)
+    emp[0, :8] = -99999.0    # no empirical depth here: all depth starts
+    emp[1, :8] = -0.5        # > -1: "very shallow", start at 1 m
+    rc, out = _call(bands, -9999.0, 30.0, 2, emp=emp)
+    assert rc == 0
+    ss = 1.0 / np.cos(30.0 * np.pi / 180.0)
+    flat = bands.reshape(4, -1)
+    e = emp.reshape(-1)
+    depth = -out["depth"].reshape(-1)
+    agree = []
+    for p in range(flat.shape[1]):
+        if e[p] == -99999.0:
+            o = O.invert(flat[:, p:p + 1], None, mode=0, n_bottoms=2, reqmin=1e-2, kcount=5000,
+                         sec_sun0=ss)
+        else:
+            h = 1.0 if e[p] > -1.0 else abs(e[p])
+            nbot = 1 if h > 8.0 else 2
+            o = O.invert(flat[:, p:p + 1], None, mode=0, n_bottoms=nbot, bottom_idx=(0, 1, 2),
+                         reqmin=1e-2, kcount=5000, hstart=(float(np.float32(h)),), sec_sun0=ss)
+        if o["status"][0] >= 0:
+            same_fit = abs(out["err"].reshape(-1)[p] - o["rrs_error"][0]) <= 1e-6 * o["rrs_error"][0]
+            agree.append((abs(depth[p] - o["depth"][0]) < 0.01, same_fit))
+    agree = np.array(agree)
+    print("empirical: %d pixels, %.4f within 1 cm" % (len(agree), agree[:, 0].mean()))
+    assert np.all(agree[:, 0] | agree[:, 1]) and agree[:, 0].mean() > 0.99
+
+
+def test_samodel_rejects_unsupported():
+    bands, _ = _scene(41, 4, 4)
+    rc, out = _call(bands, -9999.0, 30.0, 4)
+    assert rc == P.EUNSUPPORTED
+    assert np.all(out["btype"] == -9999.0)
