@@ -162,6 +162,12 @@ int photic_mlp_forward(photic_mlp *mlp, const float *x, long n, float *y, void *
 int photic_mlp_seed(photic_mlp *mlp, const photic_cfg *cfg, long n, const float *rrs,
                     const float *sec_sun, float *seed, void *stream);
 
+/* The band coefficients the kernels use for cfg's wavelengths, interpolated from the
+ * reference's 41-node tables (model/samodel.c:103-286) as model/samodel.c:560-618 does:
+ * out [12][n_bands] = a0, a1, a_w, b_bw, then the 8 bottom reflectances in the order of
+ * model/samodel.c:466-478; *aw640 = a_w(640 nm) (may be NULL).  Host only, no GPU needed. */
+int photic_band_coefficients(const photic_cfg *cfg, double *out, double *aw640);
+
 /* Launch-shape facts of the compiled inversion kernel for cfg. */
 int photic_kernel_info(const photic_cfg *cfg, int *vgprs, int *scratch_bytes,
                        int *blocks_per_cu, int *n_cu);

@@ -20,7 +20,8 @@ def _p(a):
 class PyOut(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("x", "depth", "rrs_error", "model_error", "k_min", "iod",
                                           "albedo", "bottom_type", "b_pct", "n_iter",
-                                          "converged", "n_evals", "n_starts", "status")]
+                                          "converged", "n_evals", "n_starts", "status",
+                                          "misfit")]
 
 
 _lib = None
@@ -41,6 +42,10 @@ def lib():
         _lib.orc_py_objective.argtypes = [v, i, l, v, v, v, v, v]
         _lib.orc_py_gen.argtypes = [C.c_uint64, l, l, v, i, d, d, v, v, v, v, v]
         _lib.orc_py_band_tables.argtypes = [v, i, v, v]
+        _lib.orc_py_start_step.argtypes = [i, i, v, i, v, d, v, v]
+        _lib.orc_py_start_step.restype = i
+        _lib.orc_py_Y.argtypes = [d]
+        _lib.orc_py_Y.restype = d
         _lib.orc_py_nelmin_test.argtypes = [v, i, i, v, v, v, v, d, i, i, v, v, v, v, v, v]
         _lib.orc_py_interp.restype = d
         _lib.orc_py_interp.argtypes = [v, v, i, d]
@@ -97,6 +102,7 @@ def invert(rrs, sec_sun=None, mode=1, n_bottoms=2, bottom_idx=(0, 1, 2), wl=WL4,
         "b_pct": np.zeros((3, n), np.float32), "n_iter": np.zeros(n, np.int32),
         "converged": np.zeros(n, np.int32), "n_evals": np.zeros(n, np.int32),
         "n_starts": np.zeros(n, np.int32), "status": np.zeros(n, np.int32),
+        "misfit": np.zeros(n),
     }
     o = PyOut(**{k: v.ctypes.data for k, v in out.items()})
     wl = np.asarray(wl, np.float64)
@@ -202,3 +208,24 @@ def hostsim_invert(cfg_const, n_bottoms, rrs, sec_sun=None, seed=None, math="lib
                                                       "status", "n_evals", "n_iter", "n_starts")])
     assert rc == 0
     return out
+
+
+def Y_of_chi(chi):
+    return lib().orc_py_Y(chi)
+
+
+def start_step(rrs, h0, mode=1, n_bottoms=2, wl=WL4):
+    """Start point and steps of one depth start for a single-date pixel (rrs [nb])."""
+    r = np.ascontiguousarray(rrs, np.float64)
+    w = np.asarray(wl, np.float64)
+    start, step = np.zeros(64), np.zeros(64)
+    n = lib().orc_py_start_step(mode, n_bottoms, w.ctypes.data, len(w), r.ctypes.data, h0,
+                                start.ctypes.data, step.ctypes.data)
+    return start[:n], step[:n]
+
+
+def table(name):
+    """One of the oracle's 41-node tables: ref_wl, aw, bbw, a0, a1, or bottom (8 x 41)."""
+    if name == "bottom":
+        return np.ctypeslib.as_array((C.c_double * (8 * 41)).in_dll(lib(), "orc_bottom")).reshape(8, 41).copy()
+    return np.ctypeslib.as_array((C.c_double * 41).in_dll(lib(), "orc_" + name)).copy()

@@ -16,6 +16,7 @@ typedef struct {
     float *depth, *rrs_error, *model_error, *k_min, *iod, *albedo, *bottom_type;
     float *b_pct;     /* [3][n] */
     int *n_iter, *converged, *n_evals, *n_starts, *status;
+    double *misfit;   /* objective at the best point, in double */
 } orc_py_out;
 
 /* Invert n single-date pixels exactly as the product's single-date path is specified. */
@@ -87,6 +88,7 @@ long orc_py_invert(int mode, int n_bottoms, const int *bottom_idx, const double 
         if (o->n_evals) o->n_evals[p] = (int)res.n_evals;
         if (o->n_starts) o->n_starts[p] = res.n_starts;
         if (o->status) o->status[p] = res.converged ? 0 : 2;
+        if (o->misfit) o->misfit[p] = res.model_error;
         done++;
     }
     return done;
@@ -267,4 +269,21 @@ int orc_py_set_threads(int n)
     (void)n;
     return 1;
 #endif
+}
+
+/* Y(chi) of model/samodel.c:2898-2901. */
+double orc_py_Y(double chi) { return orc_Y_of_chi(chi); }
+
+/* Start point and steps of one depth start (model/samodel.c:2246-2354) for a single-date
+ * pixel: mode, n_bottoms (native), wavelengths wl [nb], Rrs rrs [nb]; start, step [n]. */
+int orc_py_start_step(int mode, int n_bottoms, const double *wl, int nb, const double *rrs,
+                      double h0, double *start, double *step)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    orc_pixel px;
+    double ss = 1.0, sv = 1.0;
+    orc_pixel_init(&px, mode, n_bottoms, NULL, 1, &bands, &ss, &sv, rrs);
+    orc_start_step(&px, h0, start, step);
+    return orc_n_params(&px);
 }

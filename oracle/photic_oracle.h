@@ -10,17 +10,22 @@
  * Pinning (what makes this oracle trustworthy, each check lives in tests/):
  *   - orc_nelmin             bit-identical to the reference's own nelmin (asa047.c, compiled
  *                            into oracle/_ref) on the same objective: xmin, ynewlo, icount,
- *                            numres, ifault.
+ *                            numres, ifault (tests/test_oracle_ref.py).
  *   - orc_interp_1d,         bit-identical to the reference's compiled common.c.
  *     orc_approx_equal
- *   - band tables            equal to the numbers in the reference's text (golden fixture).
- *   - orc_rrs_band           closed forms: H=0 -> r = rho/pi; H large -> r = r_dp; X=0 ->
- *                            b_b = b_bw; lambda = 440 nm -> a = a_w + P + G, b_bp = X; Y and
- *                            K clamps; hand-computed values at those points.
- *   - orc_objective          hand-computed one- and two-band cases, K-penalty branches.
- *   - orc_optimise           start/step rules checked against hand-computed values; the
- *                            multi-start loop run with the reference nelmin must give the
- *                            same result as with orc_nelmin.
+ *   - band tables            equal to the numbers in the reference's text (golden fixture
+ *                            tests/golden/reference_tables.json, tests/test_oracle_forward.py).
+ *   - orc_rrs_core           closed forms: H = 0 -> r = rho/pi; deep pure water -> r = r_dp
+ *                            with u = b_bw/(a_w+b_bw); each constituent alone (a_phi with a0
+ *                            and a1, a_g's slope, b_bp's (440/lambda)^Y); affine in rho; the
+ *                            attenuation coefficients and the K clamp from dr/dH at H = 0.
+ *   - orc_Y_of_chi           clamps and a hand value.
+ *   - orc_objective          E_Rrs's definition and the 80/120 weight; every K-penalty tier
+ *                            and the K_min > 0.7 term against hand-computed values.
+ *   - orc_start_step         hand-computed start points and steps, both parameter layouts.
+ *   - orc_optimise           equal to a brute-force restatement from single-start runs (order,
+ *                            early exit at 2.5, first minimum kept); with the reference's
+ *                            nelmin plugged in it gives the same results as with orc_nelmin.
  *   - generator              Philox4x32-10 known-answer vectors (Salmon et al. 2011).
  *   - MLP                    finite differences on the FP64 forward.
  * samodel.c itself cannot be built in this image (it needs netcdf.h and cpgplot.h), so the

@@ -112,6 +112,9 @@ def _load():
                                  C.POINTER(C.c_float), vp]),
         "photic_mlp_forward": (i, [vp, vp, l, vp, vp]),
         "photic_mlp_seed": (i, [vp, C.POINTER(Cfg), l, vp, vp, vp, vp]),
+        "photic_invert_h": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, vp, C.POINTER(Outputs),
+                                vp]),
+        "photic_band_coefficients": (i, [C.POINTER(Cfg), vp, C.POINTER(d)]),
         "ph_build_const": (i, [C.POINTER(Cfg), i, i, vp]),
         "ph_const_size": (i, []),
     }
@@ -293,6 +296,14 @@ def objective(cfg, x, rrs, sec_sun, precision=FP64):
                                precision, None))
     check(lib.photic_synchronize())
     return bo.to_array(np.float64, (n,)), be.to_array(np.float64, (n,))
+
+
+def band_coefficients(cfg):
+    """(coefficients [12][n_bands]: a0, a1, a_w, b_bw, 8 bottoms; a_w(640)) for cfg."""
+    out = np.zeros((12, cfg.n_bands))
+    aw640 = C.c_double()
+    check(lib.photic_band_coefficients(C.byref(cfg), out.ctypes.data, C.byref(aw640)))
+    return out, aw640.value
 
 
 def kernel_info(cfg):

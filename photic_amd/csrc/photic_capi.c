@@ -230,6 +230,24 @@ int ph_build_const(const struct photic_cfg *cfg, int n_dates, int has_seed, PhCo
 
 int ph_const_size(void) { return (int)sizeof(PhConst); }
 
+int photic_band_coefficients(const photic_cfg *cfg, double *out, double *aw640)
+{
+    PhConst c;
+    int rc = ph_make_const(cfg, 1, 0, &c);
+    if (rc) return rc;
+    if (!out) return ph_fail(PHOTIC_EINVAL, "out is NULL");
+    const int nb = c.n_bands;
+    for (int b = 0; b < nb; b++) {
+        out[0 * nb + b] = c.a0[b];
+        out[1 * nb + b] = c.a1[b];
+        out[2 * nb + b] = c.aw[b];
+        out[3 * nb + b] = c.bbw[b];
+        for (int k = 0; k < PH_NBOT; k++) out[(4 + k) * nb + b] = c.bot[k][b];
+    }
+    if (aw640) *aw640 = c.aw640;
+    return PHOTIC_OK;
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* inversion                                                                              */
 /* ------------------------------------------------------------------------------------ */

@@ -1,0 +1,52 @@
+"""The C-ABI library: it loads without a GPU, exports every function include/*.h declares, and
+carries nothing of the oracle."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import photic_amd as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in ("photic.h", "photic_samodel.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \t\*]*?\b(\w+)\s*\(", text, flags=re.M):
+            if not m.group(0).lstrip().startswith(("typedef", "#")):
+                names.add(m.group(1))
+    return names
+
+
+def test_every_declared_function_is_exported():
+    names = _declared()
+    assert {"samodel", "photic_samodel", "photic_invert", "photic_invert_h", "photic_forward",
+            "photic_objective", "photic_mlp_train", "photic_band_coefficients"} <= names
+    lib = C.CDLL(P.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_library_holds_no_oracle_code():
+    out = subprocess.run(["nm", "-D", "--defined-only", P.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    assert " orc_" not in out
+    deps = subprocess.run(["ldd", P.LIB_PATH], capture_output=True, text=True).stdout
+    assert "oracle" not in deps
+
+
+def test_entry_points_fail_cleanly_without_a_gpu():
+    """With no GPU every computing entry point returns PHOTIC_EDEVICE (no CPU fallback); on a
+    GPU box this only checks that the status is a defined code."""
+    cfg = P.cfg_default()
+    rc = P.lib.photic_invert(C.byref(cfg), 0, None, None, None, None, None, None)
+    assert rc in (P.OK, P.EINVAL, P.EDEVICE)
+    if P.device_count() == 0:
+        out = P.Outputs()
+        rc = P.lib.photic_invert(C.byref(cfg), 16, C.c_void_p(16), None, None, None,
+                                 C.byref(out), None)
+        assert rc == P.EDEVICE
+        assert b"GPU" in P.lib.photic_last_error()
