@@ -117,8 +117,14 @@ __device__ inline void ph_write_nodata(const PhInvertArgs& a, long pix, int n_pa
     if (a.n_starts) a.n_starts[pix] = 0;
 }
 
+// Waves per SIMD the kernel is compiled for: two for the five-parameter mix (its lane state
+// fits 256 registers and its simplexes 80 KB of LDS per block, so two blocks share a CU), one
+// for the native layouts, whose larger simplexes fill the LDS with one block.
+template <int NBOT>
+constexpr int ph_invert_waves() { return NBOT == 0 ? 2 : 1; }
+
 template <typename FT, int NB, int NBOT>
-__global__ void __launch_bounds__(256) ph_invert_kernel(PhInvertArgs a)
+__global__ void __launch_bounds__(256, ph_invert_waves<NBOT>()) ph_invert_kernel(PhInvertArgs a)
 {
     using L = PhLayout<NBOT>;
     constexpr int N = L::template n<1>();
@@ -209,9 +215,9 @@ __global__ void __launch_bounds__(256) ph_invert_kernel(PhInvertArgs a)
         }
 
         // ---- one evaluation per lane, in lock-step ----
-        // A pass in which some lane makes its final re-evaluation also keeps the side
-        // quantities the outputs need; every other pass evaluates the misfit alone.
-        if (__ballot(S.phase == PH_FINAL) != 0ull) {
+        // The side quantities the outputs need are kept on every evaluation: one inlined copy
+        // of the objective keeps the kernel within 256 registers.
+        {
             PhSide<1, NB> side;
             double f = 0.0;
             if (S.phase != PH_IDLE) f = ph_objective<FT, 1, NB, NBOT, true>(S.xe, c, px, &side);
@@ -222,11 +228,6 @@ __global__ void __launch_bounds__(256) ph_invert_kernel(PhInvertArgs a)
                     S.phase = PH_IDLE;
                 }
             }
-        } else {
-            double f = 0.0;
-            if (S.phase != PH_IDLE)
-                f = ph_objective<FT, 1, NB, NBOT, false>(S.xe, c, px, nullptr);
-            if (S.phase != PH_IDLE) ph_nm_consume(S, c, f);
         }
     }
 }
