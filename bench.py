@@ -6,13 +6,15 @@ Gaussian noise 2e-4 1/sr), Nelder-Mead over the five parameters (H, B, P, G, X) 
 reqmin 1e-10, konvge 100, kcount 2000 and the reference's depth starts with early exit.
 A step is one inversion of the whole scene with inputs already resident in HBM.  With the
 warm-start network available the headline is the MLP-seeded inversion (network inference +
-seeded Nelder-Mead + plain fallback); the plain GPU Nelder-Mead is timed on the same scene
-beside it.
+seeded Nelder-Mead + plain fallback); beside it, on the same scene, the plain GPU Nelder-Mead
+with the reference's early exit, and the plain method at equal median misfit (its early exit
+lowered step by step until its median misfit is at or below the seeded one, on 2^22 pixels).
 
 Prints one JSON line.  Besides the contract's fields it carries the roofline of the dominant
 kernel (algorithmic FLOPs k(46 n_b + 15) + 6 = 205 per objective evaluation over the kernel's
-duration from HIP events on its stream) and the CPU baseline (the FP64 oracle, OpenMP over
-pixels, timed on a bounded sample of the same scene on this host's cores).
+duration from HIP events on its stream; HBM bytes per launch from the profiler counters in
+profiles/) and the CPU baseline (the FP64 oracle, OpenMP over pixels, timed on a bounded
+sample of the same scene on this host's cores).
 """
 import argparse
 import ctypes as C
@@ -73,11 +75,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--side", type=int, default=8192, help="scene is side x side pixels")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64")
-    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--cpu-sample", type=int, default=524288)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one plain device copy of BYTES (the known-traffic launch that "
+                         "calibrates the profiler's HBM byte counters) and exit")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="skip the warm-start network")
     ap.add_argument("--no-equal-misfit", action="store_true")
-    ap.add_argument("--equal-misfit-pixels", type=int, default=1 << 24)
+    ap.add_argument("--equal-misfit-pixels", type=int, default=1 << 22)
     ap.add_argument("--train-samples", type=int, default=1 << 24)
     ap.add_argument("--train-epochs", type=int, default=2)
     args = ap.parse_args()
@@ -88,6 +93,14 @@ def main():
         sys.exit(2)
 
     import photic_amd as P
+    if args.copy_calibration:
+        nbytes = args.copy_calibration
+        src, dst = P.DeviceBuffer(nbytes), P.DeviceBuffer(nbytes)
+        P.check(P.lib.photic_memset_zero(src.value, nbytes))
+        P.check(P.lib.photic_device_copy(dst.value, src.value, nbytes, None))
+        P.check(P.lib.photic_synchronize())
+        print(json.dumps({"copy_bytes": nbytes}))
+        return
     prec = P.FP64 if args.precision == "fp64" else P.FP32
     n = args.side * args.side
     cfg = P.cfg_default(WL, reqmin=1e-10, konvge=100, kcount=2000, precision=prec,
@@ -153,6 +166,7 @@ def main():
     def summary(ms, res, n_run):
         v = res["status"] >= 0
         dH = res["depth"][v].astype(np.float64) - truth_H[:n_run][v]
+        fit = res["rrs_error"][v] < 1.0  # pixels the model fits to 1% (E_Rrs)
         return {
             "pixels_per_s": n_run / (ms / 1e3),
             "ms": ms,
@@ -161,18 +175,18 @@ def main():
             "converged_frac": float((res["status"][v] == 0).mean()),
             "median_rrs_error_pct": float(np.median(res["rrs_error"][v])),
             "depth_rmse_m": float(np.sqrt(np.mean(dH * dH))),
+            "depth_median_abs_error_m": float(np.median(np.abs(dH))),
+            "depth_rmse_fitted_m": float(np.sqrt(np.mean(dH[fit] ** 2))),
+            "fitted_frac": float(fit.mean()),
             "valid_pixels": int(v.sum()),
         }
 
-    # ---- warm-up on a slice ----
-    n_w = min(n, 1 << 20)  # a multiple of 4, so the slices stay 16-byte aligned
-    for _ in range(args.warmup):
-        run("plain", cfg, n_w)
-        if net is not None:
-            run("seeded", cfg, n_w)
-
-    # ---- timed: K whole-scene steps of the headline method ----
+    # ---- warm-up: W whole-scene steps of the headline method, untimed ----
     head = "plain" if net is None else "seeded"
+    for _ in range(args.warmup):
+        run(head, cfg, n)
+
+    # ---- timed: K whole-scene steps ----
     walls, kms = [], []
     res = None
     for _ in range(args.steps):
@@ -191,14 +205,27 @@ def main():
         w_ms, _, pres = run("plain", cfg, n)
         comparisons["plain"] = summary(w_ms, pres, n)
         if not args.no_equal_misfit:
-            # plain at equal misfit: the reference's starts without its early exit (every
-            # start run, best kept) -- what the plain method needs to reach the seeded misfit
-            cfg_eq = P.cfg_default(WL, reqmin=1e-10, konvge=100, kcount=2000, precision=prec,
-                                   watchdog_s=900.0, early_exit=1e-300)
+            # The plain method at equal misfit: the reference's depth starts with the early
+            # exit lowered until the plain median misfit is at or below the seeded one, on the
+            # first n_eq pixels of the scene (early exit 0 = every start run, best kept).
             n_eq = min(n, args.equal_misfit_pixels)
-            w_ms, _, eres = run("plain", cfg_eq, n_eq)
-            comparisons["plain_all_starts"] = summary(w_ms, eres, n_eq)
-            comparisons["plain_all_starts"]["pixels"] = n_eq
+            v_eq = res["status"][:n_eq] >= 0
+            seeded_med = float(np.median(res["rrs_error"][:n_eq][v_eq]))
+            sweep = []
+            for ee in (1.0, 0.1, 0.01, 1e-3, 1e-4, 1e-300):
+                cfg_eq = P.cfg_default(WL, reqmin=1e-10, konvge=100, kcount=2000,
+                                       precision=prec, watchdog_s=900.0, early_exit=ee)
+                w_ms, _, eres = run("plain", cfg_eq, n_eq)
+                s_eq = summary(w_ms, eres, n_eq)
+                s_eq["early_exit"] = ee
+                sweep.append(s_eq)
+                if s_eq["median_rrs_error_pct"] <= seeded_med:
+                    break
+            comparisons["plain_equal_misfit"] = dict(sweep[-1], pixels=n_eq,
+                                                     seeded_median_rrs_error_pct=seeded_med)
+            comparisons["plain_equal_misfit_sweep"] = [
+                {k: e[k] for k in ("early_exit", "pixels_per_s", "median_rrs_error_pct",
+                                   "evals_per_pixel")} for e in sweep]
     achieved = evals * FLOP_PER_EVAL / (k_ms / 1e3) / 1e12
     # HBM bytes per launch: the per-pixel FETCH_SIZE / WRITE_SIZE of the same kernel from
     # rocprofv3 --pmc, each calibrated by a copy kernel of known size (profiles/)
@@ -206,10 +233,9 @@ def main():
     tj = os.path.join(ROOT, "profiles", "invert_fp64_traffic.json")
     if prec == P.FP64 and os.path.exists(tj):
         with open(tj) as f:
-            t = json.load(f)
-        traffic = (t["fetch_bytes_per_pixel"] + t["write_bytes_per_pixel"]) * n
-        if head == "seeded":
-            traffic += 20.0 * n  # the five seed floats each pixel reads
+            t = json.load(f).get(head)
+        if t:
+            traffic = (t["fetch_bytes_per_pixel"] + t["write_bytes_per_pixel"]) * n
 
     line = {
         "metric": METRIC,
@@ -240,7 +266,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": traffic,
-            "kernel": "ph_invert_kernel<double,4,0>" if prec == P.FP64 else "ph_invert_kernel<float,4,0>",
+            "kernel": "ph_invert_kernel<%s, 4, 0, %s>" % (
+                "double" if prec == P.FP64 else "float", "true" if head == "seeded" else "false"),
             "kernel_ms": k_ms,
             "flop_per_eval": FLOP_PER_EVAL,
             "evals_per_launch": evals,
@@ -252,8 +279,9 @@ def main():
     line.update(comparisons)
     if "plain" in comparisons:
         line["seeded_over_plain"] = px_s / comparisons["plain"]["pixels_per_s"]
-    if "plain_all_starts" in comparisons:
-        line["seeded_over_plain_equal_misfit"] = px_s / comparisons["plain_all_starts"]["pixels_per_s"]
+    if "plain_equal_misfit" in comparisons:
+        line["seeded_over_plain_equal_misfit"] = (
+            px_s / comparisons["plain_equal_misfit"]["pixels_per_s"])
 
     if not args.no_cpu:
         ns = min(args.cpu_sample, n)

@@ -123,7 +123,8 @@ __device__ inline void ph_write_nodata(const PhInvertArgs& a, long pix, int n_pa
 template <int NBOT>
 constexpr int ph_invert_waves() { return NBOT == 0 ? 2 : 1; }
 
-template <typename FT, int NB, int NBOT>
+// SEED: every pixel's first start is its seed point (a.seed), before the depth starts.
+template <typename FT, int NB, int NBOT, bool SEED>
 __global__ void __launch_bounds__(256, ph_invert_waves<NBOT>()) ph_invert_kernel(PhInvertArgs a)
 {
     using L = PhLayout<NBOT>;
@@ -180,7 +181,7 @@ __global__ void __launch_bounds__(256, ph_invert_waves<NBOT>()) ph_invert_kernel
                     double sv = a.sec_view ? (double)a.sec_view[pix] : c.sec_view0;
                     bool ok = ph_pixel_init<FT, 1, NB>(px, c, r, &ss, &sv);
                     if (ok) {
-                        S.kstart = c.has_seed ? -1 : 0;
+                        S.kstart = SEED ? -1 : 0;
                         S.lowest = 1.0e4;
                         S.nstart = 0;
                         S.evals = 0;
@@ -204,13 +205,17 @@ __global__ void __launch_bounds__(256, ph_invert_waves<NBOT>()) ph_invert_kernel
 
         // ---- set up a new start ----
         if (S.phase == PH_START) {
-            double sd[N];
-            if (S.kstart < 0) {
-#pragma unroll
-                for (int i = 0; i < N; i++) sd[i] = (double)a.seed[(long)i * a.n_pix + pix];
-            }
             const double hov = a.hstart ? (double)a.hstart[pix] : -1.0;
-            ph_start_step<FT, 1, NB, NBOT>(S, c, px, S.kstart, sd, hov);
+            if constexpr (SEED) {
+                double sd[N];
+                if (S.kstart < 0) {
+#pragma unroll
+                    for (int i = 0; i < N; i++) sd[i] = (double)a.seed[(long)i * a.n_pix + pix];
+                }
+                ph_start_step<FT, 1, NB, NBOT>(S, c, px, S.kstart, sd, hov);
+            } else {
+                ph_start_step<FT, 1, NB, NBOT>(S, c, px, S.kstart, nullptr, hov);
+            }
             ph_nm_begin(S, c);
         }
 
@@ -347,96 +352,97 @@ extern "C" int ph_launch_zero_u64(unsigned long long* p, int n, void* stream)
     return (int)hipGetLastError();
 }
 
-template <typename FT, int NB, int NBOT>
-static int ph_attrs_t(int* vgprs, int* sgprs, int* scratch, int* maxt)
+// The compiled instances of the inversion kernel: FP64 for the mix and native layouts with
+// 1..3 bottoms, FP32 for the mix layout; each plain and seeded.  fn receives a PhKind tag.
+template <typename FT, int NBOT, bool SEED>
+struct PhKind {
+    using ft = FT;
+    static constexpr int nbot = NBOT;
+    static constexpr bool seed = SEED;
+};
+
+template <typename FT, int NBOT, typename Fn>
+static int ph_dispatch_seed(int seeded, Fn&& fn)
 {
-    hipFuncAttributes fa;
-    hipError_t e = hipFuncGetAttributes(&fa, (const void*)ph_invert_kernel<FT, NB, NBOT>);
-    if (e != hipSuccess) return (int)e;
-    *vgprs = 0;
-    *sgprs = 0;
-    *scratch = (int)fa.localSizeBytes;
-    *maxt = fa.maxThreadsPerBlock;
-    *vgprs = fa.numRegs;
-    return 0;
+    return seeded ? fn(PhKind<FT, NBOT, true>{}) : fn(PhKind<FT, NBOT, false>{});
 }
 
-template <typename FT, int NB, int NBOT>
-static int ph_occ_t(int block, int* bpc)
+template <typename Fn>
+static int ph_dispatch(int fp32, int n_bands, int n_bottoms, int seeded, Fn&& fn)
 {
-    constexpr int N = PhLayout<NBOT>::template n<1>();
-    size_t lds = sizeof(double) * (size_t)PhLaneMem<N>::n_slots * (size_t)block;
-    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        bpc, (const void*)ph_invert_kernel<FT, NB, NBOT>, block, lds);
+    if (n_bands != 4) return PH_LAUNCH_UNSUPPORTED;
+    if (fp32) return n_bottoms == 0 ? ph_dispatch_seed<float, 0>(seeded, fn) : PH_LAUNCH_UNSUPPORTED;
+    switch (n_bottoms) {
+    case 0: return ph_dispatch_seed<double, 0>(seeded, fn);
+    case 1: return ph_dispatch_seed<double, 1>(seeded, fn);
+    case 2: return ph_dispatch_seed<double, 2>(seeded, fn);
+    case 3: return ph_dispatch_seed<double, 3>(seeded, fn);
+    default: return PH_LAUNCH_UNSUPPORTED;
+    }
+}
+
+template <typename K>
+static const void* ph_invert_fn(K)
+{
+    return (const void*)ph_invert_kernel<typename K::ft, 4, K::nbot, K::seed>;
+}
+
+template <typename K>
+static size_t ph_invert_lds(K, int block)
+{
+    constexpr int N = PhLayout<K::nbot>::template n<1>();
+    return sizeof(double) * (size_t)PhLaneMem<N>::n_slots * (size_t)block;
 }
 
 extern "C" int ph_lds_bytes(int n_bottoms, int block)
 {
     switch (n_bottoms) {
-    case 0: return (int)(sizeof(double) * PhLaneMem<PhLayout<0>::n<1>()>::n_slots * block);
-    case 1: return (int)(sizeof(double) * PhLaneMem<PhLayout<1>::n<1>()>::n_slots * block);
-    case 2: return (int)(sizeof(double) * PhLaneMem<PhLayout<2>::n<1>()>::n_slots * block);
-    case 3: return (int)(sizeof(double) * PhLaneMem<PhLayout<3>::n<1>()>::n_slots * block);
+    case 0: return (int)ph_invert_lds(PhKind<double, 0, false>{}, block);
+    case 1: return (int)ph_invert_lds(PhKind<double, 1, false>{}, block);
+    case 2: return (int)ph_invert_lds(PhKind<double, 2, false>{}, block);
+    case 3: return (int)ph_invert_lds(PhKind<double, 3, false>{}, block);
     default: return -1;
     }
 }
 
-extern "C" int ph_kernel_occupancy(int fp32, int n_bands, int n_bottoms, int block, int* bpc)
+extern "C" int ph_kernel_occupancy(int fp32, int n_bands, int n_bottoms, int seeded, int block,
+                                   int* bpc)
 {
-    if (n_bands != 4) return PH_LAUNCH_UNSUPPORTED;
-    if (fp32) return n_bottoms == 0 ? ph_occ_t<float, 4, 0>(block, bpc) : PH_LAUNCH_UNSUPPORTED;
-    switch (n_bottoms) {
-    case 0: return ph_occ_t<double, 4, 0>(block, bpc);
-    case 1: return ph_occ_t<double, 4, 1>(block, bpc);
-    case 2: return ph_occ_t<double, 4, 2>(block, bpc);
-    case 3: return ph_occ_t<double, 4, 3>(block, bpc);
-    default: return PH_LAUNCH_UNSUPPORTED;
-    }
+    return ph_dispatch(fp32, n_bands, n_bottoms, seeded, [&](auto k) {
+        return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, ph_invert_fn(k), block,
+                                                                 ph_invert_lds(k, block));
+    });
 }
 
 extern "C" int ph_kernel_attrs(int fp32, int n_bands, int n_bottoms, int* vgprs, int* sgprs,
                                int* scratch, int* maxt)
 {
-    if (n_bands != 4) return PH_LAUNCH_UNSUPPORTED;
-    if (fp32) return n_bottoms == 0 ? ph_attrs_t<float, 4, 0>(vgprs, sgprs, scratch, maxt)
-                                    : PH_LAUNCH_UNSUPPORTED;
-    switch (n_bottoms) {
-    case 0: return ph_attrs_t<double, 4, 0>(vgprs, sgprs, scratch, maxt);
-    case 1: return ph_attrs_t<double, 4, 1>(vgprs, sgprs, scratch, maxt);
-    case 2: return ph_attrs_t<double, 4, 2>(vgprs, sgprs, scratch, maxt);
-    case 3: return ph_attrs_t<double, 4, 3>(vgprs, sgprs, scratch, maxt);
-    default: return PH_LAUNCH_UNSUPPORTED;
-    }
-}
-
-template <typename FT, int NB, int NBOT>
-static int ph_launch_invert_t(const PhInvertArgs& a, int grid, int block, hipStream_t st)
-{
-    constexpr int N = PhLayout<NBOT>::template n<1>();
-    size_t lds = sizeof(double) * (size_t)PhLaneMem<N>::n_slots * (size_t)block;
-    hipError_t e = hipFuncSetAttribute((const void*)ph_invert_kernel<FT, NB, NBOT>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL((ph_invert_kernel<FT, NB, NBOT>), dim3(grid), dim3(block), lds, st, a);
-    return (int)hipGetLastError();
+    return ph_dispatch(fp32, n_bands, n_bottoms, 0, [&](auto k) {
+        hipFuncAttributes fa;
+        hipError_t e = hipFuncGetAttributes(&fa, ph_invert_fn(k));
+        if (e != hipSuccess) return (int)e;
+        *vgprs = fa.numRegs;
+        *sgprs = 0;
+        *scratch = (int)fa.localSizeBytes;
+        *maxt = fa.maxThreadsPerBlock;
+        return 0;
+    });
 }
 
 extern "C" int ph_launch_invert(const PhInvertArgs* a, int fp32, int n_bands, int n_bottoms,
                                 int grid, int block, void* stream)
 {
     hipStream_t st = (hipStream_t)stream;
-    if (n_bands != 4) return PH_LAUNCH_UNSUPPORTED;
-    if (fp32) {
-        if (n_bottoms == 0) return ph_launch_invert_t<float, 4, 0>(*a, grid, block, st);
-        return PH_LAUNCH_UNSUPPORTED;
-    }
-    switch (n_bottoms) {
-    case 0: return ph_launch_invert_t<double, 4, 0>(*a, grid, block, st);
-    case 1: return ph_launch_invert_t<double, 4, 1>(*a, grid, block, st);
-    case 2: return ph_launch_invert_t<double, 4, 2>(*a, grid, block, st);
-    case 3: return ph_launch_invert_t<double, 4, 3>(*a, grid, block, st);
-    default: return PH_LAUNCH_UNSUPPORTED;
-    }
+    return ph_dispatch(fp32, n_bands, n_bottoms, a->seed != nullptr, [&](auto k) {
+        using K = decltype(k);
+        const size_t lds = ph_invert_lds(k, block);
+        hipError_t e = hipFuncSetAttribute(ph_invert_fn(k),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return (int)e;
+        hipLaunchKernelGGL((ph_invert_kernel<typename K::ft, 4, K::nbot, K::seed>), dim3(grid),
+                           dim3(block), lds, st, *a);
+        return (int)hipGetLastError();
+    });
 }
 
 extern "C" int ph_launch_fwd(const PhConst* c, long n, const double* params, const double* sec_sun,

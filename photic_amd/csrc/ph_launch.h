@@ -76,7 +76,8 @@ int ph_launch_gen(const PhConst *c, uint64_t seed, long pix0, long n, double sig
                   double deep_frac, float *rrs, float *sec_sun, float *truth, void *stream);
 int ph_kernel_attrs(int fp32, int n_bands, int n_bottoms, int *vgprs, int *sgprs,
                     int *scratch_bytes, int *max_threads);
-int ph_kernel_occupancy(int fp32, int n_bands, int n_bottoms, int block, int *blocks_per_cu);
+int ph_kernel_occupancy(int fp32, int n_bands, int n_bottoms, int seeded, int block,
+                        int *blocks_per_cu);
 int ph_lds_bytes(int n_bottoms, int block);
 
 /* the warm-start network (ph_mlp.hip) */

@@ -272,7 +272,7 @@ int photic_kernel_info(const photic_cfg *cfg, int *vgprs, int *scratch_bytes,
     int sg = 0, mt = 0, block = cfg->block > 0 ? cfg->block : 256;
     rc = ph_kernel_attrs(fp32, c.n_bands, nbot, vgprs, &sg, scratch_bytes, &mt);
     if (rc) return ph_launch_status(rc, "kernel attributes");
-    rc = ph_kernel_occupancy(fp32, c.n_bands, nbot, block, blocks_per_cu);
+    rc = ph_kernel_occupancy(fp32, c.n_bands, nbot, 0, block, blocks_per_cu);
     if (rc) return ph_launch_status(rc, "occupancy");
     return ph_n_cu(n_cu);
 }
@@ -329,7 +329,7 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
     int n_cu = 0, bpc = 0;
     rc = ph_n_cu(&n_cu);
     if (rc) return rc;
-    rc = ph_kernel_occupancy(fp32, a.c.n_bands, nbot, block, &bpc);
+    rc = ph_kernel_occupancy(fp32, a.c.n_bands, nbot, seed != NULL, block, &bpc);
     if (rc) return ph_launch_status(rc, "occupancy");
     if (bpc < 1) return ph_fail(PHOTIC_EUNSUPPORTED, "kernel does not fit a CU at block %d", block);
     long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
