@@ -112,6 +112,29 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
                     const float *sec_view, const float *seed, const float *hstart,
                     const photic_outputs *out, void *stream);
 
+/* Multi-date stacks: each pixel seen on n_dates (<= 8) dates, some of them invalid (clouds).
+ * A pixel's problem is the reference's N_s = k run over its k valid dates, in date order
+ * (model/samodel.c:1808-1815 layout, one region, the mix layout of PHOTIC_MODE_MIX): H and B
+ * shared, P, G, X per date, 2 + 3k parameters, the misfit over every (date, band), early exit
+ * at 2.5 k (cfg->early_exit > 0: that value times k).  Device arrays: rrs [n_dates][4][n_pix],
+ * sec_sun [n_dates][n_pix], valid [n_dates][n_pix] (0 / 1), seed [2 + 3 n_dates][n_pix] or
+ * NULL (H, B, then P, G, X at each date's slot; invalid slots ignored).  Outputs as
+ * photic_invert; out->x is [2 + 3 n_dates][n_pix] with each valid date's P, G, X at its slot
+ * and 0 at invalid slots; pixels with no valid date get the no-data defaults.  Mix layout,
+ * 4 bands, FP64. */
+int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
+                        const float *sec_sun, const unsigned char *valid, const float *seed,
+                        const photic_outputs *out, void *stream);
+
+/* Synthetic stack scene (Philox as photic_gen_scene, counter word 3 = date): H ~ U(0.5, 25),
+ * B ~ U(0, 1) per pixel; per date sun zenith ~ U(20, 50) deg and log-normal P, G, X; a
+ * Bernoulli(p_cloud) cloud mask with at least k_min clear dates; a date whose noisy Rrs goes
+ * negative is invalid.  rrs [n_dates][4][n], sec_sun [n_dates][n], valid [n_dates][n], truth
+ * [2 + 4 n_dates][n] (H, B, then P, G, X, theta per date) or NULL. */
+int photic_gen_stack(const photic_cfg *cfg, uint64_t seed, long pix0, long n, int n_dates,
+                     double sigma, double p_cloud, int k_min, float *rrs, float *sec_sun,
+                     unsigned char *valid, float *truth, void *stream);
+
 /* Synthetic single-date scene (Philox4x32-10 keyed by seed, counter = pixel index): draws
  * H ~ U(0.5, 25) (deep_frac of pixels U(30, 60)), log-normal P, G, X, B ~ U(0, 1), sun zenith
  * ~ U(20, 50) deg, FP64 forward model with self-consistent Y, N(0, sigma) noise.  rrs:

@@ -43,6 +43,12 @@ def lib():
         _lib.orc_py_gen.argtypes = [C.c_uint64, l, l, v, i, d, d, v, v, v, v, v]
         _lib.orc_py_band_tables.argtypes = [v, i, v, v]
         _lib.orc_py_start_step.argtypes = [i, i, v, i, v, d, v, v]
+        _lib.orc_py_objective_stack.argtypes = [v, i, i, v, v, v, v]
+        _lib.orc_py_objective_stack.restype = d
+        _lib.orc_py_gen_stack.argtypes = [C.c_uint64, l, l, v, i, d, i, d, i, v, v, v, v]
+        _lib.orc_py_invert_stack.restype = l
+        _lib.orc_py_invert_stack.argtypes = [v, i, i, l, v, v, v, d, i, i, i, v, C.c_float, v,
+                                             v, v, C.POINTER(PyOut)]
         _lib.orc_py_start_step.restype = i
         _lib.orc_py_Y.argtypes = [d]
         _lib.orc_py_Y.restype = d
@@ -229,3 +235,85 @@ def table(name):
     if name == "bottom":
         return np.ctypeslib.as_array((C.c_double * (8 * 41)).in_dll(lib(), "orc_bottom")).reshape(8, 41).copy()
     return np.ctypeslib.as_array((C.c_double * 41).in_dll(lib(), "orc_" + name)).copy()
+
+
+def gen_stack(seed, n, n_dates=8, pix0=0, sigma=2e-4, p_cloud=0.3, k_min=3, wl=WL4):
+    """Multi-date stack scene: rrs [D][nb][n] float32, sec_sun [D][n] float32, valid [D][n]
+    uint8, truth [2 + 4D][n] (H, B, then P, G, X, theta_sun per date)."""
+    nb = len(wl)
+    rrs = np.zeros((n_dates, nb, n), np.float32)
+    ss = np.zeros((n_dates, n), np.float32)
+    valid = np.zeros((n_dates, n), np.uint8)
+    truth = np.zeros((2 + 4 * n_dates, n))
+    w = np.asarray(wl, np.float64)
+    lib().orc_py_gen_stack(seed, pix0, n, w.ctypes.data, nb, sigma, n_dates, p_cloud, k_min,
+                           rrs.ctypes.data, ss.ctypes.data, valid.ctypes.data, truth.ctypes.data)
+    return rrs, ss, valid, truth
+
+
+def invert_stack(rrs, sec_sun, valid, reqmin=1e-10, konvge=100, kcount=2000,
+                 hstart=(40.0, 30.0, 20.0, 15.0, 10.0, 7.5, 2.5, 1.0), nodata=-9999.0,
+                 seed=None, seed_rel=(0.3, 0.0, 1.0, 1.0, 1.0),
+                 seed_min=(0.5, 0.25, 0.1, 0.1, 0.02), wl=WL4):
+    """Oracle inversion of multi-date stacks (mix layout, valid dates only)."""
+    rrs = np.ascontiguousarray(rrs, np.float32)
+    nd, nb, n = rrs.shape
+    out = {
+        "x": np.zeros((2 + 3 * nd, n)), "depth": np.zeros(n, np.float32),
+        "rrs_error": np.zeros(n, np.float32), "model_error": np.zeros(n, np.float32),
+        "k_min": np.zeros(n, np.float32), "iod": np.zeros(n, np.float32),
+        "albedo": np.zeros(n, np.float32), "bottom_type": np.zeros(n, np.float32),
+        "b_pct": np.zeros((3, n), np.float32), "n_iter": np.zeros(n, np.int32),
+        "converged": np.zeros(n, np.int32), "n_evals": np.zeros(n, np.int32),
+        "n_starts": np.zeros(n, np.int32), "status": np.zeros(n, np.int32),
+        "misfit": np.zeros(n),
+    }
+    o = PyOut(**{k: v.ctypes.data for k, v in out.items()})
+    w = np.asarray(wl, np.float64)
+    hs = np.asarray(hstart, np.float64)
+    ss = np.ascontiguousarray(sec_sun, np.float32)
+    va = np.ascontiguousarray(valid, np.uint8)
+    sd = None if seed is None else np.ascontiguousarray(seed, np.float32)
+    sr = np.asarray(seed_rel, np.float64)
+    sm = np.asarray(seed_min, np.float64)
+    lib().orc_py_invert_stack(w.ctypes.data, nb, nd, n, rrs.ctypes.data, ss.ctypes.data,
+                              va.ctypes.data, reqmin, konvge, kcount, len(hs), hs.ctypes.data,
+                              nodata, _p(sd), sr.ctypes.data, sm.ctypes.data, C.byref(o))
+    return out
+
+
+def hostsim_invert_stack(cfg_const, rrs, sec_sun, valid, seed=None, math="libm"):
+    """The product's stack engine on the host (tests/hostsim); cfg_const from a single-date
+    configuration (the engine sets the date count, reqmin * n and the early exit per pixel)."""
+    path = HOSTSIM if math == "libm" else HOSTSIM.replace("libph_hostsim", "libph_hostsim_cr")
+    h = C.CDLL(path)
+    rrs = np.ascontiguousarray(rrs, np.float32)
+    nd, nb, n = rrs.shape
+    out = {"x": np.zeros((2 + 3 * nd, n)), "depth": np.zeros(n, np.float32),
+           "rrs_error": np.zeros(n, np.float32), "model_error": np.zeros(n, np.float32),
+           "status": np.zeros(n, np.int32), "n_evals": np.zeros(n, np.int32),
+           "n_iter": np.zeros(n, np.int32), "n_starts": np.zeros(n, np.int32)}
+    ss = np.ascontiguousarray(sec_sun, np.float32)
+    va = np.ascontiguousarray(valid, np.uint8)
+    sd = None if seed is None else np.ascontiguousarray(seed, np.float32)
+    h.phs_invert_stack.argtypes = [C.c_void_p, C.c_int, C.c_long] + [C.c_void_p] * 12
+    rc = h.phs_invert_stack(C.cast(cfg_const, C.c_void_p), nd, n, rrs.ctypes.data, ss.ctypes.data,
+                            va.ctypes.data, _p(sd),
+                            *[out[k].ctypes.data for k in ("x", "depth", "rrs_error",
+                                                            "model_error", "status", "n_evals",
+                                                            "n_iter", "n_starts")])
+    assert rc == 0
+    return out
+
+
+def objective_stack(x, rrs, sec_sun, wl=WL4):
+    """Objective and E_Rrs of one mix-layout vector x [2 + 3k] over k dates (rrs [k][nb])."""
+    x = np.ascontiguousarray(x, np.float64)
+    r = np.ascontiguousarray(rrs, np.float32)
+    s = np.ascontiguousarray(sec_sun, np.float32)
+    k, nb = r.shape
+    e = C.c_double()
+    w = np.asarray(wl, np.float64)
+    f = lib().orc_py_objective_stack(w.ctypes.data, nb, k, x.ctypes.data, r.ctypes.data,
+                                     s.ctypes.data, C.byref(e))
+    return f, e.value

@@ -11,6 +11,7 @@
  *   block 2      : u_Hdeep, (unused)
  *   blocks 16+a  : P attempt a (a < 32)  blocks 48+a : G attempt a   blocks 80+a : X attempt a
  *   blocks 112+b : noise of band b
+ *   stacks       : cloud mask of date d from block 3 (redraw r from block 3 + r, r = 1..16)
  * Each block yields two 53-bit uniforms; a normal uses one block (Box-Muller, cosine branch).
  */
 #include "photic_oracle.h"
@@ -135,4 +136,62 @@ int orc_gen_pixel(uint64_t seed, uint64_t pix, const orc_bands *bands, double no
         rrs_out[b] = f;
     }
     return nodata;
+}
+
+/* Multi-date stacks.  Per pixel: H, B from block 0 of date 0 (as the single-date scene); per
+ * date d: sun zenith (block 1), P, G, X (blocks 16+a, 48+a, 80+a) and noise (112+b) with
+ * counter word 3 = d; cloud mask from block 3 of date d (clear when u >= p_cloud).  A mask
+ * with fewer than k_min clear dates is redrawn from blocks 4 + r (r < 16), then the first
+ * k_min dates are forced clear.  A date whose noisy Rrs goes negative is marked invalid.
+ * truth: H, B, then (P, G, X, theta_sun) per date. */
+int orc_gen_stack_pixel(uint64_t seed, uint64_t pix, const orc_bands *bands,
+                        double noise_sigma, int n_dates, double p_cloud, int k_min,
+                        double *truth, float *rrs_out /*[n_dates][nb]*/, uint8_t *valid)
+{
+    double uH, uB, u0, u1;
+    orc_block_uniforms(seed, pix, 0, 0, &uH, &uB);
+    const double H = 0.5 + 24.5 * uH, B = uB;
+    truth[0] = H;
+    truth[1] = B;
+    int nclear = 0;
+    for (int r = 0; r < 17; r++) {
+        nclear = 0;
+        for (int d = 0; d < n_dates; d++) {
+            orc_block_uniforms(seed, pix, 3u + (uint32_t)r, (uint32_t)d, &u0, &u1);
+            valid[d] = u0 >= p_cloud;
+            nclear += valid[d];
+        }
+        if (nclear >= k_min)
+            break;
+    }
+    if (nclear < k_min)
+        for (int d = 0; d < n_dates; d++)
+            valid[d] = d < k_min;
+    int nb = bands->n_bands, k = 0;
+    for (int d = 0; d < n_dates; d++) {
+        orc_truth t;
+        orc_block_uniforms(seed, pix, 1, (uint32_t)d, &u0, &u1);
+        t.H = H;
+        t.B = B;
+        t.theta_sun = 20.0 + 30.0 * u0;
+        t.P = orc_trunc_lognormal(seed, pix, 16, (uint32_t)d, 0.005, 0.2);
+        t.G = orc_trunc_lognormal(seed, pix, 48, (uint32_t)d, 0.002, 0.2);
+        t.X = orc_trunc_lognormal(seed, pix, 80, (uint32_t)d, 0.001, 0.05);
+        truth[2 + 4 * d] = t.P;
+        truth[3 + 4 * d] = t.G;
+        truth[4 + 4 * d] = t.X;
+        truth[5 + 4 * d] = t.theta_sun;
+        double rrs[ORC_MAXB];
+        orc_fwd_truth(&t, bands, 20, rrs, NULL);
+        for (int b = 0; b < nb; b++) {
+            double v = rrs[b] + noise_sigma * orc_block_normal(seed, pix, 112 + (uint32_t)b,
+                                                               (uint32_t)d);
+            float f = (float)v;
+            if (f < 0.0f)
+                valid[d] = 0;
+            rrs_out[d * nb + b] = f;
+        }
+        k += valid[d];
+    }
+    return k;
 }

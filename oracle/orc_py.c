@@ -287,3 +287,142 @@ int orc_py_start_step(int mode, int n_bottoms, const double *wl, int nb, const d
     orc_start_step(&px, h0, start, step);
     return orc_n_params(&px);
 }
+
+/* Stack scene pixels pix0 .. pix0+n-1: rrs [n_dates][nb][n], sec_sun [n_dates][n] (sec of
+ * the drawn zenith), valid [n_dates][n], truth [2 + 4 n_dates][n] (may be NULL). */
+void orc_py_gen_stack(uint64_t seed, long pix0, long n, const double *wl, int nb, double sigma,
+                      int n_dates, double p_cloud, int k_min, float *rrs, float *sec_sun,
+                      uint8_t *valid, double *truth)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        double t[2 + 4 * ORC_MAXD];
+        float r[ORC_MAXD * ORC_MAXB];
+        uint8_t v[ORC_MAXD];
+        orc_gen_stack_pixel(seed, (uint64_t)(pix0 + i), &bands, sigma, n_dates, p_cloud, k_min,
+                            t, r, v);
+        for (int d = 0; d < n_dates; d++) {
+            for (int b = 0; b < nb; b++)
+                rrs[((long)d * nb + b) * n + i] = r[d * nb + b];
+            sec_sun[(long)d * n + i] = (float)orc_sec_deg(t[5 + 4 * d]);
+            valid[(long)d * n + i] = v[d];
+        }
+        if (truth)
+            for (int k = 0; k < 2 + 4 * n_dates; k++)
+                truth[(long)k * n + i] = t[k];
+    }
+}
+
+/* Invert n multi-date stacks (MIX layout: [H, B, (P, G, X) per valid date]).  rrs [n_dates]
+ * [nb][n], sec_sun [n_dates][n], valid [n_dates][n]; only the valid dates of a pixel enter its
+ * problem, in date order (k = their count, 2 + 3k parameters, early exit at 2.5 k).  seed
+ * [2 + 3 n_dates][n] or NULL: H, B, then (P, G, X) at each date's slot (invalid slots
+ * ignored).  x [2 + 3 n_dates][n]: H, B and each valid date's P, G, X at its date slot (0 at
+ * invalid slots).  Other outputs as orc_py_invert. */
+long orc_py_invert_stack(const double *wl, int nb, int n_dates, long n, const float *rrs,
+                         const float *sec_sun, const uint8_t *valid, double reqmin,
+                         int konvge, int kcount, int n_hstarts, const double *hstart,
+                         float nodata, const float *seed, const double *seed_rel,
+                         const double *seed_min, orc_py_out *o)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    const int np_max = 2 + 3 * n_dates;
+    long done = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : done)
+    for (long p = 0; p < n; p++) {
+        double r[ORC_MAXD * ORC_MAXB], ss[ORC_MAXD], sv[ORC_MAXD];
+        int slot[ORC_MAXD], k = 0, bad = 0;
+        for (int d = 0; d < n_dates; d++) {
+            if (!valid[(long)d * n + p])
+                continue;
+            for (int b = 0; b < nb; b++) {
+                float v = rrs[((long)d * nb + b) * n + p];
+                if (orc_approx_equal(v, nodata, 1.0e-6f) || v < 0.0f)
+                    bad = 1;
+                r[k * nb + b] = (double)v;
+            }
+            ss[k] = (double)sec_sun[(long)d * n + p];
+            sv[k] = 1.0;
+            slot[k++] = d;
+        }
+        if (o->x)
+            for (int i = 0; i < np_max; i++) o->x[(long)i * n + p] = 0.0;
+        if (bad || k == 0) {
+            if (o->status) o->status[p] = -1;
+            if (o->depth) o->depth[p] = 0.0f;
+            continue;
+        }
+        orc_pixel px;
+        orc_pixel_init(&px, ORC_MODE_MIX, 2, NULL, k, &bands, ss, sv, r);
+        orc_nm_cfg cfg;
+        orc_default_cfg(&cfg, k, reqmin, konvge, kcount);
+        cfg.n_hstarts = n_hstarts;
+        for (int h = 0; h < n_hstarts && h < 16; h++)
+            cfg.hstart[h] = hstart[h];
+        orc_result res;
+        const int np_ = orc_n_params(&px);
+        if (seed) {
+            double st[ORC_MAXN], sp[ORC_MAXN];
+            for (int i = 0; i < np_; i++) {
+                int src = i < 2 ? i : 2 + 3 * slot[(i - 2) / 3] + (i - 2) % 3;
+                int cls = i == 0 ? 0 : (i < 2 ? 1 : 2 + (i - 2) % 3);
+                st[i] = (double)seed[(long)src * n + p];
+                double s = fabs(st[i]) * seed_rel[cls];
+                if (s < seed_min[cls]) s = seed_min[cls];
+                sp[i] = s;
+            }
+            orc_optimise_seeded(&px, &cfg, NULL, st, sp, &res);
+        } else {
+            orc_optimise(&px, &cfg, NULL, &res);
+        }
+        if (o->x)
+            for (int i = 0; i < np_; i++) {
+                int dst = i < 2 ? i : 2 + 3 * slot[(i - 2) / 3] + (i - 2) % 3;
+                o->x[(long)dst * n + p] = res.x[i];
+            }
+        if (o->depth) o->depth[p] = (float)res.depth;
+        if (o->rrs_error) o->rrs_error[p] = (float)res.Rrs_error;
+        if (o->model_error) o->model_error[p] = (float)res.model_error;
+        if (o->k_min) o->k_min[p] = (float)res.K_min;
+        if (o->iod) o->iod[p] = (float)res.iod;
+        if (o->albedo) o->albedo[p] = (float)res.bottom_albedo;
+        if (o->bottom_type) o->bottom_type[p] = (float)res.bottom_type;
+        if (o->b_pct)
+            for (int q = 0; q < 3; q++) o->b_pct[(long)q * n + p] = (float)res.B_percent[q];
+        if (o->n_iter) o->n_iter[p] = res.n_iterations;
+        if (o->converged) o->converged[p] = res.converged;
+        if (o->n_evals) o->n_evals[p] = (int)res.n_evals;
+        if (o->n_starts) o->n_starts[p] = res.n_starts;
+        if (o->status) o->status[p] = res.converged ? 0 : 2;
+        if (o->misfit) o->misfit[p] = res.model_error;
+        done++;
+    }
+    return done;
+}
+
+/* Objective of one mix-layout vector over k dates: x [2 + 3k], rrs [k][nb], sec_sun [k];
+ * returns the objective and E_Rrs in *rrs_error. */
+double orc_py_objective_stack(const double *wl, int nb, int k, const double *x,
+                              const float *rrs, const float *sec_sun, double *rrs_error)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    double r[ORC_MAXD * ORC_MAXB], ss[ORC_MAXD], sv[ORC_MAXD], xv[ORC_MAXN];
+    for (int s = 0; s < k; s++) {
+        for (int b = 0; b < nb; b++)
+            r[s * nb + b] = (double)rrs[s * nb + b];
+        ss[s] = (double)sec_sun[s];
+        sv[s] = 1.0;
+    }
+    orc_pixel px;
+    orc_pixel_init(&px, ORC_MODE_MIX, 2, NULL, k, &bands, ss, sv, r);
+    for (int i = 0; i < 2 + 3 * k; i++)
+        xv[i] = x[i];
+    double f = orc_objective(xv, &px);
+    if (rrs_error)
+        *rrs_error = px.Rrs_error;
+    return f;
+}

@@ -193,6 +193,12 @@ int orc_gen_pixel(uint64_t seed, uint64_t pix, const orc_bands *bands, double no
                   double deep_frac, orc_truth *truth, float *rrs_out, double *Y_out);
 void orc_fwd_truth(const orc_truth *t, const orc_bands *bands, int n_fixed_point,
                    double *rrs_out, double *Y_out);
+/* One pixel of a multi-date stack scene: shared H and B, per-date P, G, X and sun zenith,
+ * a Bernoulli(p_cloud) cloud mask with at least k_min clear dates.  truth [2 + 4 n_dates]:
+ * H, B, then (P, G, X, theta_sun) per date; returns the number of valid dates. */
+int orc_gen_stack_pixel(uint64_t seed, uint64_t pix, const orc_bands *bands,
+                        double noise_sigma, int n_dates, double p_cloud, int k_min,
+                        double *truth, float *rrs_out, uint8_t *valid);
 
 /* --- MLP warm start, FP64 reference (5-128-128-128-5, GELU tanh form) --- */
 #define ORC_MLP_IN 5

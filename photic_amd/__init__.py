@@ -115,6 +115,10 @@ def _load():
         "photic_invert_h": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, vp, C.POINTER(Outputs),
                                 vp]),
         "photic_band_coefficients": (i, [C.POINTER(Cfg), vp, C.POINTER(d)]),
+        "photic_invert_stack": (i, [C.POINTER(Cfg), l, i, vp, vp, vp, vp, C.POINTER(Outputs),
+                                    vp]),
+        "photic_gen_stack": (i, [C.POINTER(Cfg), C.c_uint64, l, l, i, d, d, i, vp, vp, vp, vp,
+                                 vp]),
         "ph_build_const": (i, [C.POINTER(Cfg), i, i, vp]),
         "ph_const_size": (i, []),
     }
@@ -257,6 +261,66 @@ def invert(cfg, rrs, sec_sun=None, seed=None, fields=None):
                   d_seed.value if d_seed else None, outs)
     check(lib.photic_synchronize())
     return outs.fetch()
+
+
+class StackOutputs(DeviceOutputs):
+    """Device output buffers for n stacks of n_dates dates (x is [2 + 3 n_dates][n])."""
+
+    def __init__(self, cfg, n, n_dates, fields=None):
+        self.n = n
+        self.np = 2 + 3 * n_dates
+        self.bufs = {}
+        self.struct = Outputs()
+        for name, (dt, rows) in _OUT_FIELDS.items():
+            if fields is not None and name not in fields:
+                continue
+            rows = self.np if rows is None else rows
+            b = DeviceBuffer(np.dtype(dt).itemsize * rows * n)
+            self.bufs[name] = (b, dt, rows)
+            setattr(self.struct, name, b.value)
+
+
+def invert_stack_device(cfg, n, n_dates, d_rrs, d_sec_sun, d_valid, d_seed=None, outs=None,
+                        stream=None):
+    """photic_invert_stack on device pointers (ints or None); returns the StackOutputs."""
+    if outs is None:
+        outs = StackOutputs(cfg, n, n_dates)
+    check(lib.photic_invert_stack(C.byref(cfg), n, n_dates, d_rrs, d_sec_sun, d_valid, d_seed,
+                                  C.byref(outs.struct), stream))
+    return outs
+
+
+def invert_stack(cfg, rrs, sec_sun, valid, seed=None, fields=None):
+    """Invert host arrays: rrs [D][4][n] float32, sec_sun [D][n], valid [D][n] uint8,
+    seed [2 + 3D][n] or None."""
+    rrs = np.ascontiguousarray(rrs, dtype=np.float32)
+    nd, _, n = rrs.shape
+    d_rrs = DeviceBuffer.from_array(rrs)
+    d_ss = DeviceBuffer.from_array(np.asarray(sec_sun, np.float32))
+    d_v = DeviceBuffer.from_array(np.asarray(valid, np.uint8))
+    d_seed = DeviceBuffer.from_array(np.asarray(seed, np.float32)) if seed is not None else None
+    outs = StackOutputs(cfg, n, nd, fields)
+    invert_stack_device(cfg, n, nd, d_rrs.value, d_ss.value, d_v.value,
+                        d_seed.value if d_seed is not None else None, outs)
+    check(lib.photic_synchronize())
+    return outs.fetch()
+
+
+def gen_stack(cfg, seed, n, n_dates=8, pix0=0, sigma=2e-4, p_cloud=0.3, k_min=3, truth=True):
+    """Synthetic stack scene on the GPU: (rrs [D][4][n], sec_sun [D][n], valid [D][n],
+    truth [2 + 4D][n] or None)."""
+    nb = cfg.n_bands
+    d_rrs = DeviceBuffer(4 * n_dates * nb * n)
+    d_ss = DeviceBuffer(4 * n_dates * n)
+    d_v = DeviceBuffer(n_dates * n)
+    d_tr = DeviceBuffer(4 * (2 + 4 * n_dates) * n) if truth else None
+    check(lib.photic_gen_stack(C.byref(cfg), seed, pix0, n, n_dates, sigma, p_cloud, k_min,
+                               d_rrs.value, d_ss.value, d_v.value,
+                               d_tr.value if d_tr else None, None))
+    check(lib.photic_synchronize())
+    return (d_rrs.to_array(np.float32, (n_dates, nb, n)),
+            d_ss.to_array(np.float32, (n_dates, n)), d_v.to_array(np.uint8, (n_dates, n)),
+            d_tr.to_array(np.float32, (2 + 4 * n_dates, n)) if truth else None)
 
 
 def gen_scene(cfg, seed, n, pix0=0, sigma=2e-4, deep_frac=0.0, truth=True):

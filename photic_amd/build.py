@@ -26,7 +26,7 @@ CC_FLAGS = [
     "-std=c11", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-D__HIP_PLATFORM_AMD__",
     "-I" + os.path.join(ROCM, "include"), "-I" + CSRC, "-I" + os.path.join(ROOT, "include"),
 ]
-HIP_SOURCES = ["ph_kernels.hip", "ph_mlp.hip"]
+HIP_SOURCES = ["ph_kernels.hip", "ph_mlp.hip", "ph_stack.hip"]
 C_SOURCES = ["photic_capi.c", "photic_samodel.c"]
 
 
@@ -38,10 +38,17 @@ def _run(cmd):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in HIP_SOURCES:
+    procs = []
+    for src in HIP_SOURCES:  # the device compiles are independent: run them side by side
         obj = os.path.join(BUILD, src + ".o")
-        _run([os.path.join(ROCM, "bin", "hipcc")] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmd = [os.path.join(ROCM, "bin", "hipcc")] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, src),
+                                                                  "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, pr in procs if pr.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
     for src in C_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
         _run(["gcc"] + CC_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])

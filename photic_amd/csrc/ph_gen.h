@@ -12,7 +12,7 @@
 // Draw map (counter word 2 = block, word 3 = date):
 //   block 0: u_H, u_B          block 1: u_theta, u_deep     block 2: u_Hdeep
 //   blocks 16+a / 48+a / 80+a: P / G / X attempt a (a < 32)  blocks 112+b: noise of band b
-//   block 200+a: cloud mask attempt a (stacks)
+//   stacks: cloud mask of date d from block 3 (redraw r from block 3 + r, r = 1..16)
 #pragma once
 
 #include "ph_engine.h"
@@ -133,4 +133,57 @@ PH_INLINE bool ph_gen_pixel(uint64_t seed, uint64_t pix, const PhConst& c,
         rrs_out[b] = f;
     }
     return nodata;
+}
+
+// One pixel of a multi-date stack: shared H and B (block 0 of date 0), per date d the sun
+// zenith (block 1), P, G, X and the band noise with counter word 3 = d; a cloud mask
+// (clear when u >= p_cloud) with at least k_min clear dates, redrawn up to 16 times and then
+// forced clear on the first k_min dates; a date whose noisy Rrs goes negative is invalid.
+// truth: H, B, then (P, G, X, theta) per date.  Returns the number of valid dates.
+PH_INLINE int ph_gen_stack_pixel(uint64_t seed, uint64_t pix, const PhConst& c, int nb,
+                                 double sigma, int n_dates, double p_cloud, int k_min,
+                                 double* truth, float* rrs_out, unsigned char* valid)
+{
+    double uH, uB, u0, u1;
+    ph_uniforms(seed, pix, 0, 0, &uH, &uB);
+    const double H = 0.5 + 24.5 * uH, B = uB;
+    truth[0] = H;
+    truth[1] = B;
+    int nclear = 0;
+    for (int r = 0; r < 17; r++) {
+        nclear = 0;
+        for (int d = 0; d < n_dates; d++) {
+            ph_uniforms(seed, pix, 3u + (uint32_t)r, (uint32_t)d, &u0, &u1);
+            valid[d] = u0 >= p_cloud;
+            nclear += valid[d];
+        }
+        if (nclear >= k_min) break;
+    }
+    if (nclear < k_min)
+        for (int d = 0; d < n_dates; d++) valid[d] = d < k_min;
+    int k = 0;
+    for (int d = 0; d < n_dates; d++) {
+        PhTruth t;
+        ph_uniforms(seed, pix, 1, (uint32_t)d, &u0, &u1);
+        t.H = H;
+        t.B = B;
+        t.theta = 20.0 + 30.0 * u0;
+        t.P = ph_trunc_lognormal(seed, pix, 16, (uint32_t)d, 0.005, 0.2);
+        t.G = ph_trunc_lognormal(seed, pix, 48, (uint32_t)d, 0.002, 0.2);
+        t.X = ph_trunc_lognormal(seed, pix, 80, (uint32_t)d, 0.001, 0.05);
+        truth[2 + 4 * d] = t.P;
+        truth[3 + 4 * d] = t.G;
+        truth[4 + 4 * d] = t.X;
+        truth[5 + 4 * d] = t.theta;
+        double rrs[PH_MAXB];
+        ph_truth_rrs(t, c, nb, rrs);
+        for (int b = 0; b < nb; b++) {
+            float f = (float)(rrs[b] + sigma * ph_normal(seed, pix, 112 + (uint32_t)b,
+                                                         (uint32_t)d));
+            if (f < 0.0f) valid[d] = 0;
+            rrs_out[d * nb + b] = f;
+        }
+        k += valid[d];
+    }
+    return k;
 }

@@ -59,6 +59,30 @@ typedef struct PhInvertArgs {
     int *status, *n_evals, *n_iter, *n_starts;
 } PhInvertArgs;
 
+/* One launch of the stack inversion for the pixels with exactly k valid dates (k = the
+ * instance's date count, c.n_dates == k).  The full scene arrays are read through the
+ * bucket's pixel list; outputs are written at the pixels' own indices. */
+typedef struct PhStackArgs {
+    PhConst c;
+    long n_pix;                 /* pixels in the scene arrays */
+    int n_dates;                /* dates in the scene arrays (D) */
+    const float *rrs;           /* [D][n_bands][n_pix] */
+    const float *sec_sun;       /* [D][n_pix] */
+    const unsigned char *valid; /* [D][n_pix] */
+    const float *seed;          /* [2 + 3 D][n_pix]: H, B, (P, G, X) per date slot, or NULL */
+    const int *idx;             /* the bucket's pixel indices */
+    const unsigned int *count;  /* the bucket's size (device) */
+    double *work;               /* per-lane simplex memory, [wave][slot][64 lanes] */
+    long work_lanes;            /* lanes the work memory holds (grid * block) */
+    unsigned long long *queue;  /* [0] work counter, [1] watchdog flag; zero at launch */
+    int chunk;
+    unsigned long long max_ticks;
+    float *depth, *rrs_error, *model_error, *albedo, *k_min, *iod;
+    float *b_pct, *bottom_type;
+    double *x_out; /* [2 + 3 D][n_pix]: H, B, then each valid date's P, G, X at its slot */
+    int *status, *n_evals, *n_iter, *n_starts;
+} PhStackArgs;
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -79,6 +103,17 @@ int ph_kernel_attrs(int fp32, int n_bands, int n_bottoms, int *vgprs, int *sgprs
 int ph_kernel_occupancy(int fp32, int n_bands, int n_bottoms, int seeded, int block,
                         int *blocks_per_cu);
 int ph_lds_bytes(int n_bottoms, int block);
+
+/* multi-date stacks (ph_stack.hip) */
+int ph_launch_stack_bucket(const PhStackArgs *a, unsigned int *counts, int *lists,
+                           void *stream);
+int ph_launch_stack(const PhStackArgs *a, int k, int seeded, int grid, int block,
+                    void *stream);
+int ph_stack_occupancy(int k, int seeded, int block, int *blocks_per_cu);
+int ph_stack_slots(int k);
+int ph_launch_gen_stack(const PhConst *c, uint64_t seed, long pix0, long n, int n_dates,
+                        double sigma, double p_cloud, int k_min, float *rrs, float *sec_sun,
+                        unsigned char *valid, float *truth, void *stream);
 
 /* the warm-start network (ph_mlp.hip) */
 int ph_mlp_nparam(void);

@@ -393,6 +393,142 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
     return rc ? rc : rc2;
 }
 
+/* ------------------------------------------------------------------------------------ */
+/* multi-date stacks                                                                      */
+/* ------------------------------------------------------------------------------------ */
+
+int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
+                        const float *sec_sun, const unsigned char *valid, const float *seed,
+                        const photic_outputs *out, void *stream)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    if (!cfg || !out) return ph_fail(PHOTIC_EINVAL, "cfg and out are required");
+    if (n_pix < 0) return ph_fail(PHOTIC_EINVAL, "n_pix < 0");
+    if (n_dates < 1 || n_dates > PH_MAXD)
+        return ph_fail(PHOTIC_EINVAL, "n_dates %d outside 1..%d", n_dates, PH_MAXD);
+    if (cfg->mode != PHOTIC_MODE_MIX || cfg->n_bands != 4 || cfg->precision != PHOTIC_FP64)
+        return ph_fail(PHOTIC_EUNSUPPORTED,
+                       "stacks are compiled for the mix layout, 4 bands, FP64");
+    if (n_pix == 0) return PHOTIC_OK;
+    if (!rrs || !sec_sun || !valid)
+        return ph_fail(PHOTIC_EINVAL, "rrs, sec_sun and valid are required");
+    if (n_pix > 2147483647L) return ph_fail(PHOTIC_EINVAL, "n_pix above 2^31 - 1");
+
+    PhStackArgs a;
+    memset(&a, 0, sizeof(a));
+    a.n_pix = n_pix;
+    a.n_dates = n_dates;
+    a.rrs = rrs;
+    a.sec_sun = sec_sun;
+    a.valid = valid;
+    a.seed = seed;
+    a.chunk = cfg->chunk > 0 ? cfg->chunk : 64;
+    a.depth = out->depth;
+    a.rrs_error = out->rrs_error;
+    a.model_error = out->model_error;
+    a.albedo = out->albedo;
+    a.k_min = out->k_min;
+    a.iod = out->iod;
+    a.b_pct = out->b_pct;
+    a.bottom_type = out->bottom_type;
+    a.x_out = out->x;
+    a.status = out->status;
+    a.n_evals = out->n_evals;
+    a.n_iter = out->n_iter;
+    a.n_starts = out->n_starts;
+
+    int dev = 0, wrate = 0, n_cu = 0, bpc = 0;
+    rc = ph_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (hipDeviceGetAttribute(&wrate, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        wrate <= 0)
+        wrate = 100000; /* kHz */
+    const double limit_s = cfg->watchdog_s > 0.0 ? cfg->watchdog_s : 600.0;
+    a.max_ticks = (unsigned long long)(limit_s * 1000.0 * (double)wrate);
+    rc = ph_n_cu(&n_cu);
+    if (rc) return rc;
+    const int block = cfg->block > 0 ? cfg->block : 256;
+    if (block < 64 || block > 256 || block % 64)
+        return ph_fail(PHOTIC_EINVAL, "block %d must be a multiple of 64 in 64..256", block);
+    rc = ph_launch_status(ph_stack_occupancy(n_dates, seed != NULL, block, &bpc), "occupancy");
+    if (rc) return rc;
+    if (bpc < 1) bpc = 1;
+    long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
+    const long lanes = grid * block;
+    a.work_lanes = lanes;
+
+    hipStream_t st = (hipStream_t)stream;
+    unsigned int *counts = NULL;
+    int *lists = NULL;
+    unsigned long long *queues = NULL;
+    double *work = NULL;
+    const size_t work_bytes = sizeof(double) * (size_t)ph_stack_slots(n_dates) * (size_t)lanes;
+    rc = ph_hip(hipMallocAsync((void **)&counts, sizeof(unsigned int) * 16, st), "hipMallocAsync");
+    if (!rc)
+        rc = ph_hip(hipMallocAsync((void **)&lists, sizeof(int) * (size_t)(n_dates + 1) *
+                                                        (size_t)n_pix, st),
+                    "hipMallocAsync");
+    if (!rc)
+        rc = ph_hip(hipMallocAsync((void **)&queues, sizeof(unsigned long long) * 2 *
+                                                         (size_t)(n_dates + 1), st),
+                    "hipMallocAsync");
+    if (!rc) rc = ph_hip(hipMallocAsync((void **)&work, work_bytes, st), "hipMallocAsync");
+    if (!rc) rc = ph_launch_status(ph_launch_zero_u64((unsigned long long *)counts, 8, st), "zero");
+    if (!rc)
+        rc = ph_launch_status(ph_launch_zero_u64(queues, 2 * (n_dates + 1), st), "zero");
+    if (!rc) rc = ph_launch_status(ph_launch_stack_bucket(&a, counts, lists, st), "bucketing");
+    a.work = work;
+    /* one persistent launch per date count, each pulling its bucket's pixels */
+    for (int k = n_dates; k >= 1 && !rc; k--) {
+        photic_cfg one = *cfg;
+        rc = ph_make_const(&one, k, seed != NULL, &a.c);
+        if (rc) break;
+        a.c.early_exit = (cfg->early_exit > 0.0 ? cfg->early_exit : 2.5) * (double)(float)k;
+        a.idx = lists + (long)k * n_pix;
+        a.count = counts + k;
+        a.queue = queues + 2 * k;
+        rc = ph_launch_status(ph_launch_stack(&a, k, seed != NULL, (int)grid, block, st),
+                              "stack kernel");
+    }
+    if (rc == PHOTIC_OK && cfg->check_watchdog) {
+        unsigned long long flags[2 * (PH_MAXD + 1)];
+        rc = ph_hip(hipMemcpyAsync(flags, queues, sizeof(unsigned long long) * 2 *
+                                                      (size_t)(n_dates + 1),
+                                   hipMemcpyDeviceToHost, st),
+                    "watchdog read");
+        if (rc == PHOTIC_OK) rc = ph_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        for (int k = 1; rc == PHOTIC_OK && k <= n_dates; k++)
+            if (flags[2 * k + 1])
+                rc = ph_fail(PHOTIC_EDEVICE, "stack kernel (%d dates) stopped by its %.0f s "
+                                             "watchdog", k, limit_s);
+    }
+    if (work) hipFreeAsync(work, st);
+    if (queues) hipFreeAsync(queues, st);
+    if (lists) hipFreeAsync(lists, st);
+    if (counts) hipFreeAsync(counts, st);
+    return rc;
+}
+
+int photic_gen_stack(const photic_cfg *cfg, uint64_t seed, long pix0, long n, int n_dates,
+                     double sigma, double p_cloud, int k_min, float *rrs, float *sec_sun,
+                     unsigned char *valid, float *truth, void *stream)
+{
+    PhConst c;
+    int rc = ph_have_device();
+    if (rc) return rc;
+    rc = ph_make_const(cfg, 1, 0, &c);
+    if (rc) return rc;
+    if (n < 0 || pix0 < 0) return ph_fail(PHOTIC_EINVAL, "negative pixel range");
+    if (n_dates < 1 || n_dates > PH_MAXD)
+        return ph_fail(PHOTIC_EINVAL, "n_dates %d outside 1..%d", n_dates, PH_MAXD);
+    if (n > 0 && (!rrs || !sec_sun || !valid))
+        return ph_fail(PHOTIC_EINVAL, "rrs, sec_sun and valid are required");
+    return ph_launch_status(ph_launch_gen_stack(&c, seed, pix0, n, n_dates, sigma, p_cloud,
+                                                k_min, rrs, sec_sun, valid, truth, stream),
+                            "stack generator");
+}
+
 int photic_gen_scene(const photic_cfg *cfg, uint64_t seed, long pix0, long n, double sigma,
                      double deep_frac, float *rrs, float *sec_sun, float *truth, void *stream)
 {

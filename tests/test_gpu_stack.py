@@ -1,0 +1,90 @@
+"""Multi-date stacks on the GPU (photic_invert_stack): the kernel against the host build of
+the same engine with the same correctly rounded math (bit for bit on every pixel) and against
+the FP64 oracle (H within 1 cm and the other parameters within 1e-4 relative wherever both
+converge, or the same misfit where it is flat), the bucketing by valid-date count, no-data
+pixels, seeds, and the GPU stack generator against the oracle's."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import photic_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(**kw):
+    return P.cfg_default(reqmin=1e-10, konvge=100, kcount=2000, check_watchdog=1, **kw)
+
+
+@pytest.mark.parametrize("n_dates,k_min", [(8, 3), (4, 2)])
+def test_stack_bit_identical_to_host_engine(n_dates, k_min):
+    rrs, ss, valid, truth = O.gen_stack(31 + n_dates, 384, n_dates=n_dates, k_min=k_min)
+    g = P.invert_stack(_cfg(), rrs, ss, valid)
+    h = O.hostsim_invert_stack(P.const_bytes(_cfg()), rrs, ss, valid, math="cr")
+    print("dates %d: k histogram %s" % (n_dates, np.bincount(valid.sum(0))))
+    assert np.array_equal(g["status"], h["status"])
+    assert np.array_equal(g["x"], h["x"])
+    for k in ("n_iter", "n_evals", "n_starts", "rrs_error", "model_error", "depth"):
+        assert np.array_equal(g[k], h[k]), k
+
+
+def test_stack_against_oracle():
+    rrs, ss, valid, truth = O.gen_stack(41, 256, n_dates=6)
+    g = P.invert_stack(_cfg(), rrs, ss, valid)
+    o = O.invert_stack(rrs, ss, valid)
+    assert np.array_equal(g["status"], o["status"])
+    both = (o["status"] == 0) & (g["status"] == 0)
+    dH = np.abs(g["x"][0] - o["x"][0])
+    rel = np.abs(g["x"][1:] - o["x"][1:]) / np.maximum(np.abs(o["x"][1:]), 1e-300)
+    within = (dH < 0.01) & np.all((rel < 1e-4) | (o["x"][1:] == 0), axis=0)
+    same_fit = np.abs(g["rrs_error"] - o["rrs_error"]) <= 1e-6 * np.abs(o["rrs_error"])
+    exact = np.all(g["x"] == o["x"], axis=0)
+    print("stacks: %d converged, within %.4f, bit-identical %.4f" %
+          (both.sum(), within[both].mean(), exact[o["status"] >= 0].mean()))
+    assert np.all((within | same_fit)[both])
+    for k in ("k_min", "iod", "albedo", "bottom_type"):
+        sel = (o["status"] >= 0) & exact
+        assert np.allclose(g[k][sel], o[k][sel], rtol=1e-6, atol=1e-7), k
+
+
+def test_stack_seeded_and_nodata():
+    nd = 4
+    rrs, ss, valid, truth = O.gen_stack(43, 192, n_dates=nd, k_min=2)
+    valid[:, :5] = 0                    # no valid date
+    rrs[1, 2, 7] = -9999.0              # no-data value on a valid date
+    valid[1, 7] = 1
+    seed = np.zeros((2 + 3 * nd, rrs.shape[2]), np.float32)
+    seed[0], seed[1] = truth[0] * 0.9, truth[1]
+    for d in range(nd):
+        seed[2 + 3 * d:5 + 3 * d] = 100 * truth[2 + 4 * d:5 + 4 * d]
+    g = P.invert_stack(_cfg(), rrs, ss, valid, seed=seed)
+    h = O.hostsim_invert_stack(P.const_bytes(_cfg(), has_seed=True), rrs, ss, valid, seed=seed,
+                               math="cr")
+    assert np.all(g["status"][:5] == -1) and g["status"][7] == -1
+    assert np.all(g["b_pct"][:, :5] == -9999.0)
+    assert np.array_equal(g["status"], h["status"])
+    assert np.array_equal(g["x"], h["x"])
+    assert np.array_equal(g["n_starts"], h["n_starts"])
+
+
+def test_stack_generator_matches_oracle():
+    cfg = _cfg()
+    rrs, ss, valid, truth = P.gen_stack(cfg, 17, 2048, n_dates=8)
+    orrs, oss, ovalid, otruth = O.gen_stack(17, 2048, n_dates=8)
+    assert np.array_equal(valid[ovalid == valid], ovalid[ovalid == valid])
+    assert (valid == ovalid).mean() > 0.999
+    ok = (valid == 1) & (ovalid == 1)
+    d = np.abs(rrs.astype(np.float64) - orrs)[:, :, :][np.broadcast_to(ok[:, None, :], rrs.shape)]
+    ulp = np.spacing(np.abs(orrs[np.broadcast_to(ok[:, None, :], rrs.shape)]))
+    assert np.all(d <= ulp)
+    assert np.allclose(ss, oss, rtol=2e-7)
+    assert np.allclose(truth, otruth.astype(np.float32), rtol=2e-7)
+
+
+def test_stack_empty_and_unsupported():
+    out = P.invert_stack(_cfg(), np.zeros((3, 4, 0), np.float32), np.zeros((3, 0), np.float32),
+                         np.zeros((3, 0), np.uint8))
+    assert all(v.size == 0 for v in out.values())
+    with pytest.raises(P.PhoticError):
+        P.invert_stack(_cfg(mode=P.MODE_NATIVE, n_bottoms=2), np.zeros((2, 4, 8), np.float32),
+                       np.ones((2, 8), np.float32), np.ones((2, 8), np.uint8))

@@ -1,0 +1,87 @@
+"""Multi-date stacks on the CPU: the product's stack engine (photic_amd/csrc/ph_stackengine.h,
+built for the host by tests/hostsim) against the FP64 oracle bit for bit, and the oracle's
+multi-date problem against properties the reference's formulation fixes."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import photic_amd as P
+
+
+def _cfg():
+    return P.cfg_default(reqmin=1e-10, konvge=100, kcount=2000)
+
+
+def test_stack_engine_matches_oracle():
+    rrs, ss, valid, truth = O.gen_stack(5, 96)
+    h = O.hostsim_invert_stack(P.const_bytes(_cfg()), rrs, ss, valid)
+    o = O.invert_stack(rrs, ss, valid)
+    v = o["status"] >= 0
+    assert np.array_equal(h["status"], o["status"])
+    assert np.array_equal(h["x"], o["x"])
+    for k in ("n_iter", "n_starts", "rrs_error", "model_error", "depth"):
+        assert np.array_equal(h[k], o[k]), k
+    # the oracle evaluates each depth start once before nelmin, as the reference does
+    assert np.array_equal(h["n_evals"][v] + o["n_starts"][v], o["n_evals"][v])
+    ks = valid.sum(0)
+    assert len(set(ks[v])) >= 4  # several date counts exercised
+
+
+def test_stack_engine_seeded_matches_oracle():
+    rrs, ss, valid, truth = O.gen_stack(6, 48, n_dates=4, k_min=2)
+    nd = 4
+    seed = np.zeros((2 + 3 * nd, rrs.shape[2]), np.float32)
+    seed[0] = truth[0] * 1.2
+    seed[1] = truth[1]
+    for d in range(nd):
+        seed[2 + 3 * d:5 + 3 * d] = 100 * truth[2 + 4 * d:5 + 4 * d]
+    h = O.hostsim_invert_stack(P.const_bytes(_cfg(), has_seed=True), rrs, ss, valid, seed=seed)
+    o = O.invert_stack(rrs, ss, valid, seed=seed)
+    assert np.array_equal(h["x"], o["x"])
+    assert np.array_equal(h["n_iter"], o["n_iter"])
+    assert np.array_equal(h["n_starts"], o["n_starts"])
+
+
+def test_stack_cr_math_against_oracle():
+    """With the correctly rounded exp/log/pow the GPU uses: within the parity bar on every
+    converged pixel (or with the same misfit where the misfit is flat)."""
+    rrs, ss, valid, truth = O.gen_stack(7, 96, n_dates=6)
+    h = O.hostsim_invert_stack(P.const_bytes(_cfg()), rrs, ss, valid, math="cr")
+    o = O.invert_stack(rrs, ss, valid)
+    both = (o["status"] == 0) & (h["status"] == 0)
+    dH = np.abs(h["x"][0] - o["x"][0])
+    rel = np.abs(h["x"][1:] - o["x"][1:]) / np.maximum(np.abs(o["x"][1:]), 1e-300)
+    within = (dH < 0.01) & np.all((rel < 1e-4) | (o["x"][1:] == 0), axis=0)
+    same_fit = np.abs(h["rrs_error"] - o["rrs_error"]) <= 1e-6 * np.abs(o["rrs_error"])
+    assert np.all((within | same_fit)[both])
+    assert np.all(h["x"] == o["x"], axis=0)[o["status"] >= 0].mean() > 0.95
+
+
+def test_stack_of_identical_dates_is_the_single_date_problem():
+    """k copies of one date, with no K penalty in play (H >= 5, K_min <= 0.7): E_Rrs is a
+    mean over (date, band), so the k-date misfit equals the single-date one at the same H, B,
+    P, G, X (model/samodel.c:2545-2579)."""
+    rrs1, ss1, _, _, _ = O.gen(9, 8)
+    x1 = np.array([[7.0, 0.4, 3.0, 2.0, 1.0]])
+    for k in (2, 3):
+        xk = np.concatenate([[7.0, 0.4], np.tile([3.0, 2.0, 1.0], k)])
+        for p in range(8):
+            o1, e1 = O.objective(x1, rrs1[:, p:p + 1].T, ss1[p:p + 1])
+            r = np.repeat(rrs1[:, p][None, :], k, axis=0)
+            s = np.repeat(ss1[p:p + 1], k)
+            ok, ek = O.objective_stack(xk, r, s)
+            assert np.isclose(ek, e1[0], rtol=1e-13)
+            assert np.isclose(ok, o1[0], rtol=1e-13)
+
+
+def test_stack_generator_draws():
+    rrs, ss, valid, truth = O.gen_stack(11, 4000, n_dates=8, p_cloud=0.3, k_min=3)
+    k = valid.sum(0)
+    clear = (rrs >= 0).all(1)
+    assert np.all(valid <= clear)            # a negative date is never valid
+    assert np.all(k >= 1)
+    frac_valid = valid.mean()
+    assert 0.6 < frac_valid < 0.75           # Bernoulli(0.3) clouds, less negative dates
+    theta = truth[5::4]
+    assert np.allclose(ss, 1 / np.cos(np.radians(theta)), rtol=1e-6)
+    assert np.all((truth[0] >= 0.5) & (truth[0] <= 25)) and np.all((truth[1] >= 0) & (truth[1] <= 1))
