@@ -191,6 +191,26 @@ int photic_mlp_seed(photic_mlp *mlp, const photic_cfg *cfg, long n, const float 
  * model/samodel.c:466-478; *aw640 = a_w(640 nm) (may be NULL).  Host only, no GPU needed. */
 int photic_band_coefficients(const photic_cfg *cfg, double *out, double *aw640);
 
+/* ---- the launch-shape autotuner ----------------------------------------------------
+ * Times the inversion on a device-resident sample (n_pix >= 64 pixels or stacks) for every
+ * candidate shape -- lanes per block 64 / 128 / 192 / 256 (as the simplexes' LDS allows),
+ * the occupancy's blocks per CU or half of them, queue chunks of 16 .. 256 pixels; for
+ * stacks blocks of 64 / 128 / 256 at full occupancy and chunks of 16 and 64 -- and
+ * keeps the fastest for the kernel instance cfg selects (precision, layout, seeded or not).
+ * Later calls whose cfg->block, grid and chunk are 0 use it.  Per-pixel results do not
+ * depend on the shape.  Process-wide; not to be run concurrently with inversions. */
+typedef struct photic_tuning {
+    int block, blocks_per_cu, chunk;
+    float ms;     /* the winner's time on the sample */
+    int n_tried;  /* shapes timed */
+} photic_tuning;
+
+int photic_autotune(const photic_cfg *cfg, long n_pix, const float *rrs, const float *sec_sun,
+                    const float *seed, photic_tuning *best, void *stream);
+int photic_autotune_stack(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
+                          const float *sec_sun, const unsigned char *valid, const float *seed,
+                          photic_tuning *best, void *stream);
+
 /* Launch-shape facts of the compiled inversion kernel for cfg. */
 int photic_kernel_info(const photic_cfg *cfg, int *vgprs, int *scratch_bytes,
                        int *blocks_per_cu, int *n_cu);

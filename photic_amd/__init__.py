@@ -71,6 +71,11 @@ class Outputs(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("chunk", C.c_int),
+                ("ms", C.c_float), ("n_tried", C.c_int)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError("libphotic.so is not built: run photic_amd/build.py (or "
@@ -117,6 +122,9 @@ def _load():
         "photic_band_coefficients": (i, [C.POINTER(Cfg), vp, C.POINTER(d)]),
         "photic_invert_stack": (i, [C.POINTER(Cfg), l, i, vp, vp, vp, vp, C.POINTER(Outputs),
                                     vp]),
+        "photic_autotune": (i, [C.POINTER(Cfg), l, vp, vp, vp, C.POINTER(Tuning), vp]),
+        "photic_autotune_stack": (i, [C.POINTER(Cfg), l, i, vp, vp, vp, vp, C.POINTER(Tuning),
+                                      vp]),
         "photic_gen_stack": (i, [C.POINTER(Cfg), C.c_uint64, l, l, i, d, d, i, vp, vp, vp, vp,
                                  vp]),
         "ph_build_const": (i, [C.POINTER(Cfg), i, i, vp]),
@@ -247,6 +255,24 @@ def invert_device(cfg, n, d_rrs, d_sec_sun=None, d_seed=None, outs=None, stream=
     check(lib.photic_invert(C.byref(cfg), n, d_rrs, d_sec_sun, d_sec_view, d_seed,
                             C.byref(outs.struct), stream))
     return outs
+
+
+def autotune(cfg, n, d_rrs, d_sec_sun=None, d_seed=None, stream=None):
+    """Choose and keep the launch shape for cfg's kernel instance on a device-resident sample
+    of n pixels; returns the winner as a dict."""
+    t = Tuning()
+    check(lib.photic_autotune(C.byref(cfg), n, d_rrs, d_sec_sun, d_seed, C.byref(t), stream))
+    return {"block": t.block, "blocks_per_cu": t.blocks_per_cu, "chunk": t.chunk,
+            "ms": t.ms, "n_tried": t.n_tried}
+
+
+def autotune_stack(cfg, n, n_dates, d_rrs, d_sec_sun, d_valid, d_seed=None, stream=None):
+    """The same for the stack kernels, on a sample of n stacks."""
+    t = Tuning()
+    check(lib.photic_autotune_stack(C.byref(cfg), n, n_dates, d_rrs, d_sec_sun, d_valid, d_seed,
+                                    C.byref(t), stream))
+    return {"block": t.block, "blocks_per_cu": t.blocks_per_cu, "chunk": t.chunk,
+            "ms": t.ms, "n_tried": t.n_tried}
 
 
 def invert(cfg, rrs, sec_sun=None, seed=None, fields=None):

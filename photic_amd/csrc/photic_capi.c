@@ -284,6 +284,14 @@ int photic_invert(const photic_cfg *cfg, long n_pix, const float *rrs, const flo
     return photic_invert_h(cfg, n_pix, rrs, sec_sun, sec_view, seed, NULL, out, stream);
 }
 
+/* Launch shapes chosen by photic_autotune / photic_autotune_stack, per kernel instance
+ * (zero: not tuned, the defaults apply).  Results never depend on the shape. */
+typedef struct {
+    int block, bpc, chunk;
+} ph_shape;
+static ph_shape ph_tuned_single[2][4][2]; /* [fp32][n_bottoms (0 = mix)][seeded] */
+static ph_shape ph_tuned_stack[2];        /* [seeded] */
+
 int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const float *sec_sun,
                     const float *sec_view, const float *seed, const float *hstart,
                     const photic_outputs *out, void *stream)
@@ -319,8 +327,10 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
                 "hipDeviceGetAttribute");
     if (rc) return rc;
     const int lds_lane = ph_lds_bytes(nbot, 1);
+    const ph_shape *tu = &ph_tuned_single[fp32][nbot][seed != NULL];
     int block = cfg->block > 0 ? cfg->block : (max_lds / lds_lane) / 64 * 64;
     if (cfg->block <= 0 && block > 256) block = 256;
+    if (cfg->block <= 0 && tu->block > 0) block = tu->block;
     if (block < 64 || block % 64 != 0 || block > 256)
         return ph_fail(PHOTIC_EINVAL, "block %d must be a multiple of 64 in 64..256", block);
     if ((long)lds_lane * block > max_lds)
@@ -332,6 +342,7 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
     rc = ph_kernel_occupancy(fp32, a.c.n_bands, nbot, seed != NULL, block, &bpc);
     if (rc) return ph_launch_status(rc, "occupancy");
     if (bpc < 1) return ph_fail(PHOTIC_EUNSUPPORTED, "kernel does not fit a CU at block %d", block);
+    if (cfg->grid <= 0 && cfg->block <= 0 && tu->bpc > 0 && tu->bpc < bpc) bpc = tu->bpc;
     long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
     long waves_needed = (n_pix + 63) / 64;
     long blocks_needed = (waves_needed * 64 + block - 1) / block;
@@ -344,7 +355,7 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
     a.sec_view = sec_view;
     a.seed = seed;
     a.hstart = hstart;
-    a.chunk = cfg->chunk > 0 ? cfg->chunk : 64;
+    a.chunk = cfg->chunk > 0 ? cfg->chunk : (tu->chunk > 0 ? tu->chunk : 64);
     a.depth = out->depth;
     a.rrs_error = out->rrs_error;
     a.model_error = out->model_error;
@@ -423,7 +434,8 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
     a.sec_sun = sec_sun;
     a.valid = valid;
     a.seed = seed;
-    a.chunk = cfg->chunk > 0 ? cfg->chunk : 64;
+    a.chunk = cfg->chunk > 0 ? cfg->chunk : (ph_tuned_stack[seed != NULL].chunk > 0
+                                                 ? ph_tuned_stack[seed != NULL].chunk : 64);
     a.depth = out->depth;
     a.rrs_error = out->rrs_error;
     a.model_error = out->model_error;
@@ -448,12 +460,14 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
     a.max_ticks = (unsigned long long)(limit_s * 1000.0 * (double)wrate);
     rc = ph_n_cu(&n_cu);
     if (rc) return rc;
-    const int block = cfg->block > 0 ? cfg->block : 256;
+    const ph_shape *tu = &ph_tuned_stack[seed != NULL];
+    const int block = cfg->block > 0 ? cfg->block : (tu->block > 0 ? tu->block : 256);
     if (block < 64 || block > 256 || block % 64)
         return ph_fail(PHOTIC_EINVAL, "block %d must be a multiple of 64 in 64..256", block);
     rc = ph_launch_status(ph_stack_occupancy(n_dates, seed != NULL, block, &bpc), "occupancy");
     if (rc) return rc;
     if (bpc < 1) bpc = 1;
+    if (cfg->grid <= 0 && cfg->block <= 0 && tu->bpc > 0 && tu->bpc < bpc) bpc = tu->bpc;
     long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
     const long lanes = grid * block;
     a.work_lanes = lanes;
@@ -527,6 +541,125 @@ int photic_gen_stack(const photic_cfg *cfg, uint64_t seed, long pix0, long n, in
     return ph_launch_status(ph_launch_gen_stack(&c, seed, pix0, n, n_dates, sigma, p_cloud,
                                                 k_min, rrs, sec_sun, valid, truth, stream),
                             "stack generator");
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* the launch-shape autotuner                                                             */
+/* ------------------------------------------------------------------------------------ */
+
+/* Time one inversion call of the candidate shape on `st` (HIP events), in ms. */
+static int ph_time_call(int stack, const photic_cfg *c, long n, int n_dates, const float *rrs,
+                        const float *ss, const unsigned char *valid, const float *seed,
+                        const photic_outputs *o, hipStream_t st, float *ms)
+{
+    hipEvent_t e0, e1;
+    int rc = ph_hip(hipEventCreate(&e0), "hipEventCreate");
+    if (rc) return rc;
+    rc = ph_hip(hipEventCreate(&e1), "hipEventCreate");
+    if (rc) {
+        hipEventDestroy(e0);
+        return rc;
+    }
+    rc = ph_hip(hipEventRecord(e0, st), "hipEventRecord");
+    if (!rc)
+        rc = stack ? photic_invert_stack(c, n, n_dates, rrs, ss, valid, seed, o, st)
+                   : photic_invert_h(c, n, rrs, ss, NULL, seed, NULL, o, st);
+    if (!rc) rc = ph_hip(hipEventRecord(e1, st), "hipEventRecord");
+    if (!rc) rc = ph_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+    if (!rc) rc = ph_hip(hipEventElapsedTime(ms, e0, e1), "hipEventElapsedTime");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+static int ph_autotune(int stack, const photic_cfg *cfg, long n_pix, int n_dates,
+                       const float *rrs, const float *ss, const unsigned char *valid,
+                       const float *seed, photic_tuning *best, void *stream)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    if (!cfg || n_pix < 64 || !rrs) return ph_fail(PHOTIC_EINVAL, "a sample of >= 64 pixels");
+    const int fp32 = cfg->precision == PHOTIC_FP32;
+    const int nbot = cfg->mode == PHOTIC_MODE_MIX ? 0 : cfg->n_bottoms;
+    if (nbot < 0 || nbot > 3) return ph_fail(PHOTIC_EINVAL, "n_bottoms %d", nbot);
+    ph_shape *slot = stack ? &ph_tuned_stack[seed != NULL] : &ph_tuned_single[fp32][nbot][seed != NULL];
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0, max_lds = 0, n_cu = 0;
+    rc = ph_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (!rc) rc = ph_hip(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "attr");
+    if (!rc) rc = ph_n_cu(&n_cu);
+    if (rc) return rc;
+    int *d_status = NULL;
+    rc = ph_hip(hipMalloc((void **)&d_status, sizeof(int) * (size_t)n_pix), "hipMalloc");
+    if (rc) return rc;
+    photic_outputs o;
+    memset(&o, 0, sizeof(o));
+    o.status = d_status;
+    photic_cfg c = *cfg;
+    c.check_watchdog = 0;
+    const int blocks[4] = {64, 128, 192, 256};
+    const int chunks[5] = {16, 64, 32, 128, 256};
+    ph_shape win = {0, 0, 0};
+    float win_ms = 1e30f, ms = 0.0f;
+    int tried = 0;
+    *slot = win; /* candidates run with explicit shapes */
+    /* stacks: memory-latency-bound, so only the block size and chunk are swept at full
+     * occupancy (fewer lanes in flight only lose) */
+    const int n_blocks = 4, n_chunks = stack ? 2 : 5, n_halves = stack ? 1 : 2;
+    for (int bi = 0; bi < n_blocks && !rc; bi++) {
+        const int block = blocks[bi];
+        if (stack && block == 192) continue;
+        if (!stack && (long)ph_lds_bytes(nbot, 1) * block > max_lds) continue;
+        int bpc = 0;
+        rc = stack ? ph_launch_status(ph_stack_occupancy(n_dates, seed != NULL, block, &bpc), "occupancy")
+                   : ph_launch_status(ph_kernel_occupancy(fp32, cfg->n_bands, nbot, seed != NULL,
+                                                          block, &bpc), "occupancy");
+        if (rc || bpc < 1) continue;
+        for (int half = 0; half < n_halves && !rc; half++) {
+            const int b = half ? bpc / 2 : bpc;
+            if (b < 1 || (half && b == bpc)) continue;
+            for (int ci = 0; ci < n_chunks && !rc; ci++) {
+                c.block = block;
+                c.grid = n_cu * b;
+                c.chunk = chunks[ci];
+                if (tried == 0) /* first launch of the instance: load the code object */
+                    rc = ph_time_call(stack, &c, n_pix, n_dates, rrs, ss, valid, seed, &o, st, &ms);
+                if (!rc)
+                    rc = ph_time_call(stack, &c, n_pix, n_dates, rrs, ss, valid, seed, &o, st, &ms);
+                tried++;
+                if (!rc && ms < win_ms) {
+                    win_ms = ms;
+                    win.block = block;
+                    win.bpc = b;
+                    win.chunk = chunks[ci];
+                }
+            }
+        }
+    }
+    hipFree(d_status);
+    if (rc) return rc;
+    *slot = win;
+    if (best) {
+        best->block = win.block;
+        best->blocks_per_cu = win.bpc;
+        best->chunk = win.chunk;
+        best->ms = win_ms;
+        best->n_tried = tried;
+    }
+    return PHOTIC_OK;
+}
+
+int photic_autotune(const photic_cfg *cfg, long n_pix, const float *rrs, const float *sec_sun,
+                    const float *seed, photic_tuning *best, void *stream)
+{
+    return ph_autotune(0, cfg, n_pix, 1, rrs, sec_sun, NULL, seed, best, stream);
+}
+
+int photic_autotune_stack(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
+                          const float *sec_sun, const unsigned char *valid, const float *seed,
+                          photic_tuning *best, void *stream)
+{
+    return ph_autotune(1, cfg, n_pix, n_dates, rrs, sec_sun, valid, seed, best, stream);
 }
 
 int photic_gen_scene(const photic_cfg *cfg, uint64_t seed, long pix0, long n, double sigma,
