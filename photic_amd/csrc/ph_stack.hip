@@ -125,6 +125,8 @@ __global__ void __launch_bounds__(256) ph_stack_kernel(PhStackArgs a)
     PhStackNM<N> S;
     S.base = a.work + (gl >> 6) * (long)PhStackNM<N>::n_slots * 64 + lane;
     S.stride = 64;
+#pragma unroll
+    for (int k = 0; k <= N; k++) S.yv[k] = 0.0;
     S.phase = PH_IDLE;
     long p = -1;
     unsigned int slots = 0; // date index of valid date s in bits 4s..4s+3

@@ -5,8 +5,10 @@
 // model/samodel.c:2119-2427), for the mix layout over ND dates: [H, B, (P, G, X) per date],
 // N = 2 + 3 ND parameters.  With N up to 26 the simplex, steps, best point, trial points and
 // vertex values (about 780 doubles) cannot be registers, so they live in the lane's slice of
-// memory and the loops over them stay loops; only the scalars of the state machine and the
-// per-pixel constants are registers.  Every arithmetic operation and its order is the one of
+// memory; the vertex values, which every iteration scans, and the scalars of the state machine
+// are registers.  Every memory pass is shaped to have its loads in flight together: the
+// centroid is accumulated one simplex row at a time (N loads per round trip, the per-
+// coordinate summation order unchanged).  Every arithmetic operation and its order is the one of
 // ph_engine.h, so the two engines give identical results where both apply (ND = 1).
 #pragma once
 
@@ -23,9 +25,8 @@ struct PhStackNM {
     static constexpr int XS0 = BEST0 + N;        // start point / reflected point
     static constexpr int XB0 = XS0 + N;          // centroid / local-minimum probe
     static constexpr int XE0 = XB0 + N;          // point being evaluated
-    static constexpr int Y0 = XE0 + N;           // vertex values, [N + 1]
     static constexpr int ND = (N - 2) / 3;       // dates
-    static constexpr int RRS0 = Y0 + N + 1;      // measured Rrs, [ND][4]
+    static constexpr int RRS0 = XE0 + N;         // measured Rrs, [ND][4]
     static constexpr int PW0 = RRS0 + 4 * ND;    // (440 / lambda)^Y, [ND][4]
     static constexpr int SS0 = PW0 + 4 * ND;     // sec(sun), [ND]
     static constexpr int SV0 = SS0 + ND;         // sec(view), [ND]
@@ -47,14 +48,13 @@ struct PhStackNM {
     PH_INLINE void set_xb(int i, double v) const { st(XB0 + i, v); }
     PH_INLINE double xe(int i) const { return ld(XE0 + i); }
     PH_INLINE void set_xe(int i, double v) const { st(XE0 + i, v); }
-    PH_INLINE double y(int k) const { return ld(Y0 + k); }
-    PH_INLINE void set_y(int k, double v) const { st(Y0 + k, v); }
     PH_INLINE double rrs(int s, int b) const { return ld(RRS0 + 4 * s + b); }
     PH_INLINE double pw(int s, int b) const { return ld(PW0 + 4 * s + b); }
     PH_INLINE double ss(int s) const { return ld(SS0 + s); }
     PH_INLINE double sv(int s) const { return ld(SV0 + s); }
 
-    double tot; // sum of measured Rrs in (date, band) order
+    double yv[N + 1]; // vertex values (registers: every iteration scans them)
+    double tot;       // sum of measured Rrs in (date, band) order
     double ylo, ystar, ynewlo, del, lowest, yr;
     int ilo, ihi, icount, jcount, numres, ifault, phase, sub, kstart, nstart;
     int best_icount, best_conv, evals;
@@ -299,8 +299,7 @@ PH_INLINE void ph_stack_nm_consume(PhStackNM<N>& S, const PhConst& c, double f)
             mv_phase = PH_EXPAND;
         } else {
             int l = 0;
-#pragma unroll
-            for (int i = 0; i <= N; i++) l += (f < S.y(i)) ? 1 : 0;
+            ph_for<N + 1>([&](auto i) PH_BODY { l += (f < S.yv[i]) ? 1 : 0; });
             if (l > 1) {
                 wr = true;
                 wj = S.ihi;
@@ -323,7 +322,7 @@ PH_INLINE void ph_stack_nm_consume(PhStackNM<N>& S, const PhConst& c, double f)
         wf = ph_pick(wr_xs, S.ystar, f);
         act = A_ITER_END;
     } else if (ph == PH_CONTRACT_IN) {
-        if (S.y(S.ihi) < f) {
+        if (ph_sel(S.yv, S.ihi) < f) {
             S.sub = 0;
             act = A_SHRINK_NEXT;
         } else {
@@ -351,7 +350,7 @@ PH_INLINE void ph_stack_nm_consume(PhStackNM<N>& S, const PhConst& c, double f)
     // ---- the vertex write ----
     if (wr) {
         for (int i = 0; i < N; i++) S.set_p(wj, i, wr_xs ? S.xs(i) : S.xe(i));
-        S.set_y(wj, wf);
+        ph_for<N + 1>([&](auto k) PH_BODY { S.yv[k] = ph_pick(wj == (int)k, wf, S.yv[k]); });
     }
 
     // ---- follow-up ----
@@ -417,21 +416,20 @@ PH_INLINE void ph_stack_nm_consume(PhStackNM<N>& S, const PhConst& c, double f)
 
     bool iter_begin = false, nm_finish = false;
     if (act == A_AFTER_BUILD) {
-        S.ylo = S.y(0);
+        S.ylo = S.yv[0];
         S.ilo = 0;
-#pragma unroll
-        for (int i = 1; i <= N; i++) {
-            const double yi = S.y(i);
-            if (yi < S.ylo) {
-                S.ylo = yi;
+        ph_for<N>([&](auto i0) PH_BODY {
+            constexpr int i = i0 + 1;
+            if (S.yv[i] < S.ylo) {
+                S.ylo = S.yv[i];
                 S.ilo = i;
             }
-        }
+        });
         iter_begin = true;
     }
     // ---- end of an iteration: asa047.c:397-434 ----
     if (act == A_ITER_END) {
-        const double yhi = S.y(S.ihi);
+        const double yhi = ph_sel(S.yv, S.ihi);
         if (yhi < S.ylo) {
             S.ylo = yhi;
             S.ilo = S.ihi;
@@ -442,15 +440,10 @@ PH_INLINE void ph_stack_nm_consume(PhStackNM<N>& S, const PhConst& c, double f)
         } else if (S.icount <= c.kcount) {
             S.jcount = c.konvge;
             double z = 0.0;
-#pragma unroll
-            for (int i = 0; i <= N; i++) z = z + S.y(i);
+            ph_for<N + 1>([&](auto i) PH_BODY { z = z + S.yv[i]; });
             const double xm = ph_div_const(z, dnn, 1.0 / dnn);
             z = 0.0;
-#pragma unroll
-            for (int i = 0; i <= N; i++) {
-                const double yi = S.y(i);
-                z = z + (yi - xm) * (yi - xm);
-            }
+            ph_for<N + 1>([&](auto i) PH_BODY { z = z + (S.yv[i] - xm) * (S.yv[i] - xm); });
             if (z <= c.rq) nm_finish = true;
             else iter_begin = true;
         } else {
@@ -462,29 +455,39 @@ PH_INLINE void ph_stack_nm_consume(PhStackNM<N>& S, const PhConst& c, double f)
         if (c.kcount <= S.icount) {
             nm_finish = true;
         } else {
-            S.ynewlo = S.y(0);
+            S.ynewlo = S.yv[0];
             S.ihi = 0;
-#pragma unroll
-            for (int i = 1; i <= N; i++) {
-                const double yi = S.y(i);
-                if (S.ynewlo < yi) {
-                    S.ynewlo = yi;
+            ph_for<N>([&](auto i0) PH_BODY {
+                constexpr int i = i0 + 1;
+                if (S.ynewlo < S.yv[i]) {
+                    S.ynewlo = S.yv[i];
                     S.ihi = i;
                 }
-            }
+            });
             const int hi = S.ihi;
-            for (int i = 0; i < N; i++) {
-                double z = 0.0;
-#pragma unroll
-                for (int k = 0; k <= N; k++) z = z + S.p(k, i);
-                const double phi = S.p(hi, i);
-                z = z - phi;
-                const double b = ph_div_const(z, dn, 1.0 / dn);
-                const double r = b + 1.0 * (b - phi);
+            // centroid: z_i = sum over vertices k = 0..N of p(k, i), in k order, accumulated
+            // a row at a time; phi_i = p(hi, i) taken on the way
+            double z[N], phi[N];
+            ph_for<N>([&](auto i) PH_BODY {
+                z[i] = 0.0;
+                phi[i] = 0.0;
+            });
+#pragma unroll 2
+            for (int k = 0; k <= N; k++) {
+                ph_for<N>([&](auto i) PH_BODY {
+                    const double v = S.p(k, i);
+                    z[i] = z[i] + v;
+                    phi[i] = ph_pick(k == hi, v, phi[i]);
+                });
+            }
+            ph_for<N>([&](auto i) PH_BODY {
+                const double zi = z[i] - phi[i];
+                const double b = ph_div_const(zi, dn, 1.0 / dn);
+                const double r = b + 1.0 * (b - phi[i]);
                 S.set_xb(i, b);
                 S.set_xs(i, r);
                 S.set_xe(i, r);
-            }
+            });
             S.phase = PH_REFLECT;
             return;
         }
@@ -493,7 +496,7 @@ PH_INLINE void ph_stack_nm_consume(PhStackNM<N>& S, const PhConst& c, double f)
     if (nm_finish) {
         const int lo = S.ilo;
         for (int i = 0; i < N; i++) S.set_xb(i, S.p(lo, i));
-        S.ynewlo = S.y(lo);
+        S.ynewlo = ph_sel(S.yv, lo);
         S.yr = ph_trunc10(S.ynewlo);
         if (c.kcount < S.icount) {
             S.ifault = 2;

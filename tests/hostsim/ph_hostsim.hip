@@ -99,6 +99,7 @@ static void run_stack_pixel(PhConst c, long n_pix, int n_dates, long p, const in
     PhStackNM<N> S;
     S.base = mem.data();
     S.stride = 1;
+    for (int k = 0; k <= N; k++) S.yv[k] = 0.0;
     float r[K][4];
     double ss[K], sv[K];
     for (int s = 0; s < K; s++) {
