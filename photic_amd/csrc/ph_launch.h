@@ -72,8 +72,6 @@ typedef struct PhStackArgs {
     const float *seed;          /* [2 + 3 D][n_pix]: H, B, (P, G, X) per date slot, or NULL */
     const int *idx;             /* the bucket's pixel indices */
     const unsigned int *count;  /* the bucket's size (device) */
-    double *work;               /* per-lane simplex memory, [wave][slot][64 lanes] */
-    long work_lanes;            /* lanes the work memory holds (grid * block) */
     unsigned long long *queue;  /* [0] work counter, [1] watchdog flag; zero at launch */
     int chunk;
     unsigned long long max_ticks;
@@ -110,7 +108,6 @@ int ph_launch_stack_bucket(const PhStackArgs *a, unsigned int *counts, int *list
 int ph_launch_stack(const PhStackArgs *a, int k, int seeded, int grid, int block,
                     void *stream);
 int ph_stack_occupancy(int k, int seeded, int block, int *blocks_per_cu);
-int ph_stack_slots(int k);
 int ph_launch_gen_stack(const PhConst *c, uint64_t seed, long pix0, long n, int n_dates,
                         double sigma, double p_cloud, int k_min, float *rrs, float *sec_sun,
                         unsigned char *valid, float *truth, void *stream);

@@ -7,20 +7,15 @@
 //
 //   ph_stack_bucket_kernel  sorts the pixels into buckets by k (wave-aggregated atomics) and
 //                           writes the no-data outputs of pixels with no valid date
-//   ph_stack_kernel<K, S>   the lane engine of ph_stackengine.h with K dates compiled in: one
-//                           persistent launch per bucket pulls its pixels from a work queue.
-//                           A simplex of up to 27 x 26 doubles does not fit LDS at useful
-//                           occupancy, so the Nelder-Mead arrays live in a per-lane slice of
-//                           global memory, each wavefront's slice contiguous ([slot][64
-//                           lanes]): one slot of a wave is one 512-byte run
+//   ph_stack_kernel<K, S>   the group engine of ph_grpengine.h with K dates compiled in: one
+//                           persistent launch per bucket pulls its stacks from a work queue,
+//                           one stack per group of 8, 16 or 32 lanes, the simplex in LDS
 //   ph_gen_stack_kernel     the synthetic stack scene
 #include <hip/hip_runtime.h>
 
 #include "ph_gen.h"
 #include "ph_launch.h"
-#include "ph_stackengine.h"
-
-static constexpr int PH_STACK_NB = 4;
+#include "ph_grpengine.h"
 
 __device__ inline void ph_stack_write_nodata(const PhStackArgs& a, long p, int n_params)
 {
@@ -67,23 +62,22 @@ __global__ void ph_stack_bucket_kernel(PhStackArgs a, unsigned int* counts, int*
     if (live && k == 0) ph_stack_write_nodata(a, p, 2 + 3 * a.n_dates);
 }
 
+// Outputs of a finished stack: lane li < N writes its coordinate at its date slot (zeros at the
+// slots of clouded dates), lane 0 the scalars.
 template <int ND>
 __device__ __attribute__((always_inline)) inline void ph_stack_write_outputs(
-    const PhStackArgs& a, long p, unsigned int slots, const PhStackNM<2 + 3 * ND>& S,
-    const PhStackSide& side)
+    const PhStackArgs& a, long p, unsigned int vmask, int dst, const PhGrp<ND>& S, double B,
+    const PhGrpSide& side)
 {
-    constexpr int N = 2 + 3 * ND;
+    constexpr int N = PhGrpLayout<ND>::N, G = PhGrpLayout<ND>::G;
+    const int li = S.li;
     if (a.x_out) {
-        for (int i = 0; i < 2 + 3 * a.n_dates; i++) a.x_out[(long)i * a.n_pix + p] = 0.0;
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            const int s = i < 2 ? 0 : (i - 2) / 3;
-            const int d = (int)((slots >> (4 * s)) & 15u);
-            const int dst = i < 2 ? i : 2 + 3 * d + (i - 2) % 3;
-            a.x_out[(long)dst * a.n_pix + p] = fabs(S.best(i));
-        }
+        if (li < N) a.x_out[(long)dst * a.n_pix + p] = fabs(S.best);
+        for (int i = 2 + li; i < 2 + 3 * a.n_dates; i += G)
+            if (!((vmask >> ((i - 2) / 3)) & 1u)) a.x_out[(long)i * a.n_pix + p] = 0.0;
     }
-    const double H = fabs(S.best(0));
+    if (li != 0) return;
+    const double H = fabs(S.best);
     // one region: origin weight sqrt(1), depth = H (model/samodel.c:2000-2017)
     if (a.depth) a.depth[p] = (float)((0.0 + 1.0 * H) / (1.0 + 1.0 - 1.0));
     if (a.rrs_error) a.rrs_error[p] = (float)side.Rrs_error;
@@ -93,7 +87,7 @@ __device__ __attribute__((always_inline)) inline void ph_stack_write_outputs(
     if (a.iod) a.iod[p] = (float)(100.0 * side.iod_sum / side.nobs);
     if (a.b_pct || a.bottom_type) {
         // the mix's two bottoms: q = (|B|, |1 - B|) (model/samodel.c:2028-2047)
-        const double q0 = fabs(S.best(1)), q1 = fabs(1.0 - S.best(1));
+        const double q0 = fabs(B), q1 = fabs(1.0 - B);
         const double qs = q0 + q1;
         const double p0 = 100.0 * q0 / qs, p1 = 100.0 * q1 / qs;
         double largest = 0.0, bt = 0.0;
@@ -112,24 +106,112 @@ __device__ __attribute__((always_inline)) inline void ph_stack_write_outputs(
     if (a.n_starts) a.n_starts[p] = S.nstart;
 }
 
-// SEED: every pixel's first start is its seed (a.seed at the pixel's date slots).
+// Load stack p into the group (model/samodel.c:1785-1799, 2898-2903, per date): pair data,
+// the start values of B, P, G, X, the sum of the measured Rrs.  False when any value is no-data
+// or negative (model/samodel.c:933-947).  *dst: coordinate li's row of x_out and of the seed.
+template <int ND>
+__device__ __attribute__((always_inline)) inline bool ph_stack_load(
+    PhGrp<ND>& S, const PhStackArgs& a, long p, unsigned long long gmask, int* dst,
+    unsigned int* vmask_out)
+{
+    using L = PhGrpLayout<ND>;
+    constexpr int N = L::N, NP = L::NP;
+    const PhConst& c = a.c;
+    const int li = S.li;
+    unsigned int slots = 0, vmask = 0; // slot s's date in bits 4s..4s+3
+    int s = 0;
+    for (int d = 0; d < a.n_dates; d++) {
+        if (!a.valid[(long)d * a.n_pix + p]) continue;
+        vmask |= 1u << d;
+        if (s < ND) slots |= (unsigned int)d << (4 * s);
+        s++;
+    }
+    *vmask_out = vmask;
+    const int b = li & 3;
+    const int sp = li < NP ? (li >> 2) : 0;                 // the pair's date slot
+    const int sc = (li >= 2 && li < N) ? (li - 2) / 3 : 0; // the coordinate's date slot
+    const long dp = (long)((slots >> (4 * sp)) & 15u), dc = (long)((slots >> (4 * sc)) & 15u);
+    double rp[4], rc[4];
+#pragma unroll
+    for (int bb = 0; bb < 4; bb++) {
+        rp[bb] = (double)a.rrs[(dp * 4 + bb) * a.n_pix + p];
+        rc[bb] = (double)a.rrs[(dc * 4 + bb) * a.n_pix + p];
+    }
+    const double rv = ph_band_sel(rp, b);
+    const float v = (float)rv;
+    const bool bad = li < NP && (ph_approx_equal(v, (float)c.nodata, 1.0e-6f) || v < 0.0f);
+    if (__ballot(bad) & gmask) return false;
+
+    // the sum of the measured Rrs, in (date, band) order
+    ph_wave_sync();
+    if (li < NP) S.g[L::D0 + li] = rv;
+    ph_wave_sync();
+    double tot = 0.0;
+#pragma unroll
+    for (int j = 0; j < NP; j++) tot += S.g[L::D0 + j];
+    S.tot = tot;
+
+    // pair li: its Rrs, (440 / lambda)^Y of its date, geometry, band coefficients
+    S.rrs = rv;
+    double r440 = ph_rrs_at(c, rp, 0);
+    const double r490 = ph_rrs_at(c, rp, 1);
+    if (r440 < 0.0) r440 = 0.0001;
+    const double chi = r440 / r490;
+    double Y = 3.44 * (1.0 - 3.17 * PH_EXP(-2.01 * chi));
+    if (Y < 0.0) Y = 0.0;
+    if (Y > 2.5) Y = 2.5;
+    S.pw = PH_POW(440.0 / ph_band_sel(c.wl, b), Y);
+    S.ss = (double)a.sec_sun[dp * a.n_pix + p];
+    S.sv = 1.0;
+    S.a0 = ph_band_sel(c.a0, b);
+    S.a1 = ph_band_sel(c.a1, b);
+    S.aw = ph_band_sel(c.aw, b);
+    S.bbw = ph_band_sel(c.bbw, b);
+    S.eg = ph_band_sel(c.eg, b);
+    S.bot0 = ph_band_sel(c.bot[0], b);
+    S.bot1 = ph_band_sel(c.bot[1], b);
+
+    // coordinate li: B = 0.5 (step 0.5); per date P from the 490/550 ratio, G = 1.5 P, X from
+    // a_w(640) Rrs(640), steps twice the start (model/samodel.c:2246-2354)
+    S.xs0 = 0.0;
+    S.step0 = 0.0;
+    if (li == 1) {
+        S.xs0 = 0.5;
+        S.step0 = 0.5;
+    } else if (li >= 2 && li < N) {
+        const double m490 = ph_rrs_at(c, rc, 1) / (double)1;
+        const double m550 = ph_rrs_at(c, rc, 2) / (double)1;
+        const double m640 = ph_rrs_at(c, rc, 3) / (double)1;
+        const double P0 = 100.0 * 0.072 * PH_POW(m490 / m550, -1.7);
+        const double G0 = 1.5 * P0;
+        const double X0 = 100.0 * 30.0 * c.aw640 * m640;
+        const int comp = (li - 2) % 3;
+        S.xs0 = comp == 0 ? P0 : (comp == 1 ? G0 : X0);
+        S.step0 = 2.0 * S.xs0;
+    }
+    *dst = li < 2 ? li : 2 + 3 * (int)dc + (li - 2) % 3;
+    return true;
+}
+
+// SEED: every stack's first start is its seed (a.seed at the stack's date slots).  Needs
+// blockDim.x a multiple of 64 and (blockDim.x / G) * PhGrpLayout<ND>::n doubles of dynamic LDS.
 template <int ND, bool SEED>
 __global__ void __launch_bounds__(256) ph_stack_kernel(PhStackArgs a)
 {
-    constexpr int NB = PH_STACK_NB;
-    constexpr int N = PhLayout<0>::n<ND>();
+    using L = PhGrpLayout<ND>;
+    constexpr int N = L::N, G = L::G;
+    extern __shared__ double ph_stack_lds[];
     const PhConst& c = a.c;
     const int lane = (int)__lane_id();
-    const long gl = (long)blockIdx.x * blockDim.x + threadIdx.x;
-
-    PhStackNM<N> S;
-    S.base = a.work + (gl >> 6) * (long)PhStackNM<N>::n_slots * 64 + lane;
-    S.stride = 64;
-#pragma unroll
-    for (int k = 0; k <= N; k++) S.yv[k] = 0.0;
+    PhGrp<ND> S;
+    S.li = lane & (G - 1);
+    S.g = ph_stack_lds + (long)(threadIdx.x / G) * L::n;
     S.phase = PH_IDLE;
+    const int gbase = lane & ~(G - 1);
+    const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << gbase;
     long p = -1;
-    unsigned int slots = 0; // date index of valid date s in bits 4s..4s+3
+    int dst = 0;
+    unsigned int vmask = 0;
 
     const long m = (long)*a.count;
     long q_next = 0, q_end = 0;
@@ -141,14 +223,14 @@ __global__ void __launch_bounds__(256) ph_stack_kernel(PhStackArgs a)
             if (lane == 0) atomicOr(a.queue + 1, 1ull);
             break;
         }
-        // ---- refill idle lanes from the bucket's queue ----
-        const unsigned long long idle = __ballot(S.phase == PH_IDLE);
+        // ---- refill idle groups from the bucket's queue (one stack per group) ----
+        const unsigned long long idle = __ballot(S.li == 0 && S.phase == PH_IDLE);
         if (idle != 0ull && !exhausted) {
             if (q_next >= q_end) {
+                const int ld = __ffsll((long long)idle) - 1;
                 unsigned long long base = 0;
-                if (lane == __ffsll((long long)idle) - 1)
-                    base = atomicAdd(a.queue, (unsigned long long)a.chunk);
-                base = __shfl(base, __ffsll((long long)idle) - 1);
+                if (lane == ld) base = atomicAdd(a.queue, (unsigned long long)a.chunk);
+                base = __shfl(base, ld);
                 if ((long)base >= m) {
                     exhausted = true;
                 } else {
@@ -158,38 +240,20 @@ __global__ void __launch_bounds__(256) ph_stack_kernel(PhStackArgs a)
                 }
             }
             if (!exhausted || q_next < q_end) {
-                const int rank = __popcll(idle & ((1ull << lane) - 1ull));
+                const int rank = __popcll(idle & ((1ull << gbase) - 1ull));
                 const long avail = q_end - q_next;
                 if (S.phase == PH_IDLE && rank < avail) {
                     p = (long)a.idx[q_next + rank];
-                    float r[ND][NB];
-                    double ss[ND], sv[ND];
-                    slots = 0;
-                    int s = 0;
-                    for (int d = 0; d < a.n_dates && s < ND; d++) {
-                        if (!a.valid[(long)d * a.n_pix + p]) continue;
-                        slots |= (unsigned int)d << (4 * s);
-                        s++;
-                    }
-#pragma unroll
-                    for (int t = 0; t < ND; t++) {
-                        const long d = (long)((slots >> (4 * t)) & 15u);
-#pragma unroll
-                        for (int b = 0; b < NB; b++)
-                            r[t][b] = a.rrs[(d * NB + b) * a.n_pix + p];
-                        ss[t] = (double)a.sec_sun[d * a.n_pix + p];
-                        sv[t] = 1.0;
-                    }
-                    if (ph_stack_pixel_init<ND>(S, c, r, ss, sv)) {
+                    if (ph_stack_load<ND>(S, a, p, gmask, &dst, &vmask)) {
                         S.kstart = SEED ? -1 : 0;
                         S.lowest = 1.0e4;
                         S.nstart = 0;
                         S.evals = 0;
                         S.best_icount = 0;
                         S.best_conv = 0;
-                        for (int i = 0; i < N; i++) S.set_best(i, 0.0);
+                        S.best = 0.0;
                         S.phase = PH_START;
-                    } else {
+                    } else if (S.li == 0) {
                         ph_stack_write_nodata(a, p, 2 + 3 * a.n_dates);
                     }
                 }
@@ -204,32 +268,23 @@ __global__ void __launch_bounds__(256) ph_stack_kernel(PhStackArgs a)
 
         // ---- set up a new start ----
         if (S.phase == PH_START) {
+            double sd = 0.0;
             if constexpr (SEED) {
-                double sd[N];
-                if (S.kstart < 0) {
-                    for (int i = 0; i < N; i++) {
-                        const int s = i < 2 ? 0 : (i - 2) / 3;
-                        const long d = (long)((slots >> (4 * s)) & 15u);
-                        const long src = i < 2 ? i : 2 + 3 * d + (i - 2) % 3;
-                        sd[i] = (double)a.seed[src * a.n_pix + p];
-                    }
-                }
-                ph_stack_start_step<ND>(S, c, S.kstart, sd);
-            } else {
-                ph_stack_start_step<ND>(S, c, S.kstart, nullptr);
+                if (S.kstart < 0 && S.li < N) sd = (double)a.seed[(long)dst * a.n_pix + p];
             }
-            ph_stack_nm_begin(S, c);
+            ph_grp_start_step<ND>(S, c, S.kstart, sd);
+            ph_grp_nm_begin<ND>(S, c);
         }
 
-        // ---- one evaluation per lane, in lock-step ----
+        // ---- one evaluation per group, in lock-step ----
         if (S.phase != PH_IDLE) {
-            const double f = ph_stack_objective<ND, false>(S, c, PhStackNM<N>::XE0, nullptr);
-            ph_stack_nm_consume(S, c, f);
+            PhGrpSide side;
+            const bool fin = S.phase == PH_FINAL; // the final re-evaluation at the best point
+            const double f = ph_grp_objective<ND>(S, c, S.xe, fin, &side);
+            ph_grp_consume<ND>(S, c, f);
             if (S.phase == PH_DONE) {
-                // the final re-evaluation's side quantities, at the best point
-                PhStackSide side;
-                ph_stack_objective<ND, true>(S, c, PhStackNM<N>::BEST0, &side);
-                ph_stack_write_outputs<ND>(a, p, slots, S, side);
+                const double B = __shfl(S.best, 1, G);
+                ph_stack_write_outputs<ND>(a, p, vmask, dst, S, B, side);
                 S.phase = PH_IDLE;
             }
         }
@@ -284,27 +339,19 @@ static int ph_stack_dispatch(int k, int seeded, Fn&& fn)
     }
 }
 
-extern "C" int ph_stack_slots(int k)
+template <int ND>
+static size_t ph_stack_lds_bytes(int block)
 {
-    switch (k) {
-    case 1: return PhStackNM<5>::n_slots;
-    case 2: return PhStackNM<8>::n_slots;
-    case 3: return PhStackNM<11>::n_slots;
-    case 4: return PhStackNM<14>::n_slots;
-    case 5: return PhStackNM<17>::n_slots;
-    case 6: return PhStackNM<20>::n_slots;
-    case 7: return PhStackNM<23>::n_slots;
-    case 8: return PhStackNM<26>::n_slots;
-    default: return -1;
-    }
+    return sizeof(double) * (size_t)(block / PhGrpLayout<ND>::G) * PhGrpLayout<ND>::n;
 }
 
 extern "C" int ph_stack_occupancy(int k, int seeded, int block, int* bpc)
 {
     return ph_stack_dispatch(k, seeded, [&](auto kd, auto sd) {
+        constexpr int K = decltype(kd)::value;
         return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            bpc, (const void*)ph_stack_kernel<decltype(kd)::value, decltype(sd)::value>, block,
-            0);
+            bpc, (const void*)ph_stack_kernel<K, decltype(sd)::value>, block,
+            ph_stack_lds_bytes<K>(block));
     });
 }
 
@@ -312,8 +359,9 @@ extern "C" int ph_launch_stack(const PhStackArgs* a, int k, int seeded, int grid
                                void* stream)
 {
     return ph_stack_dispatch(k, seeded, [&](auto kd, auto sd) {
-        hipLaunchKernelGGL((ph_stack_kernel<decltype(kd)::value, decltype(sd)::value>),
-                           dim3(grid), dim3(block), 0, (hipStream_t)stream, *a);
+        constexpr int K = decltype(kd)::value;
+        hipLaunchKernelGGL((ph_stack_kernel<K, decltype(sd)::value>), dim3(grid), dim3(block),
+                           ph_stack_lds_bytes<K>(block), (hipStream_t)stream, *a);
         return (int)hipGetLastError();
     });
 }

@@ -464,20 +464,11 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
     const int block = cfg->block > 0 ? cfg->block : (tu->block > 0 ? tu->block : 256);
     if (block < 64 || block > 256 || block % 64)
         return ph_fail(PHOTIC_EINVAL, "block %d must be a multiple of 64 in 64..256", block);
-    rc = ph_launch_status(ph_stack_occupancy(n_dates, seed != NULL, block, &bpc), "occupancy");
-    if (rc) return rc;
-    if (bpc < 1) bpc = 1;
-    if (cfg->grid <= 0 && cfg->block <= 0 && tu->bpc > 0 && tu->bpc < bpc) bpc = tu->bpc;
-    long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
-    const long lanes = grid * block;
-    a.work_lanes = lanes;
 
     hipStream_t st = (hipStream_t)stream;
     unsigned int *counts = NULL;
     int *lists = NULL;
     unsigned long long *queues = NULL;
-    double *work = NULL;
-    const size_t work_bytes = sizeof(double) * (size_t)ph_stack_slots(n_dates) * (size_t)lanes;
     rc = ph_hip(hipMallocAsync((void **)&counts, sizeof(unsigned int) * 16, st), "hipMallocAsync");
     if (!rc)
         rc = ph_hip(hipMallocAsync((void **)&lists, sizeof(int) * (size_t)(n_dates + 1) *
@@ -487,13 +478,12 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
         rc = ph_hip(hipMallocAsync((void **)&queues, sizeof(unsigned long long) * 2 *
                                                          (size_t)(n_dates + 1), st),
                     "hipMallocAsync");
-    if (!rc) rc = ph_hip(hipMallocAsync((void **)&work, work_bytes, st), "hipMallocAsync");
     if (!rc) rc = ph_launch_status(ph_launch_zero_u64((unsigned long long *)counts, 8, st), "zero");
     if (!rc)
         rc = ph_launch_status(ph_launch_zero_u64(queues, 2 * (n_dates + 1), st), "zero");
     if (!rc) rc = ph_launch_status(ph_launch_stack_bucket(&a, counts, lists, st), "bucketing");
-    a.work = work;
-    /* one persistent launch per date count, each pulling its bucket's pixels */
+    /* one persistent launch per date count, each pulling its bucket's stacks; the group size
+     * and so the LDS per block, hence the resident blocks, depend on the date count */
     for (int k = n_dates; k >= 1 && !rc; k--) {
         photic_cfg one = *cfg;
         rc = ph_make_const(&one, k, seed != NULL, &a.c);
@@ -502,6 +492,13 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
         a.idx = lists + (long)k * n_pix;
         a.count = counts + k;
         a.queue = queues + 2 * k;
+        rc = ph_launch_status(ph_stack_occupancy(k, seed != NULL, block, &bpc), "occupancy");
+        if (rc) break;
+        if (bpc < 1) {
+            rc = ph_fail(PHOTIC_EDEVICE, "stack kernel (%d dates) cannot be resident", k);
+            break;
+        }
+        const long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
         rc = ph_launch_status(ph_launch_stack(&a, k, seed != NULL, (int)grid, block, st),
                               "stack kernel");
     }
@@ -517,7 +514,6 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
                 rc = ph_fail(PHOTIC_EDEVICE, "stack kernel (%d dates) stopped by its %.0f s "
                                              "watchdog", k, limit_s);
     }
-    if (work) hipFreeAsync(work, st);
     if (queues) hipFreeAsync(queues, st);
     if (lists) hipFreeAsync(lists, st);
     if (counts) hipFreeAsync(counts, st);
@@ -603,8 +599,9 @@ static int ph_autotune(int stack, const photic_cfg *cfg, long n_pix, int n_dates
     float win_ms = 1e30f, ms = 0.0f;
     int tried = 0;
     *slot = win; /* candidates run with explicit shapes */
-    /* stacks: memory-latency-bound, so only the block size and chunk are swept at full
-     * occupancy (fewer lanes in flight only lose) */
+    /* stacks: every date count's launch runs at its own full occupancy (the group size, and
+     * with it the LDS per block, depends on the date count), so only the block size and the
+     * chunk are swept */
     const int n_blocks = 4, n_chunks = stack ? 2 : 5, n_halves = stack ? 1 : 2;
     for (int bi = 0; bi < n_blocks && !rc; bi++) {
         const int block = blocks[bi];
@@ -620,7 +617,7 @@ static int ph_autotune(int stack, const photic_cfg *cfg, long n_pix, int n_dates
             if (b < 1 || (half && b == bpc)) continue;
             for (int ci = 0; ci < n_chunks && !rc; ci++) {
                 c.block = block;
-                c.grid = n_cu * b;
+                c.grid = stack ? 0 : n_cu * b;
                 c.chunk = chunks[ci];
                 if (tried == 0) /* first launch of the instance: load the code object */
                     rc = ph_time_call(stack, &c, n_pix, n_dates, rrs, ss, valid, seed, &o, st, &ms);
@@ -630,7 +627,7 @@ static int ph_autotune(int stack, const photic_cfg *cfg, long n_pix, int n_dates
                 if (!rc && ms < win_ms) {
                     win_ms = ms;
                     win.block = block;
-                    win.bpc = b;
+                    win.bpc = stack ? 0 : b;
                     win.chunk = chunks[ci];
                 }
             }
