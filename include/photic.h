@@ -184,6 +184,14 @@ int photic_mlp_forward(photic_mlp *mlp, const float *x, long n, float *y, void *
  * (rrs [4][n], sec_sun [n] or NULL for cfg->sec_sun). */
 int photic_mlp_seed(photic_mlp *mlp, const photic_cfg *cfg, long n, const float *rrs,
                     const float *sec_sun, float *seed, void *stream);
+/* Stack seeds [2 + 3 D][n] (H, B, then 100 P, 100 G, 100 X at each date's slot) for a D-date
+ * scene (rrs [D][4][n], sec_sun [D][n], valid [D][n]): the network runs on every date; H and B
+ * are the means of its per-date values over the pixel's valid dates, P, G, X the date's own.
+ * The seed input of photic_invert_stack.  No reference counterpart (the learned warm start is
+ * this build's addition). */
+int photic_mlp_seed_stack(photic_mlp *mlp, const photic_cfg *cfg, long n, int n_dates,
+                          const float *rrs, const float *sec_sun, const unsigned char *valid,
+                          float *seed, void *stream);
 
 /* The band coefficients the kernels use for cfg's wavelengths, interpolated from the
  * reference's 41-node tables (model/samodel.c:103-286) as model/samodel.c:560-618 does:

@@ -117,6 +117,7 @@ def _load():
                                  C.POINTER(C.c_float), vp]),
         "photic_mlp_forward": (i, [vp, vp, l, vp, vp]),
         "photic_mlp_seed": (i, [vp, C.POINTER(Cfg), l, vp, vp, vp, vp]),
+        "photic_mlp_seed_stack": (i, [vp, C.POINTER(Cfg), l, i, vp, vp, vp, vp, vp]),
         "photic_invert_h": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, vp, C.POINTER(Outputs),
                                 vp]),
         "photic_band_coefficients": (i, [C.POINTER(Cfg), vp, C.POINTER(d)]),
@@ -465,6 +466,35 @@ class Mlp:
 
     def seed_device(self, cfg, n, d_rrs, d_sec_sun, d_seed, stream=None):
         check(lib.photic_mlp_seed(self.h, C.byref(cfg), n, d_rrs, d_sec_sun, d_seed, stream))
+
+    def seed_stack_device(self, cfg, n, n_dates, d_rrs, d_sec_sun, d_valid, d_seed,
+                          stream=None):
+        """Stack seeds [2 + 3 D][n] on the device (photic_mlp_seed_stack)."""
+        check(lib.photic_mlp_seed_stack(self.h, C.byref(cfg), n, n_dates, d_rrs, d_sec_sun,
+                                        d_valid, d_seed, stream))
+
+    def seed(self, cfg, rrs, sec_sun):
+        """Seeds [5][n] for host arrays rrs [4][n], sec_sun [n]."""
+        rrs = np.ascontiguousarray(rrs, np.float32)
+        n = rrs.shape[1]
+        br, bs = DeviceBuffer.from_array(rrs), DeviceBuffer.from_array(
+            np.ascontiguousarray(sec_sun, np.float32))
+        bo = DeviceBuffer(4 * 5 * n)
+        self.seed_device(cfg, n, br.value, bs.value, bo.value)
+        check(lib.photic_synchronize())
+        return bo.to_array(np.float32, (5, n))
+
+    def seed_stack(self, cfg, rrs, sec_sun, valid):
+        """Stack seeds [2 + 3 D][n] for host arrays rrs [D][4][n], sec_sun [D][n], valid [D][n]."""
+        rrs = np.ascontiguousarray(rrs, np.float32)
+        nd, n = rrs.shape[0], rrs.shape[2]
+        br = DeviceBuffer.from_array(rrs)
+        bs = DeviceBuffer.from_array(np.ascontiguousarray(sec_sun, np.float32))
+        bv = DeviceBuffer.from_array(np.ascontiguousarray(valid, np.uint8))
+        bo = DeviceBuffer(4 * (2 + 3 * nd) * n)
+        self.seed_stack_device(cfg, n, nd, br.value, bs.value, bv.value, bo.value)
+        check(lib.photic_synchronize())
+        return bo.to_array(np.float32, (2 + 3 * nd, n))
 
 
 def trainset_device(cfg, seed, n, pix0=0, sigma=2e-4):

@@ -436,22 +436,62 @@ __global__ void ph_mlp_prep_kernel(const float* __restrict__ rrs, const float* _
     ph_mlp_inputs(r, ss ? ss[i] : ss0, x + i * 5);
 }
 
+// One sample's network outputs (the training targets' normalisation undone) -> a seed in the
+// MIX layout: s = (H, B, 100 P, 100 G, 100 X).
+__device__ __forceinline__ void ph_mlp_decode(const float* o, float* s)
+{
+    float H = __expf(fminf(fmaxf(o[0], -4.0f), 4.0f) * 1.0f + 1.25f);
+    float P = __expf(fminf(fmaxf(o[1], -4.0f), 4.0f) * 1.85f - 3.45f);
+    float G = __expf(fminf(fmaxf(o[2], -4.0f), 4.0f) * 2.30f - 3.91f);
+    float X = __expf(fminf(fmaxf(o[3], -4.0f), 4.0f) * 1.96f - 4.95f);
+    s[0] = fminf(fmaxf(H, 0.1f), 60.0f);
+    s[1] = fminf(fmaxf(0.5f * (o[4] + 1.0f), 0.0f), 1.0f);
+    s[2] = 100.0f * P;
+    s[3] = 100.0f * G;
+    s[4] = 100.0f * X;
+}
+
 // Network outputs -> simplex seeds in the MIX layout [H, B, 100 P, 100 G, 100 X] ([5][n]).
 __global__ void ph_mlp_seed_kernel(const float* __restrict__ y, long n, float* __restrict__ seed)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float* o = y + i * 5;
-    float H = __expf(fminf(fmaxf(o[0], -4.0f), 4.0f) * 1.0f + 1.25f);
-    float P = __expf(fminf(fmaxf(o[1], -4.0f), 4.0f) * 1.85f - 3.45f);
-    float G = __expf(fminf(fmaxf(o[2], -4.0f), 4.0f) * 2.30f - 3.91f);
-    float X = __expf(fminf(fmaxf(o[3], -4.0f), 4.0f) * 1.96f - 4.95f);
-    float B = fminf(fmaxf(0.5f * (o[4] + 1.0f), 0.0f), 1.0f);
-    seed[0 * n + i] = fminf(fmaxf(H, 0.1f), 60.0f);
-    seed[1 * n + i] = B;
-    seed[2 * n + i] = 100.0f * P;
-    seed[3 * n + i] = 100.0f * G;
-    seed[4 * n + i] = 100.0f * X;
+    float s[5];
+    ph_mlp_decode(y + i * 5, s);
+    for (int k = 0; k < 5; k++) seed[k * n + i] = s[k];
+}
+
+// Date d's network outputs -> its part of a stack seed [2 + 3 D][n] (H, B, then P, G, X per
+// date): P, G, X at rows 2 + 3d .. 4 + 3d; H and B summed over the pixel's valid dates into
+// rows 0 and 1 (date 0 writes them, later dates add).
+__global__ void ph_mlp_stack_seed_kernel(const float* __restrict__ y, long n, int d,
+                                         const unsigned char* __restrict__ valid,
+                                         float* __restrict__ seed)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s[5];
+    ph_mlp_decode(y + i * 5, s);
+    for (int k = 0; k < 3; k++) seed[(long)(2 + 3 * d + k) * n + i] = s[2 + k];
+    const bool v = valid[(long)d * n + i] != 0;
+    const float h = v ? s[0] : 0.0f, b = v ? s[1] : 0.0f;
+    seed[i] = d == 0 ? h : seed[i] + h;
+    seed[n + i] = d == 0 ? b : seed[n + i] + b;
+}
+
+// Rows 0 and 1 of a stack seed: the sums over valid dates -> their means (zero without one).
+__global__ void ph_mlp_stack_mean_kernel(long n, int n_dates,
+                                         const unsigned char* __restrict__ valid,
+                                         float* __restrict__ seed)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = 0;
+    for (int d = 0; d < n_dates; d++) k += valid[(long)d * n + i] ? 1 : 0;
+    if (k > 0) {
+        seed[i] = seed[i] / (float)k;
+        seed[n + i] = seed[n + i] / (float)k;
+    }
 }
 
 // Deterministic initialisation (Philox): uniform +-sqrt(6 / (fan_in + fan_out)), biases 0.
@@ -552,5 +592,23 @@ extern "C" int ph_launch_mlp_seed(const float* y, long n, float* seed, void* str
     if (n <= 0) return 0;
     hipLaunchKernelGGL(ph_mlp_seed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, y, n, seed);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ph_launch_mlp_stack_seed(const float* y, long n, int d, const unsigned char* valid,
+                                        float* seed, void* stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(ph_mlp_stack_seed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, y, n, d, valid, seed);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ph_launch_mlp_stack_mean(long n, int n_dates, const unsigned char* valid,
+                                        float* seed, void* stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(ph_mlp_stack_mean_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, n_dates, valid, seed);
     return (int)hipGetLastError();
 }

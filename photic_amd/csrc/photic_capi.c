@@ -899,6 +899,38 @@ int photic_mlp_seed(photic_mlp *m, const photic_cfg *cfg, long n, const float *r
     return rc;
 }
 
+int photic_mlp_seed_stack(photic_mlp *m, const photic_cfg *cfg, long n, int n_dates,
+                          const float *rrs, const float *sec_sun, const unsigned char *valid,
+                          float *seed, void *stream)
+{
+    if (!m || !cfg || n < 0) return ph_fail(PHOTIC_EINVAL, "bad arguments");
+    if (cfg->n_bands != 4 || cfg->mode != PHOTIC_MODE_MIX)
+        return ph_fail(PHOTIC_EUNSUPPORTED, "seeds are made for 4 bands in MIX mode");
+    if (n_dates < 1 || n_dates > PH_MAXD)
+        return ph_fail(PHOTIC_EINVAL, "n_dates %d outside 1..%d", n_dates, PH_MAXD);
+    if (n == 0) return PHOTIC_OK;
+    if (!rrs || !sec_sun || !valid || !seed)
+        return ph_fail(PHOTIC_EINVAL, "rrs, sec_sun, valid and seed are required");
+    hipStream_t st = (hipStream_t)stream;
+    float *xb = NULL, *yb = NULL;
+    int rc = ph_hip(hipMallocAsync((void **)&xb, sizeof(float) * 5 * n, st), "hipMallocAsync");
+    if (rc) return rc;
+    rc = ph_hip(hipMallocAsync((void **)&yb, sizeof(float) * 5 * n, st), "hipMallocAsync");
+    for (int d = 0; d < n_dates && !rc; d++) {
+        rc = ph_launch_status(ph_launch_mlp_prep(rrs + (long)d * 4 * n, sec_sun + (long)d * n,
+                                                 (float)cfg->sec_sun, n, xb, st),
+                              "network inputs");
+        if (!rc) rc = photic_mlp_forward(m, xb, n, yb, st);
+        if (!rc)
+            rc = ph_launch_status(ph_launch_mlp_stack_seed(yb, n, d, valid, seed, st), "seeds");
+    }
+    if (!rc)
+        rc = ph_launch_status(ph_launch_mlp_stack_mean(n, n_dates, valid, seed, st), "seeds");
+    hipFreeAsync(xb, st);
+    if (yb) hipFreeAsync(yb, st);
+    return rc;
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* device plumbing                                                                        */
 /* ------------------------------------------------------------------------------------ */

@@ -88,3 +88,30 @@ def test_stack_empty_and_unsupported():
     with pytest.raises(P.PhoticError):
         P.invert_stack(_cfg(mode=P.MODE_NATIVE, n_bottoms=2), np.zeros((2, 4, 8), np.float32),
                        np.ones((2, 8), np.float32), np.ones((2, 8), np.uint8))
+
+
+def test_stack_mlp_seeded():
+    """MLP-seeded stacks (the seeds of photic_mlp_seed_stack): bit-identical to the host
+    engine given the same seeds, and what the trained network gains over plain stacks."""
+    wl = (443.0, 482.0, 561.0, 655.0)
+    cfg = P.cfg_default(wl, reqmin=1e-10, konvge=100, kcount=2000, check_watchdog=1)
+    nd, n = 4, 8192
+    rrs, ss, valid, truth = P.gen_stack(cfg, 5, n, n_dates=nd, k_min=2)
+    dx, dt = P.trainset_device(cfg, 101, 1 << 21)
+    net = P.Mlp(init_seed=1)
+    net.train_device(dx.value, dt.value, 1 << 21, batch=8192, epochs=2, lr=2e-3, lr_final=1e-4)
+    seed = net.seed_stack(cfg, rrs, ss, valid)
+    g = P.invert_stack(cfg, rrs, ss, valid, seed=seed)
+    p = P.invert_stack(cfg, rrs, ss, valid)
+    m = 384
+    h = O.hostsim_invert_stack(P.const_bytes(cfg, has_seed=True), rrs[:, :, :m], ss[:, :m],
+                               valid[:, :m], seed=seed[:, :m], math="cr")
+    assert np.array_equal(g["status"][:m], h["status"])
+    assert np.array_equal(g["x"][:, :m], h["x"])
+    assert np.array_equal(g["n_evals"][:m], h["n_evals"])
+    v = g["status"] >= 0
+    se, pe = g["n_evals"][v].mean(), p["n_evals"][v].mean()
+    sm, pm = np.median(g["rrs_error"][v]), np.median(p["rrs_error"][v])
+    print("stacks: seeded %.0f evals/stack, median E %.3g; plain %.0f, %.3g" % (se, sm, pe, pm))
+    assert se < 0.8 * pe
+    assert sm <= pm
