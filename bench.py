@@ -15,6 +15,14 @@ kernel (algorithmic FLOPs k(46 n_b + 15) + 6 = 205 per objective evaluation over
 duration from HIP events on its stream; HBM bytes per launch from the profiler counters in
 profiles/) and the CPU baseline (the FP64 oracle, OpenMP over pixels, timed on a bounded
 sample of the same scene on this host's cores).
+
+Two further workloads, each with its own JSON line (not the headline):
+  --workload stacks  2048 x 2048 pixels seen on 8 dates, 3-8 valid per pixel (Bernoulli(0.3)
+                     clouds): plain stack inversion (2 + 3k parameters), inputs resident in HBM,
+                     launch shape from the stack autotuner.
+  --workload stream  12662 x 16077 pixels on 4 dates, 2-4 valid: MLP-seeded stack inversion of
+                     a host-resident scene streamed through HBM in 2048-row tiles (pinned
+                     double-buffered copies on a side stream); wall time includes the copies.
 """
 import argparse
 import ctypes as C
@@ -86,6 +94,15 @@ def main():
     ap.add_argument("--equal-misfit-pixels", type=int, default=1 << 22)
     ap.add_argument("--train-samples", type=int, default=1 << 24)
     ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--workload", choices=("scene", "stacks", "stream"), default="scene")
+    ap.add_argument("--stack-side", type=int, default=2048)
+    ap.add_argument("--stack-cpu-sample", type=int, default=32768)
+    ap.add_argument("--stream-cpu-sample", type=int, default=65536)
+    ap.add_argument("--stream-rows", type=int, default=12662)
+    ap.add_argument("--stream-cols", type=int, default=16077)
+    ap.add_argument("--stream-tile-rows", type=int, default=2048)
+    ap.add_argument("--stream-plain-rows", type=int, default=0,
+                    help="also stream the plain method over the first ROWS rows")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has one MI355X and the "
@@ -94,6 +111,10 @@ def main():
         sys.exit(2)
 
     import photic_amd as P
+    if args.workload == "stacks":
+        return bench_stacks(args, P)
+    if args.workload == "stream":
+        return bench_stream(args, P)
     if args.copy_calibration:
         nbytes = args.copy_calibration
         src, dst = P.DeviceBuffer(nbytes), P.DeviceBuffer(nbytes)
@@ -328,6 +349,232 @@ def main():
     P.lib.photic_stream_destroy(stream)
     print(json.dumps(line))
 
+
+
+def _stack_flops(valid, n_evals):
+    """Algorithmic FLOPs of a stack inversion: evaluations x (k (46 n_b + 15) + 6) per stack,
+    k its valid dates, n_b = 4 bands."""
+    k = valid.astype(np.int64).sum(0)
+    return float((n_evals.astype(np.int64) * (k * (46 * 4 + 15) + 6)).sum())
+
+
+def _stack_cpu(rrs_h, ss_h, valid_h, n_sample, seed=None):
+    """The FP64 oracle's stack inversion (OpenMP over stacks) on the first n_sample stacks."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_bindings as O
+    threads = min(16, os.cpu_count() or 1)
+    used = O.set_threads(threads)
+    t = time.perf_counter()
+    O.invert_stack(np.ascontiguousarray(rrs_h[:, :, :n_sample]),
+                   np.ascontiguousarray(ss_h[:, :n_sample]),
+                   np.ascontiguousarray(valid_h[:, :n_sample]))
+    dt = time.perf_counter() - t
+    return n_sample / dt, used, dt
+
+
+def bench_stacks(args, P):
+    """Plain stack inversion of a side x side x 8-date scene resident in HBM."""
+    nd, side = 8, args.stack_side
+    n = side * side
+    cfg = P.cfg_default(WL, reqmin=1e-10, konvge=100, kcount=2000, watchdog_s=900.0)
+    stream = C.c_void_p()
+    P.check(P.lib.photic_stream_create(C.byref(stream)))
+    d_rrs, d_ss, d_v = P.DeviceBuffer(16 * nd * n), P.DeviceBuffer(4 * nd * n), P.DeviceBuffer(nd * n)
+    d_tr = P.DeviceBuffer(4 * (2 + 4 * nd) * n)
+    P.check(P.lib.photic_gen_stack(C.byref(cfg), 3, 0, n, nd, 2e-4, 0.3, 3, d_rrs.value,
+                                   d_ss.value, d_v.value, d_tr.value, stream))
+    P.check(P.lib.photic_synchronize())
+    truth_H = d_tr.to_array(np.float32, (2 + 4 * nd, n))[0]
+    del d_tr
+    valid_h = d_v.to_array(np.uint8, (nd, n))
+
+    # autotuned launch shape on the first n_t stacks (untimed)
+    n_t = min(n, 1 << 20)
+    sub = [P.DeviceBuffer(16 * nd * n_t), P.DeviceBuffer(4 * nd * n_t), P.DeviceBuffer(nd * n_t)]
+    for q in range(4 * nd):
+        P.check(P.lib.photic_device_copy(sub[0].value + 4 * q * n_t, d_rrs.value + 4 * q * n,
+                                         4 * n_t, stream))
+    for d in range(nd):
+        P.check(P.lib.photic_device_copy(sub[1].value + 4 * d * n_t, d_ss.value + 4 * d * n,
+                                         4 * n_t, stream))
+        P.check(P.lib.photic_device_copy(sub[2].value + d * n_t, d_v.value + d * n, n_t, stream))
+    P.check(P.lib.photic_synchronize())
+    tuning = P.autotune_stack(cfg, n_t, nd, sub[0].value, sub[1].value, sub[2].value, None, stream)
+    del sub
+
+    fields = ("depth", "rrs_error", "status", "n_evals", "n_starts")
+
+    def step():
+        outs = P.StackOutputs(cfg, n, nd, fields=fields)
+        P.check(P.lib.photic_synchronize())
+        e0, e1 = _events()
+        t0 = time.perf_counter()
+        P.check(P.lib.photic_event_record(e0, stream))
+        P.invert_stack_device(cfg, n, nd, d_rrs.value, d_ss.value, d_v.value, None, outs,
+                              stream=stream)
+        P.check(P.lib.photic_event_record(e1, stream))
+        P.check(P.lib.photic_synchronize())
+        wall = 1e3 * (time.perf_counter() - t0)
+        ms = C.c_float()
+        P.check(P.lib.photic_event_elapsed_ms(e0, e1, C.byref(ms)))
+        P.lib.photic_event_destroy(e0)
+        P.lib.photic_event_destroy(e1)
+        return wall, float(ms.value), outs.fetch()
+
+    for _ in range(args.warmup):
+        step()
+    walls, kms, res = [], [], None
+    for _ in range(args.steps):
+        w, k_ms, res = step()
+        walls.append(w)
+        kms.append(k_ms)
+    ms_step, k_ms = float(np.mean(walls)), float(np.mean(kms))
+    v = res["status"] >= 0
+    dH = res["depth"][v].astype(np.float64) - truth_H[v]
+    flops = _stack_flops(valid_h, res["n_evals"])
+    achieved = flops / (k_ms / 1e3) / 1e12
+    line = {
+        "metric": "pixels/sec inverted, plain multi-date stack Nelder-Mead, 1 MI355X",
+        "value": n / (ms_step / 1e3), "unit": "pixels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Philox-seeded stack scene, no real imagery)",
+        "config": {"workload": "plain stack inversion, %dx%d pixels x 8 dates, 3-8 valid "
+                               "(Bernoulli(0.3) clouds), 2+3k params, reqmin 1e-10 kcount 2000 "
+                               "konvge 100" % (side, side),
+                   "pixels": n, "seed": 3, "noise_sigma": 2e-4,
+                   "parallelism": "1 GPU, one persistent work-queue launch per valid-date "
+                                  "count, 8/16/32-lane groups per stack"},
+        "stacks": {"evals_per_stack": float(res["n_evals"][v].mean()),
+                   "starts_per_stack": float(res["n_starts"][v].mean()),
+                   "median_rrs_error_pct": float(np.median(res["rrs_error"][v])),
+                   "depth_rmse_m": float(np.sqrt(np.mean(dH * dH))),
+                   "depth_median_abs_error_m": float(np.median(np.abs(dH))),
+                   "valid_date_histogram": np.bincount(valid_h.sum(0), minlength=nd + 1).tolist()},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "ph_stack_kernel<k, false>, k = 1..8 (one launch each)",
+                     "kernel_ms": k_ms,
+                     "flop_per_eval": "k (46 x 4 + 15) + 6 per stack of k valid dates",
+                     "flop_per_launch": flops},
+        "cpu_baseline": None, "launch": tuning,
+    }
+    if not args.no_cpu:
+        ns = min(args.stack_cpu_sample, n)
+        cpx, threads, dt = _stack_cpu(d_rrs.to_array(np.float32, (nd, 4, n))[:, :, :ns],
+                                      d_ss.to_array(np.float32, (nd, n))[:, :ns],
+                                      valid_h[:, :ns], ns)
+        line["cpu_baseline"] = {"value": cpx, "unit": "pixels/s", "cores": threads,
+                                "kind": "port",
+                                "sample": "first %d stacks of the same scene, FP64 oracle, "
+                                          "OpenMP dynamic schedule, %.1f s" % (ns, dt)}
+        line["gpu_over_cpu"] = line["value"] / cpx
+    P.lib.photic_stream_destroy(stream)
+    print(json.dumps(line))
+
+
+def bench_stream(args, P):
+    """MLP-seeded stack inversion of a host-resident rows x cols x 4-date scene, streamed."""
+    nd, rows, cols = 4, args.stream_rows, args.stream_cols
+    n = rows * cols
+    cfg = P.cfg_default(WL, reqmin=1e-10, konvge=100, kcount=2000, watchdog_s=900.0)
+    # the scene in host memory, generated on the GPU a tile at a time (untimed)
+    t0 = time.perf_counter()
+    rrs_h = np.empty((nd, 4, n), np.float32)
+    ss_h = np.empty((nd, n), np.float32)
+    v_h = np.empty((nd, n), np.uint8)
+    H_h = np.empty(n, np.float32)
+    tile = args.stream_tile_rows * cols
+    g = [P.DeviceBuffer(16 * nd * tile), P.DeviceBuffer(4 * nd * tile), P.DeviceBuffer(nd * tile),
+         P.DeviceBuffer(4 * (2 + 4 * nd) * tile)]
+    for p0 in range(0, n, tile):
+        nt = min(tile, n - p0)
+        P.check(P.lib.photic_gen_stack(C.byref(cfg), 5, p0, nt, nd, 2e-4, 0.3, 2, g[0].value,
+                                       g[1].value, g[2].value, g[3].value, None))
+        for q in range(4 * nd):
+            P.check(P.lib.photic_memcpy_dtoh(rrs_h[q // 4, q % 4, p0:].ctypes.data,
+                                             g[0].value + 4 * q * nt, 4 * nt))
+        for d in range(nd):
+            P.check(P.lib.photic_memcpy_dtoh(ss_h[d, p0:].ctypes.data, g[1].value + 4 * d * nt,
+                                             4 * nt))
+            P.check(P.lib.photic_memcpy_dtoh(v_h[d, p0:].ctypes.data, g[2].value + d * nt, nt))
+        P.check(P.lib.photic_memcpy_dtoh(H_h[p0:].ctypes.data, g[3].value, 4 * nt))
+    del g
+    print("scene %d x %d x %d dates generated in %.1f s" % (rows, cols, nd, time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    dx, dt = P.trainset_device(cfg, 101, args.train_samples)
+    net = P.Mlp(init_seed=1)
+    loss = net.train_device(dx.value, dt.value, args.train_samples, batch=8192,
+                            epochs=args.train_epochs, lr=2e-3, lr_final=1e-4)
+    P.check(P.lib.photic_synchronize())
+    del dx, dt
+    train_info = {"samples": args.train_samples, "epochs": args.train_epochs,
+                  "final_loss": loss, "seconds": time.perf_counter() - t0}
+
+    fields = ("depth", "rrs_error", "status", "n_evals", "n_starts")
+
+    def report(d, tot):
+        print("tile %d/%d done at %.1f s" % (d, tot, time.perf_counter() - t_run),
+              file=sys.stderr, flush=True)
+
+    def summary(res, rows_run, wall):
+        m = rows_run * cols
+        v = res["status"] >= 0
+        dH = res["depth"][v].astype(np.float64) - H_h[:m][v]
+        fit = res["rrs_error"][v] < 1.0
+        return {"pixels_per_s": m / wall if wall else None, "seconds": wall,
+                "evals_per_stack": float(res["n_evals"][v].mean()),
+                "fallback_rate": float((res["n_starts"][v] > 1).mean()),
+                "median_rrs_error_pct": float(np.median(res["rrs_error"][v])),
+                "depth_rmse_m": float(np.sqrt(np.mean(dH * dH))),
+                "depth_median_abs_error_m": float(np.median(np.abs(dH))),
+                "depth_rmse_fitted_m": float(np.sqrt(np.mean(dH[fit] ** 2))),
+                "fitted_frac": float(fit.mean()), "valid_pixels": int(v.sum())}
+
+    t_run = time.perf_counter()
+    res = P.invert_stack_stream(cfg, rows, cols, rrs_h, ss_h, v_h,
+                                tile_rows=args.stream_tile_rows, net=net, fields=fields,
+                                progress=report)
+    wall = time.perf_counter() - t_run
+    seeded = summary(res, rows, wall)
+    line = {
+        "metric": "pixels/sec inverted, MLP-seeded multi-date stacks streamed through HBM, "
+                  "1 MI355X",
+        "value": n / wall, "unit": "pixels/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * wall, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Philox-seeded stack scene in host memory, no real imagery)",
+        "config": {"workload": "MLP-seeded stack inversion with plain fallback, %dx%d pixels "
+                               "x 4 dates, 2-4 valid, streamed in %d-row tiles (pinned "
+                               "double-buffered copies on a side stream; wall time includes "
+                               "them)" % (rows, cols, args.stream_tile_rows),
+                   "pixels": n, "seed": 5, "noise_sigma": 2e-4,
+                   "parallelism": "1 GPU, copy and compute streams"},
+        "seeded": seeded, "mlp_training": train_info,
+    }
+    if args.stream_plain_rows > 0:
+        pr = min(args.stream_plain_rows, rows)
+        m = pr * cols
+        t_run = time.perf_counter()
+        pres = P.invert_stack_stream(cfg, pr, cols, np.ascontiguousarray(rrs_h[:, :, :m]),
+                                     np.ascontiguousarray(ss_h[:, :m]),
+                                     np.ascontiguousarray(v_h[:, :m]),
+                                     tile_rows=args.stream_tile_rows, net=None, fields=fields,
+                                     progress=report)
+        line["plain_first_rows"] = dict(summary(pres, pr, time.perf_counter() - t_run), rows=pr)
+        sres = {k: a[..., :m] for k, a in res.items()}
+        line["seeded_first_rows"] = dict(summary(sres, pr, None), rows=pr)
+    if not args.no_cpu:
+        ns = min(args.stream_cpu_sample, n)
+        cpx, threads, dt = _stack_cpu(rrs_h[:, :, :ns], ss_h[:, :ns], v_h[:, :ns], ns)
+        line["cpu_baseline"] = {"value": cpx, "unit": "pixels/s", "cores": threads,
+                                "kind": "port",
+                                "sample": "first %d stacks of the same scene, FP64 oracle "
+                                          "(plain), OpenMP dynamic schedule, %.1f s" % (ns, dt)}
+        line["gpu_over_cpu"] = line["value"] / cpx
+    net.close()
+    print(json.dumps(line))
 
 if __name__ == "__main__":
     main()

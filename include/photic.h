@@ -193,6 +193,25 @@ int photic_mlp_seed_stack(photic_mlp *mlp, const photic_cfg *cfg, long n, int n_
                           const float *rrs, const float *sec_sun, const unsigned char *valid,
                           float *seed, void *stream);
 
+/* Streamed stack inversion of a host-resident scene of n_rows x n_cols pixels (pixel index
+ * row * n_cols + col): rrs [n_dates][4][n], sec_sun [n_dates][n], valid [n_dates][n] in host
+ * memory (pageable or pinned).  The scene goes through HBM in tiles of tile_rows rows: a
+ * tile's inputs are staged into one of two pinned buffers and copied to the device on a copy
+ * stream while the previous tile is inverted on a compute stream (MLP stack seeds from mlp
+ * when it is not NULL, then photic_invert_stack), and its outputs come back the same way
+ * into out's HOST arrays (photic_outputs layout over the whole scene, each may be NULL).
+ * progress, when not NULL, is called from the calling thread after each tile's outputs are
+ * in place.  Returns when the whole scene is done.  With cfg->check_watchdog each tile's
+ * inversion is waited for, so staging overlaps only the copies.  The tiling is this build's
+ * (the reference inverts a scene held in memory, model/samodel.c:289-1763); each pixel's
+ * result is the one photic_invert_stack gives for it. */
+typedef void (*photic_progress_fn)(long tiles_done, long n_tiles, void *user);
+int photic_invert_stack_stream(const photic_cfg *cfg, photic_mlp *mlp, long n_rows, long n_cols,
+                               int n_dates, const float *rrs, const float *sec_sun,
+                               const unsigned char *valid, long tile_rows,
+                               const photic_outputs *out, photic_progress_fn progress,
+                               void *user);
+
 /* The band coefficients the kernels use for cfg's wavelengths, interpolated from the
  * reference's 41-node tables (model/samodel.c:103-286) as model/samodel.c:560-618 does:
  * out [12][n_bands] = a0, a1, a_w, b_bw, then the 8 bottom reflectances in the order of

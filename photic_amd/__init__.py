@@ -71,6 +71,10 @@ class Outputs(C.Structure):
     ]
 
 
+# progress callback of photic_invert_stack_stream: (tiles done, tiles, user)
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_long, C.c_long, C.c_void_p)
+
+
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("blocks_per_cu", C.c_int), ("chunk", C.c_int),
                 ("ms", C.c_float), ("n_tried", C.c_int)]
@@ -118,6 +122,8 @@ def _load():
         "photic_mlp_forward": (i, [vp, vp, l, vp, vp]),
         "photic_mlp_seed": (i, [vp, C.POINTER(Cfg), l, vp, vp, vp, vp]),
         "photic_mlp_seed_stack": (i, [vp, C.POINTER(Cfg), l, i, vp, vp, vp, vp, vp]),
+        "photic_invert_stack_stream": (i, [C.POINTER(Cfg), vp, l, l, i, vp, vp, vp, l,
+                                           C.POINTER(Outputs), PROGRESS_FN, vp]),
         "photic_invert_h": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, vp, C.POINTER(Outputs),
                                 vp]),
         "photic_band_coefficients": (i, [C.POINTER(Cfg), vp, C.POINTER(d)]),
@@ -331,6 +337,37 @@ def invert_stack(cfg, rrs, sec_sun, valid, seed=None, fields=None):
                         d_seed.value if d_seed is not None else None, outs)
     check(lib.photic_synchronize())
     return outs.fetch()
+
+
+def invert_stack_stream(cfg, n_rows, n_cols, rrs, sec_sun, valid, tile_rows=2048, net=None,
+                        fields=None, progress=None):
+    """photic_invert_stack_stream on HOST arrays of an n_rows x n_cols scene: rrs [D][4][n]
+    float32, sec_sun [D][n], valid [D][n] uint8 (n = n_rows * n_cols), streamed through the
+    GPU in tiles of tile_rows rows, MLP-seeded when net is given.  Returns host outputs as
+    invert_stack; progress(tiles_done, n_tiles) is called as tiles finish."""
+    nd = rrs.shape[0]
+    n = n_rows * n_cols
+    assert rrs.shape == (nd, 4, n) and rrs.dtype == np.float32 and rrs.flags.c_contiguous
+    assert sec_sun.shape == (nd, n) and sec_sun.dtype == np.float32 and sec_sun.flags.c_contiguous
+    assert valid.shape == (nd, n) and valid.dtype == np.uint8 and valid.flags.c_contiguous
+    struct = Outputs()
+    res = {}
+    for name, (dt, rows) in _OUT_FIELDS.items():
+        if fields is not None and name not in fields:
+            continue
+        rows = 2 + 3 * nd if rows is None else rows
+        a = np.empty((rows, n) if rows > 1 else (n,), dt)
+        res[name] = a
+        setattr(struct, name, a.ctypes.data)
+    def _cb(done, tot, user):
+        if progress is not None:
+            progress(done, tot)
+    cb = PROGRESS_FN(_cb)
+    check(lib.photic_invert_stack_stream(C.byref(cfg), net.h if net is not None else None,
+                                         n_rows, n_cols, nd, rrs.ctypes.data,
+                                         sec_sun.ctypes.data, valid.ctypes.data, tile_rows,
+                                         C.byref(struct), cb, None))
+    return res
 
 
 def gen_stack(cfg, seed, n, n_dates=8, pix0=0, sigma=2e-4, p_cloud=0.3, k_min=3, truth=True):

@@ -932,6 +932,164 @@ int photic_mlp_seed_stack(photic_mlp *m, const photic_cfg *cfg, long n, int n_da
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* streamed stack inversion of a host-resident scene                                      */
+/* ------------------------------------------------------------------------------------ */
+
+/* One output field of the stream: host array, element size, rows of n. */
+typedef struct {
+    void *host;
+    size_t esz;
+    int rows;
+    size_t off; /* byte offset of the field in a tile's output block (rows x tile pixels) */
+} ph_sfield;
+
+typedef struct {
+    unsigned char *pin_in, *dev_in, *pin_out, *dev_out;
+    float *dev_seed;
+    hipEvent_t ev_in, ev_comp, ev_done;
+    long p0, nt; /* the tile in flight: first pixel and pixel count (nt 0: none) */
+} ph_sslot;
+
+int photic_invert_stack_stream(const photic_cfg *cfg, photic_mlp *mlp, long n_rows, long n_cols,
+                               int n_dates, const float *rrs, const float *sec_sun,
+                               const unsigned char *valid, long tile_rows,
+                               const photic_outputs *out, photic_progress_fn progress,
+                               void *user)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    if (!cfg || !out) return ph_fail(PHOTIC_EINVAL, "cfg and out are required");
+    if (n_rows < 0 || n_cols < 0 || tile_rows < 1)
+        return ph_fail(PHOTIC_EINVAL, "n_rows, n_cols >= 0 and tile_rows >= 1");
+    if (n_dates < 1 || n_dates > PH_MAXD)
+        return ph_fail(PHOTIC_EINVAL, "n_dates %d outside 1..%d", n_dates, PH_MAXD);
+    const long n = n_rows * n_cols;
+    if (n == 0) return PHOTIC_OK;
+    if (!rrs || !sec_sun || !valid)
+        return ph_fail(PHOTIC_EINVAL, "rrs, sec_sun and valid are required");
+    const long D = n_dates;
+    const long tile_px = (tile_rows < n_rows ? tile_rows : n_rows) * n_cols;
+    const long n_tiles = (n_rows + tile_rows - 1) / tile_rows;
+
+    ph_sfield f[13] = {
+        {out->depth, 4, 1, 0},       {out->rrs_error, 4, 1, 0}, {out->model_error, 4, 1, 0},
+        {out->albedo, 4, 1, 0},      {out->k_min, 4, 1, 0},     {out->iod, 4, 1, 0},
+        {out->b_pct, 4, 3, 0},       {out->bottom_type, 4, 1, 0},
+        {out->x, 8, (int)(2 + 3 * D), 0},
+        {out->status, 4, 1, 0},      {out->n_evals, 4, 1, 0},   {out->n_iter, 4, 1, 0},
+        {out->n_starts, 4, 1, 0}};
+    size_t out_per_px = 0;
+    for (int k = 0; k < 13; k++) {
+        f[k].off = out_per_px; /* times the tile's pixel count */
+        if (f[k].host) out_per_px += f[k].esz * (size_t)f[k].rows;
+    }
+    const size_t in_per_px = (size_t)D * (16 + 4 + 1);
+
+    hipStream_t cs = NULL, xs = NULL; /* compute, copy */
+    ph_sslot sl[2];
+    memset(sl, 0, sizeof(sl));
+    rc = ph_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+    if (!rc) rc = ph_hip(hipStreamCreateWithFlags(&xs, hipStreamNonBlocking), "hipStreamCreate");
+    for (int s = 0; s < 2 && !rc; s++) {
+        rc = ph_hip(hipHostMalloc((void **)&sl[s].pin_in, in_per_px * tile_px, 0), "hipHostMalloc");
+        if (!rc) rc = ph_hip(hipMalloc((void **)&sl[s].dev_in, in_per_px * tile_px), "hipMalloc");
+        if (!rc && out_per_px)
+            rc = ph_hip(hipHostMalloc((void **)&sl[s].pin_out, out_per_px * tile_px, 0),
+                        "hipHostMalloc");
+        if (!rc && out_per_px)
+            rc = ph_hip(hipMalloc((void **)&sl[s].dev_out, out_per_px * tile_px), "hipMalloc");
+        if (!rc && mlp)
+            rc = ph_hip(hipMalloc((void **)&sl[s].dev_seed, sizeof(float) * (2 + 3 * D) * tile_px),
+                        "hipMalloc");
+        if (!rc) rc = ph_hip(hipEventCreateWithFlags(&sl[s].ev_in, hipEventDisableTiming), "event");
+        if (!rc) rc = ph_hip(hipEventCreateWithFlags(&sl[s].ev_comp, hipEventDisableTiming), "event");
+        if (!rc) rc = ph_hip(hipEventCreateWithFlags(&sl[s].ev_done, hipEventDisableTiming), "event");
+    }
+    /* cfg->check_watchdog makes every tile's inversion wait for its kernels (to read their
+     * watchdog flags), so the next tile's staging then no longer overlaps the compute */
+    const photic_cfg c1 = *cfg;
+
+    /* Tile t is staged, copied in, inverted and copied out in slot t % 2; slot t % 2's
+     * previous tile (t - 2) is retired first: its outputs leave the pinned buffer, which
+     * the CPU refills with tile t while the GPU works on tile t - 1. */
+    long done = 0;
+    for (long t = 0; t < n_tiles + 2 && !rc; t++) {
+        ph_sslot *s = &sl[t % 2];
+        if (s->nt > 0) {
+            rc = ph_hip(hipEventSynchronize(s->ev_done), "tile");
+            if (rc) break;
+            for (int k = 0; k < 13; k++) {
+                if (!f[k].host) continue;
+                for (int r = 0; r < f[k].rows; r++)
+                    memcpy((char *)f[k].host + ((size_t)r * n + s->p0) * f[k].esz,
+                           s->pin_out + f[k].off * s->nt + (size_t)r * s->nt * f[k].esz,
+                           (size_t)s->nt * f[k].esz);
+            }
+            s->nt = 0;
+            done++;
+            if (progress) progress(done, n_tiles, user);
+        }
+        if (t >= n_tiles) continue;
+        const long p0 = t * tile_px;
+        const long nt = (n - p0 < tile_px) ? n - p0 : tile_px;
+        /* staging: rrs [D][4][nt] | sec_sun [D][nt] | valid [D][nt] */
+        float *pr = (float *)s->pin_in, *ps = pr + 4 * D * nt;
+        unsigned char *pv = (unsigned char *)(ps + D * nt);
+        for (long q = 0; q < 4 * D; q++)
+            memcpy(pr + q * nt, rrs + q * n + p0, sizeof(float) * (size_t)nt);
+        for (long d = 0; d < D; d++) {
+            memcpy(ps + d * nt, sec_sun + d * n + p0, sizeof(float) * (size_t)nt);
+            memcpy(pv + d * nt, valid + d * n + p0, (size_t)nt);
+        }
+        rc = ph_hip(hipMemcpyAsync(s->dev_in, s->pin_in, in_per_px * (size_t)nt,
+                                   hipMemcpyHostToDevice, xs), "tile in");
+        if (!rc) rc = ph_hip(hipEventRecord(s->ev_in, xs), "hipEventRecord");
+        if (!rc) rc = ph_hip(hipStreamWaitEvent(cs, s->ev_in, 0), "hipStreamWaitEvent");
+        const float *dr = (const float *)s->dev_in, *dss = dr + 4 * D * nt;
+        const unsigned char *dv = (const unsigned char *)(dss + D * nt);
+        if (!rc && mlp)
+            rc = photic_mlp_seed_stack(mlp, cfg, nt, n_dates, dr, dss, dv, s->dev_seed, cs);
+        photic_outputs o;
+        memset(&o, 0, sizeof(o));
+        void **dst[13] = {(void **)&o.depth, (void **)&o.rrs_error, (void **)&o.model_error,
+                          (void **)&o.albedo, (void **)&o.k_min, (void **)&o.iod,
+                          (void **)&o.b_pct, (void **)&o.bottom_type, (void **)&o.x,
+                          (void **)&o.status, (void **)&o.n_evals, (void **)&o.n_iter,
+                          (void **)&o.n_starts};
+        for (int k = 0; k < 13; k++)
+            if (f[k].host) *dst[k] = s->dev_out + f[k].off * nt;
+        if (!rc)
+            rc = photic_invert_stack(&c1, nt, n_dates, dr, dss, dv, mlp ? s->dev_seed : NULL,
+                                     &o, cs);
+        if (!rc) rc = ph_hip(hipEventRecord(s->ev_comp, cs), "hipEventRecord");
+        if (!rc) rc = ph_hip(hipStreamWaitEvent(xs, s->ev_comp, 0), "hipStreamWaitEvent");
+        if (!rc && out_per_px)
+            rc = ph_hip(hipMemcpyAsync(s->pin_out, s->dev_out, out_per_px * (size_t)nt,
+                                       hipMemcpyDeviceToHost, xs), "tile out");
+        if (!rc) rc = ph_hip(hipEventRecord(s->ev_done, xs), "hipEventRecord");
+        if (!rc) {
+            s->p0 = p0;
+            s->nt = nt;
+        }
+    }
+    if (cs) hipStreamSynchronize(cs);
+    if (xs) hipStreamSynchronize(xs);
+    for (int s = 0; s < 2; s++) {
+        if (sl[s].pin_in) hipHostFree(sl[s].pin_in);
+        if (sl[s].pin_out) hipHostFree(sl[s].pin_out);
+        if (sl[s].dev_in) hipFree(sl[s].dev_in);
+        if (sl[s].dev_out) hipFree(sl[s].dev_out);
+        if (sl[s].dev_seed) hipFree(sl[s].dev_seed);
+        if (sl[s].ev_in) hipEventDestroy(sl[s].ev_in);
+        if (sl[s].ev_comp) hipEventDestroy(sl[s].ev_comp);
+        if (sl[s].ev_done) hipEventDestroy(sl[s].ev_done);
+    }
+    if (cs) hipStreamDestroy(cs);
+    if (xs) hipStreamDestroy(xs);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* device plumbing                                                                        */
 /* ------------------------------------------------------------------------------------ */
 

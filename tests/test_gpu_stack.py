@@ -115,3 +115,37 @@ def test_stack_mlp_seeded():
     print("stacks: seeded %.0f evals/stack, median E %.3g; plain %.0f, %.3g" % (se, sm, pe, pm))
     assert se < 0.8 * pe
     assert sm <= pm
+
+
+@pytest.mark.parametrize("seeded", [False, True])
+def test_stack_stream_matches_whole_scene(seeded):
+    """photic_invert_stack_stream (host scene, 2-slot pinned staging, copy and compute
+    streams, a ragged last tile) gives every pixel exactly what photic_invert_stack gives on
+    the whole scene at once, and reports each tile as it lands."""
+    wl = (443.0, 482.0, 561.0, 655.0)
+    cfg = P.cfg_default(wl, reqmin=1e-10, konvge=100, kcount=2000)
+    nd, n_rows, n_cols = 4, 37, 101
+    n = n_rows * n_cols
+    rrs, ss, valid, truth = P.gen_stack(cfg, 5, n, n_dates=nd, k_min=2)
+    valid[:, 5:9] = 0
+    net = None
+    seed = None
+    if seeded:
+        dx, dt = P.trainset_device(cfg, 101, 1 << 18)
+        net = P.Mlp(init_seed=1)
+        net.train_device(dx.value, dt.value, 1 << 18, batch=8192, epochs=1, lr=2e-3,
+                         lr_final=1e-4)
+        seed = net.seed_stack(cfg, rrs, ss, valid)
+    whole = P.invert_stack(cfg, rrs, ss, valid, seed=seed)
+    seen = []
+    st = P.invert_stack_stream(cfg, n_rows, n_cols, np.ascontiguousarray(rrs),
+                               np.ascontiguousarray(ss), np.ascontiguousarray(valid),
+                               tile_rows=8, net=net, progress=lambda d, t: seen.append((d, t)))
+    assert seen == [(k, 5) for k in range(1, 6)]
+    for k in whole:
+        assert np.array_equal(st[k], whole[k]), k
+    one = P.invert_stack_stream(cfg, n_rows, n_cols, np.ascontiguousarray(rrs),
+                                np.ascontiguousarray(ss), np.ascontiguousarray(valid),
+                                tile_rows=1000, net=net, fields=("depth", "status"))
+    assert np.array_equal(one["depth"], whole["depth"])
+    assert np.array_equal(one["status"], whole["status"])
