@@ -143,6 +143,14 @@ int photic_gen_stack(const photic_cfg *cfg, uint64_t seed, long pix0, long n, in
 int photic_gen_scene(const photic_cfg *cfg, uint64_t seed, long pix0, long n, double sigma,
                      double deep_frac, float *rrs, float *sec_sun, float *truth, void *stream);
 
+/* No-data blobs for a synthetic scene of n_rows x n_cols pixels (pixel index row * n_cols +
+ * col): n_discs discs keyed by seed (Philox, counter = disc index), disc i centred at
+ * (u0 n_cols, u1 n_rows) with radius r_min + (r_max - r_min) u2; every band of a pixel whose
+ * centre lies inside a disc is set to nodata in rrs [n_bands][n] (device).  The masked
+ * fraction is about 1 - exp(-n_discs pi E[r^2] / (n_rows n_cols)). */
+int photic_gen_nodata_discs(uint64_t seed, long n_rows, long n_cols, int n_discs, double r_min,
+                            double r_max, int n_bands, float nodata, float *rrs, void *stream);
+
 /* Test entry points: the forward model alone (params [n][5] = H, P, G, X, B; sec_sun [n];
  * Y [n]; out [n][n_bands]) and the misfit alone (x [n][5] MIX layout; rrs [n][n_bands];
  * sec_sun [n]; out, rrs_error [n]). */

@@ -41,6 +41,7 @@ def lib():
         _lib.orc_py_forward.argtypes = [v, i, l, v, v, v, v]
         _lib.orc_py_objective.argtypes = [v, i, l, v, v, v, v, v]
         _lib.orc_py_gen.argtypes = [C.c_uint64, l, l, v, i, d, d, v, v, v, v, v]
+        _lib.orc_py_disc_mask.argtypes = [C.c_uint64, l, l, i, d, d, v, l, v]
         _lib.orc_py_band_tables.argtypes = [v, i, v, v]
         _lib.orc_py_start_step.argtypes = [i, i, v, i, v, d, v, v]
         _lib.orc_py_objective_stack.argtypes = [v, i, i, v, v, v, v]
@@ -161,6 +162,15 @@ def gen(seed, n, pix0=0, sigma=2e-4, deep_frac=0.0, wl=WL4):
     lib().orc_py_gen(seed, pix0, n, w.ctypes.data, nb, sigma, deep_frac, rrs.ctypes.data,
                      ss.ctypes.data, truth.ctypes.data, nodata.ctypes.data, Y.ctypes.data)
     return rrs, ss, truth, nodata, Y
+
+
+def disc_mask(seed, n_rows, n_cols, n_discs, r_min, r_max, idx):
+    """The no-data disc mask (uint8) at pixel indices idx of an n_rows x n_cols scene."""
+    idx = np.ascontiguousarray(idx, np.int64)
+    mask = np.zeros(idx.shape[0], np.uint8)
+    lib().orc_py_disc_mask(seed, n_rows, n_cols, n_discs, r_min, r_max, idx.ctypes.data,
+                           idx.shape[0], mask.ctypes.data)
+    return mask
 
 
 def band_tables(wl=WL4):

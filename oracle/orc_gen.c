@@ -195,3 +195,24 @@ int orc_gen_stack_pixel(uint64_t seed, uint64_t pix, const orc_bands *bands,
     }
     return k;
 }
+
+/* No-data discs (the build's scene-scale masks): disc i's centre (u0 n_cols, u1 n_rows) from
+ * block 250 of counter i, radius r_min + (r_max - r_min) u from block 251; 1 when the pixel's
+ * centre lies strictly inside a disc.  Not a reference function: the masks are this build's
+ * synthetic input, drawn identically on both sides. */
+int orc_in_nodata_disc(uint64_t seed, long row, long col, long n_rows, long n_cols, int n_discs,
+                       double r_min, double r_max)
+{
+    const double y = (double)row + 0.5, x = (double)col + 0.5;
+    for (int i = 0; i < n_discs; i++) {
+        double u0, u1, u2, u3;
+        orc_block_uniforms(seed, (uint64_t)i, 250u, 0u, &u0, &u1);
+        orc_block_uniforms(seed, (uint64_t)i, 251u, 0u, &u2, &u3);
+        const double cx = u0 * (double)n_cols, cy = u1 * (double)n_rows;
+        const double r = r_min + (r_max - r_min) * u2;
+        const double dx = x - cx, dy = y - cy;
+        if (dx * dx + dy * dy < r * r)
+            return 1;
+    }
+    return 0;
+}

@@ -426,3 +426,13 @@ double orc_py_objective_stack(const double *wl, int nb, int k, const double *x,
         *rrs_error = px.Rrs_error;
     return f;
 }
+
+/* The no-data disc mask at the pixel indices idx[m] (row * n_cols + col). */
+void orc_py_disc_mask(uint64_t seed, long n_rows, long n_cols, int n_discs, double r_min,
+                      double r_max, const long *idx, long m, uint8_t *mask)
+{
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < m; k++)
+        mask[k] = (uint8_t)orc_in_nodata_disc(seed, idx[k] / n_cols, idx[k] % n_cols, n_rows,
+                                              n_cols, n_discs, r_min, r_max);
+}

@@ -193,6 +193,10 @@ int orc_gen_pixel(uint64_t seed, uint64_t pix, const orc_bands *bands, double no
                   double deep_frac, orc_truth *truth, float *rrs_out, double *Y_out);
 void orc_fwd_truth(const orc_truth *t, const orc_bands *bands, int n_fixed_point,
                    double *rrs_out, double *Y_out);
+/* 1 when pixel (row, col) of an n_rows x n_cols scene lies inside one of its n_discs no-data
+ * discs (Philox keyed by seed, counter = disc index). */
+int orc_in_nodata_disc(uint64_t seed, long row, long col, long n_rows, long n_cols, int n_discs,
+                       double r_min, double r_max);
 /* One pixel of a multi-date stack scene: shared H and B, per-date P, G, X and sun zenith,
  * a Bernoulli(p_cloud) cloud mask with at least k_min clear dates.  truth [2 + 4 n_dates]:
  * H, B, then (P, G, X, theta_sun) per date; returns the number of valid dates. */

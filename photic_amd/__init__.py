@@ -91,6 +91,7 @@ def _load():
         "photic_n_params": (i, [C.POINTER(Cfg)]),
         "photic_invert": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, C.POINTER(Outputs), vp]),
         "photic_gen_scene": (i, [C.POINTER(Cfg), C.c_uint64, l, l, d, d, vp, vp, vp, vp]),
+        "photic_gen_nodata_discs": (i, [C.c_uint64, l, l, i, d, d, i, C.c_float, vp, vp]),
         "photic_forward": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, i, vp]),
         "photic_objective": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, vp, i, vp]),
         "photic_kernel_info": (i, [C.POINTER(Cfg), C.POINTER(i), C.POINTER(i), C.POINTER(i),
@@ -385,6 +386,19 @@ def gen_stack(cfg, seed, n, n_dates=8, pix0=0, sigma=2e-4, p_cloud=0.3, k_min=3,
     return (d_rrs.to_array(np.float32, (n_dates, nb, n)),
             d_ss.to_array(np.float32, (n_dates, n)), d_v.to_array(np.uint8, (n_dates, n)),
             d_tr.to_array(np.float32, (2 + 4 * n_dates, n)) if truth else None)
+
+
+def disc_count(n_rows, n_cols, frac, r_min, r_max):
+    """Discs of radius U(r_min, r_max) that mask about `frac` of an n_rows x n_cols scene."""
+    mean_r2 = (r_min * r_min + r_min * r_max + r_max * r_max) / 3.0
+    return int(np.ceil(-np.log1p(-frac) * n_rows * n_cols / (np.pi * mean_r2)))
+
+
+def nodata_discs_device(seed, n_rows, n_cols, n_discs, r_min, r_max, n_bands, d_rrs,
+                        nodata=-9999.0, stream=None):
+    """photic_gen_nodata_discs on a device Rrs array [n_bands][n_rows * n_cols]."""
+    check(lib.photic_gen_nodata_discs(seed, n_rows, n_cols, n_discs, r_min, r_max, n_bands,
+                                      nodata, d_rrs, stream))
 
 
 def gen_scene(cfg, seed, n, pix0=0, sigma=2e-4, deep_frac=0.0, truth=True):

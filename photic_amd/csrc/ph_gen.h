@@ -13,6 +13,7 @@
 //   block 0: u_H, u_B          block 1: u_theta, u_deep     block 2: u_Hdeep
 //   blocks 16+a / 48+a / 80+a: P / G / X attempt a (a < 32)  blocks 112+b: noise of band b
 //   stacks: cloud mask of date d from block 3 (redraw r from block 3 + r, r = 1..16)
+//   no-data discs: disc i (counter word 0 = i) centre from block 250, radius from block 251
 #pragma once
 
 #include "ph_engine.h"
@@ -186,4 +187,24 @@ PH_INLINE int ph_gen_stack_pixel(uint64_t seed, uint64_t pix, const PhConst& c, 
         k += valid[d];
     }
     return k;
+}
+
+// No-data blobs of a scene of n_rows x n_cols pixels (pixel index row * n_cols + col): n_discs
+// discs, disc i centred at (u0 n_cols, u1 n_rows) (block 250 of counter i) with radius
+// r_min + (r_max - r_min) u2 (block 251); a pixel is masked when its centre lies strictly
+// inside a disc.  Expected masked fraction about 1 - exp(-n_discs pi E[r^2] / area).
+PH_INLINE bool ph_in_nodata_disc(uint64_t seed, long row, long col, long n_rows, long n_cols,
+                                 int n_discs, double r_min, double r_max)
+{
+    const double y = (double)row + 0.5, x = (double)col + 0.5;
+    for (int i = 0; i < n_discs; i++) {
+        double u0, u1, u2, u3;
+        ph_uniforms(seed, (uint64_t)i, 250u, 0u, &u0, &u1);
+        ph_uniforms(seed, (uint64_t)i, 251u, 0u, &u2, &u3);
+        const double cx = u0 * (double)n_cols, cy = u1 * (double)n_rows;
+        const double r = r_min + (r_max - r_min) * u2;
+        const double dx = x - cx, dy = y - cy;
+        if (dx * dx + dy * dy < r * r) return true;
+    }
+    return false;
 }

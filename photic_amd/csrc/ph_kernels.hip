@@ -477,6 +477,30 @@ extern "C" int ph_launch_objective(const PhConst* c, long n, const double* x, co
     return (int)hipGetLastError();
 }
 
+// Sets every band of the pixels inside the scene's no-data discs to the no-data value.
+__global__ void ph_nodata_discs_kernel(uint64_t seed, long n_rows, long n_cols, int n_discs,
+                                       double r_min, double r_max, int n_bands, float nodata,
+                                       float* rrs)
+{
+    const long n = n_rows * n_cols;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (ph_in_nodata_disc(seed, p / n_cols, p % n_cols, n_rows, n_cols, n_discs, r_min, r_max))
+        for (int b = 0; b < n_bands; b++) rrs[(long)b * n + p] = nodata;
+}
+
+extern "C" int ph_launch_nodata_discs(uint64_t seed, long n_rows, long n_cols, int n_discs,
+                                      double r_min, double r_max, int n_bands, float nodata,
+                                      float* rrs, void* stream)
+{
+    const long n = n_rows * n_cols;
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(ph_nodata_discs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, seed, n_rows, n_cols, n_discs, r_min, r_max, n_bands,
+                       nodata, rrs);
+    return (int)hipGetLastError();
+}
+
 extern "C" int ph_launch_gen(const PhConst* c, uint64_t seed, long pix0, long n, double sigma,
                              double deep_frac, float* rrs, float* sec_sun, float* truth,
                              void* stream)

@@ -94,6 +94,8 @@ int ph_launch_fwd(const PhConst *c, long n, const double *params, const double *
 int ph_launch_objective(const PhConst *c, long n, const double *x, const float *rrs,
                         const float *sec_sun, double *out, double *rrs_error, int fp32,
                         int n_bands, void *stream);
+int ph_launch_nodata_discs(uint64_t seed, long n_rows, long n_cols, int n_discs, double r_min,
+                           double r_max, int n_bands, float nodata, float *rrs, void *stream);
 int ph_launch_gen(const PhConst *c, uint64_t seed, long pix0, long n, double sigma,
                   double deep_frac, float *rrs, float *sec_sun, float *truth, void *stream);
 int ph_kernel_attrs(int fp32, int n_bands, int n_bottoms, int *vgprs, int *sgprs,

@@ -674,6 +674,20 @@ int photic_gen_scene(const photic_cfg *cfg, uint64_t seed, long pix0, long n, do
         "scene generator");
 }
 
+int photic_gen_nodata_discs(uint64_t seed, long n_rows, long n_cols, int n_discs, double r_min,
+                            double r_max, int n_bands, float nodata, float *rrs, void *stream)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    if (n_rows < 0 || n_cols < 0 || n_discs < 0 || n_bands < 1 || n_bands > PH_MAXB ||
+        !(r_min >= 0.0) || !(r_max >= r_min))
+        return ph_fail(PHOTIC_EINVAL, "bad disc mask arguments");
+    if (n_rows * n_cols > 0 && !rrs) return ph_fail(PHOTIC_EINVAL, "rrs is required");
+    return ph_launch_status(ph_launch_nodata_discs(seed, n_rows, n_cols, n_discs, r_min, r_max,
+                                                   n_bands, nodata, rrs, stream),
+                            "no-data discs");
+}
+
 int photic_forward(const photic_cfg *cfg, long n, const double *params, const double *sec_sun,
                    const double *Y, double *out, int precision, void *stream)
 {

@@ -123,3 +123,34 @@ def test_pixel_inversion_reference_nelmin(mode, nbot):
               "status", "k_min", "iod", "b_pct", "bottom_type"):
         assert np.array_equal(a[k], b[k]), k
     assert (a["status"] == -1).sum() == nodata.sum()
+
+
+# --- the scene-scale no-data discs: geometry from the KAT-pinned Philox ---
+def _uniforms(seed, ctr0, block):
+    c = np.array([ctr0 & 0xFFFFFFFF, ctr0 >> 32, block, 0], np.uint32)
+    k = np.array([seed & 0xFFFFFFFF, seed >> 32], np.uint32)
+    o = np.zeros(4, np.uint32)
+    O.lib().orc_py_philox(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+    a = ((int(o[0]) << 32) | int(o[1])) >> 11
+    b = ((int(o[2]) << 32) | int(o[3])) >> 11
+    return a * 2.0 ** -53, b * 2.0 ** -53
+
+
+def test_nodata_discs_geometry():
+    """Every pixel of a small scene: masked exactly when its centre is strictly inside one of
+    the discs whose centres and radii come from the Philox draws (blocks 250 and 251 of counter
+    = disc index), and the masked fraction near 1 - exp(-n pi E[r^2] / area)."""
+    seed, rows, cols, nd, r0, r1 = 9, 180, 240, 30, 6.0, 14.0
+    yy, xx = np.mgrid[0:rows, 0:cols]
+    yc, xc = yy + 0.5, xx + 0.5
+    expect = np.zeros((rows, cols), bool)
+    for i in range(nd):
+        u0, u1 = _uniforms(seed, i, 250)
+        u2, _ = _uniforms(seed, i, 251)
+        cx, cy, r = u0 * cols, u1 * rows, r0 + (r1 - r0) * u2
+        expect |= (xc - cx) ** 2 + (yc - cy) ** 2 < r * r
+    got = O.disc_mask(seed, rows, cols, nd, r0, r1, np.arange(rows * cols)).reshape(rows, cols)
+    assert np.array_equal(got.astype(bool), expect)
+    mean_r2 = (r0 * r0 + r0 * r1 + r1 * r1) / 3.0
+    frac = 1.0 - np.exp(-nd * np.pi * mean_r2 / (rows * cols))
+    assert abs(got.mean() - frac) < 0.06
