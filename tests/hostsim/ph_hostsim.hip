@@ -83,7 +83,7 @@ extern "C" int phs_invert(const PhConst* c, int n_bottoms, long n_pix, const flo
     }
 }
 
-// Multi-date stacks (photic_amd/csrc/ph_stackengine.h): the pixels' valid dates in order,
+// Multi-date stacks (ph_stackengine.h beside this file): the pixels' valid dates in order,
 // k = their count, the instance for k; x [2 + 3 n_dates][n] at the dates' slots.
 template <int K>
 static void run_stack_pixel(PhConst c, long n_pix, int n_dates, long p, const int* slot,

@@ -1,22 +1,19 @@
-// photic_amd -- the lane engine for multi-date stacks.
+// Test infrastructure: the per-lane engine for multi-date stacks, on the host.
 //
-// The same state machine as ph_engine.h (one objective evaluation per kernel-loop pass per
-// lane, the modified AS 47 Nelder-Mead of model/asa047.c:10-502 and the multi-start of
-// model/samodel.c:2119-2427), for the mix layout over ND dates: [H, B, (P, G, X) per date],
-// N = 2 + 3 ND parameters.  With N up to 26 the simplex, steps, best point, trial points and
-// vertex values (about 780 doubles) cannot be registers, so they live in the lane's slice of
-// memory; the vertex values, which every iteration scans, and the scalars of the state machine
-// are registers.  Every memory pass is shaped to have its loads in flight together: the
-// centroid is accumulated one simplex row at a time (N loads per round trip, the per-
-// coordinate summation order unchanged).  Every arithmetic operation and its order is the one of
-// ph_engine.h, so the two engines give identical results where both apply (ND = 1).
+// The state machine of the product's ph_engine.h (one objective evaluation per step; the
+// modified AS 47 Nelder-Mead of model/asa047.c:10-502 and the multi-start of
+// model/samodel.c:2119-2427) for the mix layout over ND dates, [H, B, (P, G, X) per date],
+// N = 2 + 3 ND parameters, one stack at a time.  Built by build.sh with the host C library's
+// exp/log/pow it must match the FP64 oracle bit for bit; with the correctly rounded functions
+// of ph_crmath.h it is the reference the GPU's group engine (photic_amd/csrc/ph_grpengine.h)
+// is held to bit for bit.  The simplex, steps, best point and trial points live in a
+// caller-supplied array (element e at base[e * stride]); the vertex values and the scalars of
+// the state machine are members.  Never built into the library.
 #pragma once
 
 #include "ph_engine.h"
 
-// Element e of the lane lives at base[e * stride].  The kernel lays the lanes of a wavefront
-// side by side (stride 64, each wave's slice contiguous), so one slot of a wave is one
-// 512-byte run and a lane's slots are 512 bytes apart.
+// Element e of the stack's arrays lives at base[e * stride].
 template <int N>
 struct PhStackNM {
     static constexpr int P0 = 0;                 // vertices, [N + 1][N]

@@ -1,6 +1,8 @@
-"""Multi-date stacks on the CPU: the product's stack engine (photic_amd/csrc/ph_stackengine.h,
-built for the host by tests/hostsim) against the FP64 oracle bit for bit, and the oracle's
-multi-date problem against properties the reference's formulation fixes."""
+"""Multi-date stacks on the CPU: the per-lane stack engine (tests/hostsim/ph_stackengine.h, the
+state machine of the product's ph_engine.h over 2 + 3k parameters, built for the host) against
+the FP64 oracle bit for bit, and the oracle's multi-date problem against properties the
+reference's formulation fixes.  The GPU's group engine is held to this host engine bit for bit
+by tests/test_gpu_stack.py."""
 import numpy as np
 import pytest
 
