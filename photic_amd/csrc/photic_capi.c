@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime_api.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdlib.h>
 #include <stdio.h>
@@ -408,6 +409,30 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
 /* multi-date stacks                                                                      */
 /* ------------------------------------------------------------------------------------ */
 
+/* Two streams per device on which photic_invert_stack spreads its per-date-count launches,
+ * created on first use and kept (destroying a stream may wait for its work, which would make
+ * the call synchronous).  Calls on one device share them, so their launches queue there in
+ * call order; each call still waits only on events recorded after its own launches. */
+#define PH_SIDE 2
+#define PH_MAX_DEVICES 64
+static hipStream_t ph_side_tab[PH_MAX_DEVICES][PH_SIDE];
+static pthread_mutex_t ph_side_lock = PTHREAD_MUTEX_INITIALIZER;
+
+static int ph_side_streams(int dev, hipStream_t *out)
+{
+    if (dev < 0 || dev >= PH_MAX_DEVICES) return ph_fail(PHOTIC_EDEVICE, "device %d", dev);
+    int rc = PHOTIC_OK;
+    pthread_mutex_lock(&ph_side_lock);
+    for (int s = 0; s < PH_SIDE && !rc; s++) {
+        if (!ph_side_tab[dev][s])
+            rc = ph_hip(hipStreamCreateWithFlags(&ph_side_tab[dev][s], hipStreamNonBlocking),
+                        "hipStreamCreate");
+        out[s] = ph_side_tab[dev][s];
+    }
+    pthread_mutex_unlock(&ph_side_lock);
+    return rc;
+}
+
 int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
                         const float *sec_sun, const unsigned char *valid, const float *seed,
                         const photic_outputs *out, void *stream)
@@ -482,8 +507,19 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
     if (!rc)
         rc = ph_launch_status(ph_launch_zero_u64(queues, 2 * (n_dates + 1), st), "zero");
     if (!rc) rc = ph_launch_status(ph_launch_stack_bucket(&a, counts, lists, st), "bucketing");
-    /* one persistent launch per date count, each pulling its bucket's stacks; the group size
-     * and so the LDS per block, hence the resident blocks, depend on the date count */
+    /* One persistent launch per date count, each pulling its bucket's stacks; the group size
+     * and so the LDS per block, hence the resident blocks, depend on the date count.  The
+     * launches alternate between two side streams forked from the caller's, so one launch's
+     * tail (its last stacks finishing while most lanes idle) overlaps the next launch. */
+    hipStream_t side[PH_SIDE] = {NULL, NULL};
+    hipEvent_t ev_fork = NULL, ev_join[PH_SIDE] = {NULL, NULL};
+    if (!rc) rc = ph_side_streams(dev, side);
+    if (!rc) rc = ph_hip(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "event");
+    if (!rc) rc = ph_hip(hipEventRecord(ev_fork, st), "hipEventRecord");
+    for (int s = 0; s < PH_SIDE && !rc; s++) {
+        rc = ph_hip(hipEventCreateWithFlags(&ev_join[s], hipEventDisableTiming), "event");
+        if (!rc) rc = ph_hip(hipStreamWaitEvent(side[s], ev_fork, 0), "hipStreamWaitEvent");
+    }
     for (int k = n_dates; k >= 1 && !rc; k--) {
         photic_cfg one = *cfg;
         rc = ph_make_const(&one, k, seed != NULL, &a.c);
@@ -499,8 +535,18 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
             break;
         }
         const long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
-        rc = ph_launch_status(ph_launch_stack(&a, k, seed != NULL, (int)grid, block, st),
+        rc = ph_launch_status(ph_launch_stack(&a, k, seed != NULL, (int)grid, block,
+                                              side[(n_dates - k) % PH_SIDE]),
                               "stack kernel");
+    }
+    /* join: the caller's stream waits for both side streams (also on error, so the frees
+     * below stay ordered after every launch that was made) */
+    for (int s = 0; s < PH_SIDE; s++) {
+        if (!side[s]) continue;
+        if (ev_join[s] && hipEventRecord(ev_join[s], side[s]) == hipSuccess)
+            hipStreamWaitEvent(st, ev_join[s], 0);
+        else
+            hipStreamSynchronize(side[s]);
     }
     if (rc == PHOTIC_OK && cfg->check_watchdog) {
         unsigned long long flags[2 * (PH_MAXD + 1)];
@@ -517,6 +563,9 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
     if (queues) hipFreeAsync(queues, st);
     if (lists) hipFreeAsync(lists, st);
     if (counts) hipFreeAsync(counts, st);
+    for (int s = 0; s < PH_SIDE; s++)
+        if (ev_join[s]) hipEventDestroy(ev_join[s]); /* returns at once, released later */
+    if (ev_fork) hipEventDestroy(ev_fork);
     return rc;
 }
 
