@@ -114,17 +114,27 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
 
 /* Multi-date stacks: each pixel seen on n_dates (<= 8) dates, some of them invalid (clouds).
  * A pixel's problem is the reference's N_s = k run over its k valid dates, in date order
- * (model/samodel.c:1808-1815 layout, one region, the mix layout of PHOTIC_MODE_MIX): H and B
- * shared, P, G, X per date, 2 + 3k parameters, the misfit over every (date, band), early exit
- * at 2.5 k (cfg->early_exit > 0: that value times k).  Device arrays: rrs [n_dates][4][n_pix],
- * sec_sun [n_dates][n_pix], valid [n_dates][n_pix] (0 / 1), seed [2 + 3 n_dates][n_pix] or
- * NULL (H, B, then P, G, X at each date's slot; invalid slots ignored).  Outputs as
- * photic_invert; out->x is [2 + 3 n_dates][n_pix] with each valid date's P, G, X at its slot
- * and 0 at invalid slots; pixels with no valid date get the no-data defaults.  Mix layout,
- * 4 bands, FP64. */
+ * (model/samodel.c:1808-1815 layout, one region): H and the bottom shared, P, G, X per date,
+ * the misfit over every (date, band), early exit at 2.5 k (cfg->early_exit > 0: that value
+ * times k).  The bottom is PHOTIC_MODE_MIX's B (2 + 3k parameters) or PHOTIC_MODE_NATIVE's
+ * amplitudes and weights of cfg->n_bottoms (1..3) bottoms (1 + 2 n_bottoms + 3k).  Device
+ * arrays: rrs [n_dates][4][n_pix], sec_sun [n_dates][n_pix], valid [n_dates][n_pix] (0 / 1),
+ * seed [2 + 3 n_dates][n_pix] or NULL (mix layout only: H, B, then P, G, X at each date's
+ * slot; invalid slots ignored).  Outputs as photic_invert; out->x is [off + 3 n_dates][n_pix]
+ * (off = the bottom's coordinates plus one) with each valid date's P, G, X at its slot and 0 at
+ * invalid slots; pixels with no valid date get the no-data defaults.  4 bands, FP64. */
 int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
                         const float *sec_sun, const unsigned char *valid, const float *seed,
                         const photic_outputs *out, void *stream);
+
+/* As photic_invert_stack (unseeded), with per-date view geometry sec_view [n_dates][n_pix]
+ * (NULL: cfg->sec_view) and one depth start per pixel, hstart [n_pix] in metres (NULL: the
+ * configured list): the reference's multi-scene run with per-scene view angles and an
+ * empirical depth grid (model/samodel.c:534-548, 2208-2231). */
+int photic_invert_stack_h(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
+                          const float *sec_sun, const float *sec_view,
+                          const unsigned char *valid, const float *hstart,
+                          const photic_outputs *out, void *stream);
 
 /* Synthetic stack scene (Philox as photic_gen_scene, counter word 3 = date): H ~ U(0.5, 25),
  * B ~ U(0, 1) per pixel; per date sun zenith ~ U(20, 50) deg and log-normal P, G, X; a

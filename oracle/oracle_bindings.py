@@ -48,6 +48,8 @@ def lib():
         _lib.orc_py_objective_stack.restype = d
         _lib.orc_py_gen_stack.argtypes = [C.c_uint64, l, l, v, i, d, i, d, i, v, v, v, v]
         _lib.orc_py_invert_stack.restype = l
+        _lib.orc_py_invert_stack_ex.argtypes = [v, i, i, l, v, v, v, v, i, i, v, d, i, i, i, v,
+                                                v, C.c_float, v, v, v, C.POINTER(PyOut)]
         _lib.orc_py_invert_stack.argtypes = [v, i, i, l, v, v, v, d, i, i, i, v, C.c_float, v,
                                              v, v, C.POINTER(PyOut)]
         _lib.orc_py_start_step.restype = i
@@ -289,6 +291,41 @@ def invert_stack(rrs, sec_sun, valid, reqmin=1e-10, konvge=100, kcount=2000,
     lib().orc_py_invert_stack(w.ctypes.data, nb, nd, n, rrs.ctypes.data, ss.ctypes.data,
                               va.ctypes.data, reqmin, konvge, kcount, len(hs), hs.ctypes.data,
                               nodata, _p(sd), sr.ctypes.data, sm.ctypes.data, C.byref(o))
+    return out
+
+
+def invert_stack_native(rrs, sec_sun, valid, n_bottoms=2, bottom_idx=(0, 1, 2), sec_view=None,
+                        hstart_px=None, reqmin=1e-2, konvge=100, kcount=5000,
+                        hstart=(40.0, 30.0, 20.0, 15.0, 10.0, 7.5, 2.5, 1.0), nodata=-9999.0,
+                        wl=WL4):
+    """Oracle inversion of multi-date stacks in the reference's native layout
+    [H, B_k, q_k, (P, G, X) per date]; sec_view [D][n] or None; hstart_px [n] or None."""
+    rrs = np.ascontiguousarray(rrs, np.float32)
+    nd, nb, n = rrs.shape
+    npar = 1 + 2 * n_bottoms + 3 * nd
+    out = {
+        "x": np.zeros((npar, n)), "depth": np.zeros(n, np.float32),
+        "rrs_error": np.zeros(n, np.float32), "model_error": np.zeros(n, np.float32),
+        "k_min": np.zeros(n, np.float32), "iod": np.zeros(n, np.float32),
+        "albedo": np.zeros(n, np.float32), "bottom_type": np.zeros(n, np.float32),
+        "b_pct": np.zeros((3, n), np.float32), "n_iter": np.zeros(n, np.int32),
+        "converged": np.zeros(n, np.int32), "n_evals": np.zeros(n, np.int32),
+        "n_starts": np.zeros(n, np.int32), "status": np.zeros(n, np.int32),
+        "misfit": np.zeros(n),
+    }
+    o = PyOut(**{k: v.ctypes.data for k, v in out.items()})
+    w = np.asarray(wl, np.float64)
+    hs = np.asarray(hstart, np.float64)
+    bi = np.asarray(bottom_idx, np.int32)
+    ss = np.ascontiguousarray(sec_sun, np.float32)
+    sv = None if sec_view is None else np.ascontiguousarray(sec_view, np.float32)
+    hp = None if hstart_px is None else np.ascontiguousarray(hstart_px, np.float32)
+    va = np.ascontiguousarray(valid, np.uint8)
+    sr = np.zeros(5)
+    lib().orc_py_invert_stack_ex(w.ctypes.data, nb, nd, n, rrs.ctypes.data, ss.ctypes.data,
+                                 _p(sv), va.ctypes.data, 0, n_bottoms, bi.ctypes.data, reqmin,
+                                 konvge, kcount, len(hs), hs.ctypes.data, _p(hp), nodata, None,
+                                 sr.ctypes.data, sr.ctypes.data, C.byref(o))
     return out
 
 

@@ -321,15 +321,21 @@ void orc_py_gen_stack(uint64_t seed, long pix0, long n, const double *wl, int nb
  * [2 + 3 n_dates][n] or NULL: H, B, then (P, G, X) at each date's slot (invalid slots
  * ignored).  x [2 + 3 n_dates][n]: H, B and each valid date's P, G, X at its date slot (0 at
  * invalid slots).  Other outputs as orc_py_invert. */
-long orc_py_invert_stack(const double *wl, int nb, int n_dates, long n, const float *rrs,
-                         const float *sec_sun, const uint8_t *valid, double reqmin,
-                         int konvge, int kcount, int n_hstarts, const double *hstart,
-                         float nodata, const float *seed, const double *seed_rel,
-                         const double *seed_min, orc_py_out *o)
+/* Multi-date stacks: each pixel's valid dates, in order, are one N_s = k problem
+ * (model/samodel.c:1808-1815): mode 1 the mix layout [H, B, (P, G, X) per date], mode 0 the
+ * native layout [H, B_k, q_k, (P, G, X) per date] with n_bottoms bottoms; per-date sec(view)
+ * [n_dates][n] or NULL (nadir); hstart_px [n]: one depth start per pixel, or NULL (the list). */
+long orc_py_invert_stack_ex(const double *wl, int nb, int n_dates, long n, const float *rrs,
+                            const float *sec_sun, const float *sec_view, const uint8_t *valid,
+                            int mode, int n_bottoms, const int *bottom_idx, double reqmin,
+                            int konvge, int kcount, int n_hstarts, const double *hstart,
+                            const float *hstart_px, float nodata, const float *seed,
+                            const double *seed_rel, const double *seed_min, orc_py_out *o)
 {
     orc_bands bands;
     orc_band_tables(wl, nb, &bands);
-    const int np_max = 2 + 3 * n_dates;
+    const int off = mode == ORC_MODE_MIX ? 2 : 1 + 2 * n_bottoms;
+    const int np_max = off + 3 * n_dates;
     long done = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : done)
     for (long p = 0; p < n; p++) {
@@ -345,7 +351,7 @@ long orc_py_invert_stack(const double *wl, int nb, int n_dates, long n, const fl
                 r[k * nb + b] = (double)v;
             }
             ss[k] = (double)sec_sun[(long)d * n + p];
-            sv[k] = 1.0;
+            sv[k] = sec_view ? (double)sec_view[(long)d * n + p] : 1.0;
             slot[k++] = d;
         }
         if (o->x)
@@ -356,19 +362,23 @@ long orc_py_invert_stack(const double *wl, int nb, int n_dates, long n, const fl
             continue;
         }
         orc_pixel px;
-        orc_pixel_init(&px, ORC_MODE_MIX, 2, NULL, k, &bands, ss, sv, r);
+        orc_pixel_init(&px, mode, n_bottoms, bottom_idx, k, &bands, ss, sv, r);
         orc_nm_cfg cfg;
         orc_default_cfg(&cfg, k, reqmin, konvge, kcount);
         cfg.n_hstarts = n_hstarts;
         for (int h = 0; h < n_hstarts && h < 16; h++)
             cfg.hstart[h] = hstart[h];
+        if (hstart_px) {
+            cfg.n_hstarts = 1;
+            cfg.hstart[0] = (double)hstart_px[p];
+        }
         orc_result res;
         const int np_ = orc_n_params(&px);
         if (seed) {
             double st[ORC_MAXN], sp[ORC_MAXN];
             for (int i = 0; i < np_; i++) {
-                int src = i < 2 ? i : 2 + 3 * slot[(i - 2) / 3] + (i - 2) % 3;
-                int cls = i == 0 ? 0 : (i < 2 ? 1 : 2 + (i - 2) % 3);
+                int src = i < off ? i : off + 3 * slot[(i - off) / 3] + (i - off) % 3;
+                int cls = i == 0 ? 0 : (i < off ? 1 : 2 + (i - off) % 3);
                 st[i] = (double)seed[(long)src * n + p];
                 double s = fabs(st[i]) * seed_rel[cls];
                 if (s < seed_min[cls]) s = seed_min[cls];
@@ -380,7 +390,7 @@ long orc_py_invert_stack(const double *wl, int nb, int n_dates, long n, const fl
         }
         if (o->x)
             for (int i = 0; i < np_; i++) {
-                int dst = i < 2 ? i : 2 + 3 * slot[(i - 2) / 3] + (i - 2) % 3;
+                int dst = i < off ? i : off + 3 * slot[(i - off) / 3] + (i - off) % 3;
                 o->x[(long)dst * n + p] = res.x[i];
             }
         if (o->depth) o->depth[p] = (float)res.depth;
@@ -401,6 +411,17 @@ long orc_py_invert_stack(const double *wl, int nb, int n_dates, long n, const fl
         done++;
     }
     return done;
+}
+
+long orc_py_invert_stack(const double *wl, int nb, int n_dates, long n, const float *rrs,
+                         const float *sec_sun, const uint8_t *valid, double reqmin,
+                         int konvge, int kcount, int n_hstarts, const double *hstart,
+                         float nodata, const float *seed, const double *seed_rel,
+                         const double *seed_min, orc_py_out *o)
+{
+    return orc_py_invert_stack_ex(wl, nb, n_dates, n, rrs, sec_sun, NULL, valid, ORC_MODE_MIX,
+                                  2, NULL, reqmin, konvge, kcount, n_hstarts, hstart, NULL,
+                                  nodata, seed, seed_rel, seed_min, o);
 }
 
 /* Objective of one mix-layout vector over k dates: x [2 + 3k], rrs [k][nb], sec_sun [k];

@@ -130,6 +130,8 @@ def _load():
         "photic_band_coefficients": (i, [C.POINTER(Cfg), vp, C.POINTER(d)]),
         "photic_invert_stack": (i, [C.POINTER(Cfg), l, i, vp, vp, vp, vp, C.POINTER(Outputs),
                                     vp]),
+        "photic_invert_stack_h": (i, [C.POINTER(Cfg), l, i, vp, vp, vp, vp, vp,
+                                      C.POINTER(Outputs), vp]),
         "photic_autotune": (i, [C.POINTER(Cfg), l, vp, vp, vp, C.POINTER(Tuning), vp]),
         "photic_autotune_stack": (i, [C.POINTER(Cfg), l, i, vp, vp, vp, vp, C.POINTER(Tuning),
                                       vp]),
@@ -298,11 +300,12 @@ def invert(cfg, rrs, sec_sun=None, seed=None, fields=None):
 
 
 class StackOutputs(DeviceOutputs):
-    """Device output buffers for n stacks of n_dates dates (x is [2 + 3 n_dates][n])."""
+    """Device output buffers for n stacks of n_dates dates (x is [off + 3 n_dates][n], off = 2
+    in the mix layout, 1 + 2 n_bottoms in the native one)."""
 
     def __init__(self, cfg, n, n_dates, fields=None):
         self.n = n
-        self.np = 2 + 3 * n_dates
+        self.np = (2 if cfg.mode == MODE_MIX else 1 + 2 * cfg.n_bottoms) + 3 * n_dates
         self.bufs = {}
         self.struct = Outputs()
         for name, (dt, rows) in _OUT_FIELDS.items():
@@ -324,9 +327,10 @@ def invert_stack_device(cfg, n, n_dates, d_rrs, d_sec_sun, d_valid, d_seed=None,
     return outs
 
 
-def invert_stack(cfg, rrs, sec_sun, valid, seed=None, fields=None):
+def invert_stack(cfg, rrs, sec_sun, valid, seed=None, fields=None, sec_view=None, hstart=None):
     """Invert host arrays: rrs [D][4][n] float32, sec_sun [D][n], valid [D][n] uint8,
-    seed [2 + 3D][n] or None."""
+    seed [2 + 3D][n] or None; sec_view [D][n] and hstart [n] (photic_invert_stack_h) or
+    None."""
     rrs = np.ascontiguousarray(rrs, dtype=np.float32)
     nd, _, n = rrs.shape
     d_rrs = DeviceBuffer.from_array(rrs)
@@ -334,8 +338,16 @@ def invert_stack(cfg, rrs, sec_sun, valid, seed=None, fields=None):
     d_v = DeviceBuffer.from_array(np.asarray(valid, np.uint8))
     d_seed = DeviceBuffer.from_array(np.asarray(seed, np.float32)) if seed is not None else None
     outs = StackOutputs(cfg, n, nd, fields)
-    invert_stack_device(cfg, n, nd, d_rrs.value, d_ss.value, d_v.value,
-                        d_seed.value if d_seed is not None else None, outs)
+    if sec_view is not None or hstart is not None:
+        assert seed is None
+        d_sv = DeviceBuffer.from_array(np.asarray(sec_view, np.float32)) if sec_view is not None else None
+        d_h = DeviceBuffer.from_array(np.asarray(hstart, np.float32)) if hstart is not None else None
+        check(lib.photic_invert_stack_h(C.byref(cfg), n, nd, d_rrs.value, d_ss.value,
+                                        d_sv.value if d_sv else None, d_v.value,
+                                        d_h.value if d_h else None, C.byref(outs.struct), None))
+    else:
+        invert_stack_device(cfg, n, nd, d_rrs.value, d_ss.value, d_v.value,
+                            d_seed.value if d_seed is not None else None, outs)
     check(lib.photic_synchronize())
     return outs.fetch()
 

@@ -1,10 +1,11 @@
 // photic_amd -- the group engine for multi-date stacks (gfx950, device only).
 //
 // A stack with k valid dates is the reference's N_s = k problem (model/samodel.c:1808-1815
-// layout, one region, sand/seagrass mix): depth H and mix B shared, P, G, X per date, so
-// N = 2 + 3k parameters, and a misfit over 4k (date, band) pairs.  With N up to 26 one lane
-// cannot hold a stack's simplex, so a group of G lanes (G = 8, 16 or 32: the smallest power of
-// two >= max(N, 4k)) runs one stack:
+// layout, one region): depth H and the bottom shared, P, G, X per date, and a misfit over the
+// 4k (date, band) pairs.  The bottom is either the sand/seagrass mix B (NBOT = 0, N = 2 + 3k)
+// or the reference's native amplitudes and weights of 1-3 bottoms (N = 1 + 2 NBOT + 3k).  With
+// N up to 31 one lane cannot hold a stack's simplex, so a group of G lanes (G = 8, 16 or 32:
+// the smallest power of two >= max(N, 4k)) runs one stack:
 //
 //   lane i < N      owns coordinate i: its start point, centroid, trial point, step and best
 //                   value in registers, its column of the simplex in LDS;
@@ -15,8 +16,8 @@
 //                   (model/asa047.c:10-502) and the multi-start (model/samodel.c:2119-2427)
 //                   on identical values, so the group stays convergent without votes.
 //
-// Cross-lane traffic is two exchanges per evaluation: the trial point's H, B and the pair's
-// date's P, G, X come from their owner lanes by ds_bpermute, and the pairs' squared residuals
+// Cross-lane traffic is two exchanges per evaluation: the trial point's H and bottom and the
+// pair's date's P, G, X come from their owner lanes by ds_bpermute, and the pairs' squared residuals
 // and attenuations go through a per-stack LDS buffer from which every lane re-forms the misfit
 // in the reference's (date, band) order.  Every floating-point operation, and its order, is the
 // one of the per-lane stack engine (tests/hostsim), so the results agree bit for bit.
@@ -26,15 +27,20 @@
 
 #define PH_DEV __device__ __attribute__((always_inline)) inline
 
-// Lanes per stack for nd dates.
-__host__ __device__ constexpr int ph_grp_lanes(int nd) { return nd <= 2 ? 8 : (nd <= 4 ? 16 : 32); }
+// Lanes per stack: the smallest of 8, 16, 32 that holds n coordinates and np pairs.
+__host__ __device__ constexpr int ph_grp_lanes(int n, int np)
+{
+    return (n <= 8 && np <= 8) ? 8 : ((n <= 16 && np <= 16) ? 16 : 32);
+}
 
 // A stack's LDS block, in doubles.  Sub-arrays start at even offsets (16-byte aligned).
-template <int ND>
+template <int ND, int NBOT>
 struct PhGrpLayout {
-    static constexpr int N = 2 + 3 * ND;
+    static constexpr int N = PhLayout<NBOT>::template n<ND>();
+    static constexpr int OFF = PhLayout<NBOT>::off;   // first P coordinate
+    static constexpr int NBT = PhLayout<NBOT>::nbot;  // bottoms in rho
     static constexpr int NP = 4 * ND;
-    static constexpr int G = ph_grp_lanes(ND);
+    static constexpr int G = ph_grp_lanes(N, NP);
     static constexpr int P0 = 0;                        // simplex, [N + 1][N] (lane i: column i)
     static constexpr int Y0 = (N + 1) * N;              // vertex values, [N + 1]
     static constexpr int D0 = Y0 + ((N + 2) & ~1);      // squared residual per pair, [NP]
@@ -65,17 +71,19 @@ PH_DEV double ph_band_sel(const double* v, int b)
     return r;
 }
 
-template <int ND>
+template <int ND, int NBOT>
 struct PhGrp {
-    using L = PhGrpLayout<ND>;
+    using L = PhGrpLayout<ND, NBOT>;
     double* g; // the stack's LDS block
     int li;    // lane index in the group
 
     // coordinate li (li < N)
     double xs, xb, xe, step, best;
     double xs0, step0; // start value and step of the depth starts (li >= 1)
-    // pair li (li < NP)
-    double rrs, pw, ss, sv, a0, a1, aw, bbw, eg, bot0, bot1;
+    double h0;         // the stack's own depth start (< 0: the configured list)
+    // pair li (li < NP): its band's reflectance of each bottom in rho
+    double rrs, pw, ss, sv, a0, a1, aw, bbw, eg;
+    double bt[L::NBT];
 
     // the state machine, identical in every lane of the group
     double tot; // sum of the measured Rrs in (date, band) order
@@ -96,30 +104,43 @@ struct PhGrpSide {
 
 // The misfit of the point whose coordinate li is xv (model/samodel.c:2432-2759, one region,
 // mix layout, ND dates).  Called by all G lanes of the group together.
-template <int ND>
-PH_DEV double ph_grp_objective(const PhGrp<ND>& S, const PhConst& c, double xv, bool side_on,
-                               PhGrpSide* side)
+template <int ND, int NBOT>
+PH_DEV double ph_grp_objective(const PhGrp<ND, NBOT>& S, const PhConst& c, double xv,
+                               bool side_on, PhGrpSide* side)
 {
-    using L = PhGrpLayout<ND>;
-    constexpr int G = L::G;
+    using L = PhGrpLayout<ND, NBOT>;
+    constexpr int G = L::G, NBT = L::NBT;
     const int li = S.li;
     const int sd = li < L::NP ? (li >> 2) : 0;
     const double x0 = __shfl(xv, 0, G);
-    const double x1 = __shfl(xv, 1, G);
-    const double xp = __shfl(xv, 2 + 3 * sd, G);
-    const double xg = __shfl(xv, 3 + 3 * sd, G);
-    const double xx = __shfl(xv, 4 + 3 * sd, G);
-    double B0 = 0.01 * fabs(100.0), B1 = 0.01 * fabs(100.0);
-    double q0 = fabs(fabs(x1)), q1 = fabs(fabs(1.0 - x1));
-    double qsum = 0.0;
-    qsum += q0;
-    qsum += q1;
-    q0 /= qsum;
-    q1 /= qsum;
-    const double qb0 = q0 * B0, qb1 = q1 * B1;
-    double alb = 0.0;
-    alb += qb0;
-    alb += qb1;
+    const double xp = __shfl(xv, L::OFF + 3 * sd, G);
+    const double xg = __shfl(xv, L::OFF + 1 + 3 * sd, G);
+    const double xx = __shfl(xv, L::OFF + 2 + 3 * sd, G);
+    // the bottom: B_k and q_k (the mix: both amplitudes 100, q = (|B|, |1 - B|)), q normalised
+    double Bk[NBT], q[NBT], qsum = 0.0;
+    if constexpr (NBOT == 0) {
+        const double x1 = __shfl(xv, 1, G);
+        Bk[0] = 0.01 * fabs(100.0);
+        Bk[1] = 0.01 * fabs(100.0);
+        q[0] = fabs(fabs(x1));
+        q[1] = fabs(fabs(1.0 - x1));
+    } else {
+#pragma unroll
+        for (int k = 0; k < NBT; k++) {
+            Bk[k] = 0.01 * fabs(__shfl(xv, 1 + k, G));
+            q[k] = fabs(__shfl(xv, 1 + NBT + k, G));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NBT; k++) qsum += q[k];
+#pragma unroll
+    for (int k = 0; k < NBT; k++) q[k] /= qsum;
+    double qb[NBT], alb = 0.0;
+#pragma unroll
+    for (int k = 0; k < NBT; k++) {
+        qb[k] = q[k] * Bk[k];
+        alb += qb[k];
+    }
     const double H = fabs(x0);
     const double kmm = 0.275, kmn = 0.185;
 
@@ -129,8 +150,8 @@ PH_DEV double ph_grp_objective(const PhGrp<ND>& S, const PhConst& c, double xv, 
         const double Gc = 0.01 * fabs(xg);
         const double X = 0.01 * fabs(xx);
         double rho = 0.0;
-        rho += qb0 * S.bot0;
-        rho += qb1 * S.bot1;
+#pragma unroll
+        for (int k = 0; k < NBT; k++) rho += qb[k] * S.bt[k];
         double K, rB, r;
         const double m = ph_band64(P, Gc, X, rho, H, S.a0, S.a1, S.aw, S.bbw, S.eg, S.pw, S.ss,
                                    S.sv, &K, &rB, &r);
@@ -202,13 +223,15 @@ PH_DEV double ph_grp_objective(const PhGrp<ND>& S, const PhConst& c, double xv, 
 }
 
 // Start point and step of coordinate li for depth start k (k < 0: seed_i), model/samodel.c:
-// 2246-2354, mix layout: H = the start depth (step 1.25 H); B, P, G, X from xs0 / step0.
-template <int ND>
-PH_DEV void ph_grp_start_step(PhGrp<ND>& S, const PhConst& c, int k, double seed_i)
+// 2246-2354: H = the start depth (step 1.25 H) -- the stack's own when it has one; the bottom,
+// P, G, X from xs0 / step0.
+template <int ND, int NBOT>
+PH_DEV void ph_grp_start_step(PhGrp<ND, NBOT>& S, const PhConst& c, int k, double seed_i)
 {
+    constexpr int OFF = PhGrpLayout<ND, NBOT>::OFF;
     const int li = S.li;
     if (k < 0) {
-        const int cls = li == 0 ? 0 : (li < 2 ? 1 : 2 + (li - 2) % 3);
+        const int cls = li == 0 ? 0 : (li < OFF ? 1 : 2 + (li - OFF) % 3);
         double rel = c.seed_rel_step[0], mn = c.seed_min_step[0];
         ph_for<4>([&](auto m0) PH_BODY {
             constexpr int m = m0 + 1;
@@ -226,30 +249,31 @@ PH_DEV void ph_grp_start_step(PhGrp<ND>& S, const PhConst& c, int k, double seed
         constexpr int m = m0 + 1;
         h0 = ph_pick(k == m, c.hstart[m], h0);
     });
+    if (S.h0 >= 0.0) h0 = S.h0;
     S.xs = li == 0 ? h0 : S.xs0;
     S.step = li == 0 ? 1.25 * h0 : S.step0;
 }
 
 // Begin a Nelder-Mead run from xs and the steps (asa047.c:162-170).
-template <int ND>
-PH_DEV void ph_grp_nm_begin(PhGrp<ND>& S, const PhConst& c)
+template <int ND, int NBOT>
+PH_DEV void ph_grp_nm_begin(PhGrp<ND, NBOT>& S, const PhConst& c)
 {
     S.icount = 0;
     S.numres = 0;
     S.jcount = c.konvge;
     S.del = 1.0;
     S.phase = PH_BUILD;
-    S.sub = PhGrpLayout<ND>::N;
+    S.sub = PhGrpLayout<ND, NBOT>::N;
     S.xe = S.xs;
 }
 
 // One step of the state machine: consume f, the value of the point in xe in phase S.phase,
 // and leave the next point to evaluate in xe (or PH_START / PH_DONE) -- the per-lane stack
 // engine's step, with each loop over coordinates done by the coordinate's lane.
-template <int ND>
-PH_DEV void ph_grp_consume(PhGrp<ND>& S, const PhConst& c, double f)
+template <int ND, int NBOT>
+PH_DEV void ph_grp_consume(PhGrp<ND, NBOT>& S, const PhConst& c, double f)
 {
-    constexpr int N = PhGrpLayout<ND>::N;
+    constexpr int N = PhGrpLayout<ND, NBOT>::N;
     const int li = S.li;
     const bool co = li < N;
     const double dn = (double)N, dnn = (double)(N + 1);

@@ -68,6 +68,9 @@ typedef struct PhStackArgs {
     int n_dates;                /* dates in the scene arrays (D) */
     const float *rrs;           /* [D][n_bands][n_pix] */
     const float *sec_sun;       /* [D][n_pix] */
+    const float *sec_view;      /* [D][n_pix], or NULL: c.sec_view0 */
+    const float *hstart;        /* [n_pix]: one depth start per stack, or NULL: c.hstart */
+    int x_off;                  /* first P row of x_out: 2 (mix) or 1 + 2 n_bottoms */
     const unsigned char *valid; /* [D][n_pix] */
     const float *seed;          /* [2 + 3 D][n_pix]: H, B, (P, G, X) per date slot, or NULL */
     const int *idx;             /* the bucket's pixel indices */
@@ -107,9 +110,9 @@ int ph_lds_bytes(int n_bottoms, int block);
 /* multi-date stacks (ph_stack.hip) */
 int ph_launch_stack_bucket(const PhStackArgs *a, unsigned int *counts, int *lists,
                            void *stream);
-int ph_launch_stack(const PhStackArgs *a, int k, int seeded, int grid, int block,
+int ph_launch_stack(const PhStackArgs *a, int k, int nbot, int seeded, int grid, int block,
                     void *stream);
-int ph_stack_occupancy(int k, int seeded, int block, int *blocks_per_cu);
+int ph_stack_occupancy(int k, int nbot, int seeded, int block, int *blocks_per_cu);
 int ph_launch_gen_stack(const PhConst *c, uint64_t seed, long pix0, long n, int n_dates,
                         double sigma, double p_cloud, int k_min, float *rrs, float *sec_sun,
                         unsigned char *valid, float *truth, void *stream);

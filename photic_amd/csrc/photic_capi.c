@@ -433,9 +433,10 @@ static int ph_side_streams(int dev, hipStream_t *out)
     return rc;
 }
 
-int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
-                        const float *sec_sun, const unsigned char *valid, const float *seed,
-                        const photic_outputs *out, void *stream)
+static int ph_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
+                           const float *sec_sun, const float *sec_view,
+                           const unsigned char *valid, const float *seed, const float *hstart,
+                           const photic_outputs *out, void *stream)
 {
     int rc = ph_have_device();
     if (rc) return rc;
@@ -443,13 +444,19 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
     if (n_pix < 0) return ph_fail(PHOTIC_EINVAL, "n_pix < 0");
     if (n_dates < 1 || n_dates > PH_MAXD)
         return ph_fail(PHOTIC_EINVAL, "n_dates %d outside 1..%d", n_dates, PH_MAXD);
-    if (cfg->mode != PHOTIC_MODE_MIX || cfg->n_bands != 4 || cfg->precision != PHOTIC_FP64)
+    const int native = cfg->mode == PHOTIC_MODE_NATIVE;
+    if (cfg->n_bands != 4 || cfg->precision != PHOTIC_FP64 ||
+        (native && (cfg->n_bottoms < 1 || cfg->n_bottoms > 3)))
         return ph_fail(PHOTIC_EUNSUPPORTED,
-                       "stacks are compiled for the mix layout, 4 bands, FP64");
+                       "stacks are compiled for 4 bands, FP64, the mix layout or 1..3 native "
+                       "bottoms");
+    if (native && seed)
+        return ph_fail(PHOTIC_EUNSUPPORTED, "seeded stacks are compiled for the mix layout");
     if (n_pix == 0) return PHOTIC_OK;
     if (!rrs || !sec_sun || !valid)
         return ph_fail(PHOTIC_EINVAL, "rrs, sec_sun and valid are required");
     if (n_pix > 2147483647L) return ph_fail(PHOTIC_EINVAL, "n_pix above 2^31 - 1");
+    const int nbot = native ? cfg->n_bottoms : 0;
 
     PhStackArgs a;
     memset(&a, 0, sizeof(a));
@@ -457,6 +464,9 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
     a.n_dates = n_dates;
     a.rrs = rrs;
     a.sec_sun = sec_sun;
+    a.sec_view = sec_view;
+    a.hstart = hstart;
+    a.x_off = native ? 1 + 2 * nbot : 2;
     a.valid = valid;
     a.seed = seed;
     a.chunk = cfg->chunk > 0 ? cfg->chunk : (ph_tuned_stack[seed != NULL].chunk > 0
@@ -528,14 +538,16 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
         a.idx = lists + (long)k * n_pix;
         a.count = counts + k;
         a.queue = queues + 2 * k;
-        rc = ph_launch_status(ph_stack_occupancy(k, seed != NULL, block, &bpc), "occupancy");
+        if (hstart) a.c.n_hstarts = 1; /* the stack's own depth start replaces the list */
+        rc = ph_launch_status(ph_stack_occupancy(k, nbot, seed != NULL, block, &bpc),
+                              "occupancy");
         if (rc) break;
         if (bpc < 1) {
             rc = ph_fail(PHOTIC_EDEVICE, "stack kernel (%d dates) cannot be resident", k);
             break;
         }
         const long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
-        rc = ph_launch_status(ph_launch_stack(&a, k, seed != NULL, (int)grid, block,
+        rc = ph_launch_status(ph_launch_stack(&a, k, nbot, seed != NULL, (int)grid, block,
                                               side[(n_dates - k) % PH_SIDE]),
                               "stack kernel");
     }
@@ -567,6 +579,23 @@ int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const fl
         if (ev_join[s]) hipEventDestroy(ev_join[s]); /* returns at once, released later */
     if (ev_fork) hipEventDestroy(ev_fork);
     return rc;
+}
+
+int photic_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
+                        const float *sec_sun, const unsigned char *valid, const float *seed,
+                        const photic_outputs *out, void *stream)
+{
+    return ph_invert_stack(cfg, n_pix, n_dates, rrs, sec_sun, NULL, valid, seed, NULL, out,
+                           stream);
+}
+
+int photic_invert_stack_h(const photic_cfg *cfg, long n_pix, int n_dates, const float *rrs,
+                          const float *sec_sun, const float *sec_view,
+                          const unsigned char *valid, const float *hstart,
+                          const photic_outputs *out, void *stream)
+{
+    return ph_invert_stack(cfg, n_pix, n_dates, rrs, sec_sun, sec_view, valid, NULL, hstart,
+                           out, stream);
 }
 
 int photic_gen_stack(const photic_cfg *cfg, uint64_t seed, long pix0, long n, int n_dates,
@@ -657,7 +686,7 @@ static int ph_autotune(int stack, const photic_cfg *cfg, long n_pix, int n_dates
         if (stack && block == 192) continue;
         if (!stack && (long)ph_lds_bytes(nbot, 1) * block > max_lds) continue;
         int bpc = 0;
-        rc = stack ? ph_launch_status(ph_stack_occupancy(n_dates, seed != NULL, block, &bpc), "occupancy")
+        rc = stack ? ph_launch_status(ph_stack_occupancy(n_dates, nbot, seed != NULL, block, &bpc), "occupancy")
                    : ph_launch_status(ph_kernel_occupancy(fp32, cfg->n_bands, nbot, seed != NULL,
                                                           block, &bpc), "occupancy");
         if (rc || bpc < 1) continue;

@@ -1,10 +1,12 @@
 /* photic_amd -- C host code: the drop-in samodel() of include/photic_samodel.h.
  *
  * Grid in, grid out, like model/samodel.c:289-1763: the caller's row-pointer band grids are
- * screened for no-data (model/samodel.c:933-947), smoothed as model/common.c:232-276 does
- * when n_smoothing_radius > 1, packed into planar pixel arrays per depth-start group, inverted
- * on the GPU by photic_invert / photic_invert_h, and the outputs scattered back into the
- * caller's grids with the reference's defaults and sign conventions. */
+ * screened for no-data on every scene (model/samodel.c:933-947), smoothed as
+ * model/common.c:232-276 does when n_smoothing_radius > 1, packed into planar pixel arrays per
+ * depth-start group, inverted on the GPU -- one scene by photic_invert_h, several (the
+ * reference's N_s = nscenes run, model/samodel.c:1808-1815) by photic_invert_stack_h with
+ * each scene's sun and view angles -- and the outputs scattered back into the caller's grids
+ * with the reference's defaults and sign conventions. */
 #include "photic_samodel.h"
 
 #include <math.h>
@@ -43,24 +45,43 @@ static float sm_smoothed(float **a, int i, int j, int nrows, int ncols, int r, f
 typedef struct {
     long n;
     long *idx;      /* pixel index i * ncols + j */
-    float *rrs;     /* [4][n] */
+    float *rrs;     /* [nd][4][n] */
     float *hstart;  /* [n] or NULL */
 } sm_group;
 
-static int sm_run_group(const photic_cfg *cfg, sm_group *g, int ncols, float **depth,
+/* nd > 1: the scenes' sec(sun) and sec(view), one value per scene */
+static int sm_run_group(const photic_cfg *cfg, sm_group *g, int nd, const double *sec_sun,
+                        const double *sec_view, int ncols, float **depth,
                         float **model_error, float **bottom_albedo, float **bottom_sand,
                         float **bottom_seagrass, float **bottom_coral, float **K_min,
                         float **bottom_type, float **index_optical_depth)
 {
     if (g->n == 0) return PHOTIC_OK;
     const long n = g->n;
-    float *d_rrs = NULL, *d_h = NULL, *d_out = NULL;
-    int rc = photic_device_malloc((void **)&d_rrs, sizeof(float) * 4 * n);
+    float *d_rrs = NULL, *d_h = NULL, *d_out = NULL, *d_geo = NULL, *geo = NULL;
+    unsigned char *d_valid = NULL, *valid = NULL;
+    int rc = photic_device_malloc((void **)&d_rrs, sizeof(float) * 4 * nd * n);
     if (!rc && g->hstart) rc = photic_device_malloc((void **)&d_h, sizeof(float) * n);
     /* outputs: depth, rrs_error, albedo, k_min, iod, bottom_type, b_pct[3] */
     if (!rc) rc = photic_device_malloc((void **)&d_out, sizeof(float) * 9 * n);
-    if (!rc) rc = photic_memcpy_htod(d_rrs, g->rrs, sizeof(float) * 4 * n);
+    if (!rc) rc = photic_memcpy_htod(d_rrs, g->rrs, sizeof(float) * 4 * nd * n);
     if (!rc && d_h) rc = photic_memcpy_htod(d_h, g->hstart, sizeof(float) * n);
+    if (!rc && nd > 1) {
+        /* every scene valid (a pixel with no data on any scene was not gathered); the
+         * scenes' geometry as [nd][n] sec(sun) then [nd][n] sec(view) */
+        geo = (float *)malloc(sizeof(float) * 2 * nd * n);
+        valid = (unsigned char *)malloc((size_t)nd * n);
+        if (!geo || !valid) rc = PHOTIC_ENOMEM;
+        for (long q = 0; !rc && q < (long)nd * n; q++) {
+            geo[q] = (float)sec_sun[q / n];
+            geo[(long)nd * n + q] = (float)sec_view[q / n];
+            valid[q] = 1;
+        }
+        if (!rc) rc = photic_device_malloc((void **)&d_geo, sizeof(float) * 2 * nd * n);
+        if (!rc) rc = photic_device_malloc((void **)&d_valid, (size_t)nd * n);
+        if (!rc) rc = photic_memcpy_htod(d_geo, geo, sizeof(float) * 2 * nd * n);
+        if (!rc) rc = photic_memcpy_htod(d_valid, valid, (size_t)nd * n);
+    }
     if (!rc) {
         photic_outputs o;
         memset(&o, 0, sizeof(o));
@@ -71,7 +92,11 @@ static int sm_run_group(const photic_cfg *cfg, sm_group *g, int ncols, float **d
         o.iod = d_out + 4 * n;
         o.bottom_type = d_out + 5 * n;
         o.b_pct = d_out + 6 * n;
-        rc = photic_invert_h(cfg, n, d_rrs, NULL, NULL, NULL, d_h, &o, NULL);
+        if (nd == 1)
+            rc = photic_invert_h(cfg, n, d_rrs, NULL, NULL, NULL, d_h, &o, NULL);
+        else
+            rc = photic_invert_stack_h(cfg, n, nd, d_rrs, d_geo, d_geo + (long)nd * n, d_valid,
+                                       d_h, &o, NULL);
         if (!rc) rc = photic_synchronize();
     }
     float *h = NULL;
@@ -96,9 +121,13 @@ static int sm_run_group(const photic_cfg *cfg, sm_group *g, int ncols, float **d
         }
     }
     free(h);
+    free(geo);
+    free(valid);
     photic_device_free(d_rrs);
     if (d_h) photic_device_free(d_h);
     if (d_out) photic_device_free(d_out);
+    if (d_geo) photic_device_free(d_geo);
+    if (d_valid) photic_device_free(d_valid);
     return rc;
 }
 
@@ -128,13 +157,23 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
             bottom_type[i][j] = -9999.0f;
             index_optical_depth[i][j] = 0.0f;
         }
-    if (nscenes != 1 || n_spatial > 1 || nb != 4 || n_bottoms < 1 || n_bottoms > 3) {
-        fprintf(stderr, "photic samodel: supported are one scene, one region (n_spatial <= 1), "
-                        "4 bands and 1..3 bottoms (got %d scenes, n_spatial %d, %d bands, %d "
-                        "bottoms); outputs left at their defaults\n",
-                nscenes, n_spatial, nb, n_bottoms);
+    int same_bands = 1;
+    for (int k = 1; k < nscenes && same_bands; k++) {
+        const struct photic_scene *t = &scene_data[scene_indexes[k]];
+        if (t->n_bands != nb || t->nrows != nrows || t->ncols != ncols) same_bands = 0;
+        for (int b = 0; same_bands && b < nb && b < 4; b++)
+            if (t->wavelengths[b] != sc->wavelengths[b]) same_bands = 0;
+    }
+    if (nscenes > 8 || n_spatial > 1 || nb != 4 || n_bottoms < 1 || n_bottoms > 3 ||
+        !same_bands) {
+        fprintf(stderr, "photic samodel: supported are 1..8 scenes of one grid with the same "
+                        "4 bands, one region (n_spatial <= 1) and 1..3 bottoms (got %d scenes, "
+                        "n_spatial %d, %d bands, %d bottoms%s); outputs left at their "
+                        "defaults\n",
+                nscenes, n_spatial, nb, n_bottoms, same_bands ? "" : ", differing bands");
         return PHOTIC_EUNSUPPORTED;
     }
+    const int nd = nscenes;
     const int r = n_smoothing_radius < 1 ? 1 : n_smoothing_radius;
 
     double wl[4];
@@ -144,14 +183,19 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     cfg.mode = PHOTIC_MODE_NATIVE;
     cfg.n_bottoms = n_bottoms;
     cfg.nodata = -1.0f; /* screened below: any no-data band arrives negative */
-    {
-        double t = sc->theta_w;
-        t *= 3.141592653589793 / 180.0;
-        cfg.sec_sun = 1.0 / cos(t);
-        t = sc->theta_v;
-        t *= 3.141592653589793 / 180.0;
-        cfg.sec_view = 1.0 / cos(t);
+    /* each scene's subsurface sun and view angles, as secants (model/samodel.c:534-548) */
+    double sec_sun[8], sec_view[8];
+    for (int k = 0; k < nd; k++) {
+        const struct photic_scene *t = &scene_data[scene_indexes[k]];
+        double a = t->theta_w;
+        a *= 3.141592653589793 / 180.0;
+        sec_sun[k] = 1.0 / cos(a);
+        a = t->theta_v;
+        a *= 3.141592653589793 / 180.0;
+        sec_view[k] = 1.0 / cos(a);
     }
+    cfg.sec_sun = sec_sun[0];
+    cfg.sec_view = sec_view[0];
 
     /* groups: 0 = no empirical depth (all depth starts), 1 = empirical depth <= 8 m, 2 =
      * empirical depth > 8 m (sand only), model/samodel.c:1826, :2208-2231, :960-976 */
@@ -161,22 +205,27 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     int rc = PHOTIC_OK;
     for (int k = 0; k < 3 && !rc; k++) {
         gr[k].idx = (long *)malloc(sizeof(long) * (size_t)(npix ? npix : 1));
-        gr[k].rrs = (float *)malloc(sizeof(float) * 4 * (size_t)(npix ? npix : 1));
+        gr[k].rrs = (float *)malloc(sizeof(float) * 4 * nd * (size_t)(npix ? npix : 1));
         gr[k].hstart = k ? (float *)malloc(sizeof(float) * (size_t)(npix ? npix : 1)) : NULL;
         if (!gr[k].idx || !gr[k].rrs || (k && !gr[k].hstart)) rc = PHOTIC_ENOMEM;
     }
-    /* pass 1: pixel screening and gather, per group (compacted later per group) */
+    /* pass 1: pixel screening over every scene and gather, per group (rows [scene][band],
+     * compacted later per group) */
     for (int i = 0; i < nrows && !rc; i++)
         for (int j = 0; j < ncols; j++) {
             int bad = 0;
-            float v[4];
-            for (int b = 0; b < 4; b++) {
-                const struct photic_geogrid *g = &gridded_data[sc->band_indexes[b]];
-                float refl = g->array[i][j];
-                if (sm_feq(refl, g->nodata_value, 1.0e-6f) || refl < 0.0f) bad = 1;
-                float s = r > 1 ? sm_smoothed(g->array, i, j, nrows, ncols, r, g->nodata_value) : refl;
-                if (sm_feq(s, g->nodata_value, 1.0e-6f)) bad = 1; /* region dropped */
-                v[b] = s;
+            float v[8 * 4];
+            for (int d = 0; d < nd; d++) {
+                const struct photic_scene *t = &scene_data[scene_indexes[d]];
+                for (int b = 0; b < 4; b++) {
+                    const struct photic_geogrid *g = &gridded_data[t->band_indexes[b]];
+                    float refl = g->array[i][j];
+                    if (sm_feq(refl, g->nodata_value, 1.0e-6f) || refl < 0.0f) bad = 1;
+                    float s = r > 1 ? sm_smoothed(g->array, i, j, nrows, ncols, r, g->nodata_value)
+                                    : refl;
+                    if (sm_feq(s, g->nodata_value, 1.0e-6f)) bad = 1; /* region dropped */
+                    v[4 * d + b] = s;
+                }
             }
             if (bad) continue;
             int k = 0;
@@ -189,14 +238,14 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
             }
             sm_group *g = &gr[k];
             g->idx[g->n] = (long)i * ncols + j;
-            for (int b = 0; b < 4; b++) g->rrs[(long)b * npix + g->n] = v[b];
+            for (int q = 0; q < 4 * nd; q++) g->rrs[(long)q * npix + g->n] = v[q];
             if (k) g->hstart[g->n] = h;
             g->n++;
         }
-    /* compact the planar band rows to n per group */
+    /* compact the planar (scene, band) rows to n per group */
     for (int k = 0; k < 3 && !rc; k++)
-        for (int b = 1; b < 4; b++)
-            memmove(gr[k].rrs + (long)b * gr[k].n, gr[k].rrs + (long)b * npix,
+        for (int q = 1; q < 4 * nd; q++)
+            memmove(gr[k].rrs + (long)q * gr[k].n, gr[k].rrs + (long)q * npix,
                     sizeof(float) * (size_t)gr[k].n);
 
     for (int k = 0; k < 3 && !rc; k++) {
@@ -205,7 +254,8 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
             c.n_bottoms = 1;
             c.bottom_idx[0] = 0; /* sand */
         }
-        rc = sm_run_group(&c, &gr[k], ncols, depth, model_error, bottom_albedo, bottom_sand,
+        rc = sm_run_group(&c, &gr[k], nd, sec_sun, sec_view, ncols, depth, model_error,
+                          bottom_albedo, bottom_sand,
                           bottom_seagrass, bottom_coral, K_min, bottom_type,
                           index_optical_depth);
     }

@@ -117,3 +117,88 @@ def test_samodel_rejects_unsupported():
     rc, out = _call(bands, -9999.0, 30.0, 4)
     assert rc == P.EUNSUPPORTED
     assert np.all(out["btype"] == -9999.0)
+
+
+def _call_scenes(bands, theta_w, theta_v, n_bottoms, nodata=-9999.0, emp=None,
+                 emp_nodata=-99999.0):
+    """samodel() over several scenes: bands [nd][4][nrows][ncols], one grid per (scene, band)."""
+    nd, nb, nrows, ncols = bands.shape
+    grids = [RowGrid(bands[d, b]) for d in range(nd) for b in range(nb)]
+    gd = (Geogrid * (nd * nb))()
+    for q in range(nd * nb):
+        gd[q].nrows, gd[q].ncols, gd[q].nodata_value = nrows, ncols, nodata
+        gd[q].array = grids[q].pp
+    scenes = (Scene * nd)()
+    for d in range(nd):
+        sc = scenes[d]
+        sc.n_bands, sc.nrows, sc.ncols = nb, nrows, ncols
+        for b, wl in enumerate((443, 482, 561, 655)):
+            sc.band_indexes[b] = d * nb + b
+            sc.wavelengths[b] = wl
+        sc.theta_w, sc.theta_v = theta_w[d], theta_v[d]
+    idx = (C.c_int * nd)(*range(nd))
+    empg = Geogrid()
+    keep = None
+    if emp is not None:
+        keep = RowGrid(emp)
+        empg.nrows, empg.ncols, empg.nodata_value, empg.array = nrows, ncols, emp_nodata, keep.pp
+    outs = {k: RowGrid(np.zeros((nrows, ncols))) for k in
+            ("depth", "sigma", "err", "alb", "sand", "grass", "coral", "kmin", "btype", "iod")}
+    P.lib.photic_samodel.restype = C.c_int
+    rc = P.lib.photic_samodel(scenes, gd, idx, nd, int(emp is not None), empg, 1, 1,
+                              n_bottoms, *[outs[k].pp for k in ("depth", "sigma", "err", "alb",
+                                                                 "sand", "grass", "coral", "kmin",
+                                                                 "btype", "iod")])
+    return rc, {k: v.array() for k, v in outs.items()}
+
+
+@pytest.mark.parametrize("nbot,with_emp", [(2, False), (1, True)])
+def test_samodel_dropin_several_scenes(nbot, with_emp):
+    """samodel() with three scenes: the reference's N_s = 3 run in its native layout, each
+    scene with its own sun and view angles, a pixel unmodelled when any scene has no data,
+    optionally an empirical depth grid (its own depth start, sand only beyond 8 m) -- against
+    the oracle's native multi-date inversion, pixel by pixel."""
+    nd, nrows, ncols = 3, 16, 24
+    n = nrows * ncols
+    rrs, _, _, _ = O.gen_stack(71 + nbot, n, n_dates=nd, p_cloud=0.0, k_min=nd)
+    bands = rrs.reshape(nd, 4, nrows, ncols).copy()
+    bands[1, :, 2, 3] = -9999.0        # no data on one scene
+    bands[2, 1, 4, 5] = -0.0004        # negative on one band of one scene
+    tw, tv = (30.0, 38.0, 45.0), (0.0, 5.0, 10.0)
+    emp = None
+    if with_emp:
+        emp = np.random.default_rng(5).uniform(-15.0, -0.5, (nrows, ncols)).astype(np.float32)
+        emp[0, :6] = -99999.0          # no empirical depth here: all depth starts
+    rc, out = _call_scenes(bands, tw, tv, nbot, emp=emp)
+    assert rc == 0
+    ss = np.repeat((1.0 / np.cos(np.radians(tw)))[:, None], n, 1).astype(np.float32)
+    sv = np.repeat((1.0 / np.cos(np.radians(tv)))[:, None], n, 1).astype(np.float32)
+    r = bands.reshape(nd, 4, n)
+    valid = np.ones((nd, n), np.uint8)
+    o_depth = np.zeros(n, np.float32)
+    o_err = np.zeros(n, np.float32)
+    o_status = np.full(n, -1, np.int32)
+    groups = [(np.arange(n), None, nbot)]
+    if with_emp:
+        e = emp.reshape(-1)
+        has = e != -99999.0
+        h = np.where(e > -1.0, 1.0, np.abs(e)).astype(np.float32)
+        groups = [(np.nonzero(~has)[0], None, nbot), (np.nonzero(has & (h <= 8.0))[0], h, nbot),
+                  (np.nonzero(has & (h > 8.0))[0], h, 1)]
+    for idx, h, nb_ in groups:
+        if len(idx) == 0:
+            continue
+        o = O.invert_stack_native(r[:, :, idx], ss[:, idx], valid[:, idx], n_bottoms=nb_,
+                                  sec_view=sv[:, idx], hstart_px=None if h is None else h[idx])
+        o_depth[idx], o_err[idx], o_status[idx] = o["depth"], o["rrs_error"], o["status"]
+    depth = -out["depth"].reshape(-1)
+    ok = o_status >= 0
+    assert not ok[2 * ncols + 3] and not ok[4 * ncols + 5]
+    assert np.array_equal(out["btype"].reshape(-1) != -9999.0, ok)
+    within = np.abs(depth[ok] - o_depth[ok]) < 0.01
+    err = out["err"].reshape(-1)[ok]
+    flat = np.abs(err - o_err[ok]) <= 1e-6 * np.abs(o_err[ok])
+    print("%d scenes, %d bottoms%s: %d pixels, within 1 cm %.4f, identical %.4f" % (
+        nd, nbot, " + empirical depths" if with_emp else "", ok.sum(), within.mean(),
+        (depth[ok] == o_depth[ok]).mean()))
+    assert (within | flat).mean() > 0.995 and within.mean() > 0.98

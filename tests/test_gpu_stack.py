@@ -85,9 +85,14 @@ def test_stack_empty_and_unsupported():
     out = P.invert_stack(_cfg(), np.zeros((3, 4, 0), np.float32), np.zeros((3, 0), np.float32),
                          np.zeros((3, 0), np.uint8))
     assert all(v.size == 0 for v in out.values())
-    with pytest.raises(P.PhoticError):
-        P.invert_stack(_cfg(mode=P.MODE_NATIVE, n_bottoms=2), np.zeros((2, 4, 8), np.float32),
-                       np.ones((2, 8), np.float32), np.ones((2, 8), np.uint8))
+    z = (np.zeros((2, 4, 8), np.float32), np.ones((2, 8), np.float32), np.ones((2, 8), np.uint8))
+    with pytest.raises(P.PhoticError):   # seeds are made for the mix layout
+        P.invert_stack(_cfg(mode=P.MODE_NATIVE, n_bottoms=2), *z, seed=np.zeros((7, 8), np.float32))
+    with pytest.raises(P.PhoticError):   # 4 native bottoms are not compiled
+        P.invert_stack(_cfg(mode=P.MODE_NATIVE, n_bottoms=4), *z)
+    with pytest.raises(P.PhoticError):   # more than 8 dates
+        P.invert_stack(_cfg(), np.zeros((9, 4, 8), np.float32), np.ones((9, 8), np.float32),
+                       np.ones((9, 8), np.uint8))
 
 
 def test_stack_mlp_seeded():
@@ -149,3 +154,56 @@ def test_stack_stream_matches_whole_scene(seeded):
                                 tile_rows=1000, net=net, fields=("depth", "status"))
     assert np.array_equal(one["depth"], whole["depth"])
     assert np.array_equal(one["status"], whole["status"])
+
+
+@pytest.mark.parametrize("nbot", [0, 1, 2, 3])
+def test_one_date_stack_is_the_single_date_inversion(nbot):
+    """A stack of one date is the single-date problem, in the mix layout and with 1-3 native
+    bottoms: the group engine (8-lane groups) and the single-date kernel (one pixel per lane)
+    give identical results."""
+    wl = (443.0, 482.0, 561.0, 655.0)
+    kw = dict(mode=P.MODE_NATIVE, n_bottoms=nbot) if nbot else {}
+    cfg = P.cfg_default(wl, reqmin=1e-10, konvge=100, kcount=2000, **kw)
+    rrs, ss, truth, nodata, _ = O.gen(23 + nbot, 1024)
+    g1 = P.invert(cfg, rrs, ss)
+    gs = P.invert_stack(cfg, rrs[None], ss[None], np.ones((1, rrs.shape[1]), np.uint8))
+    for k in ("status", "x", "n_evals", "n_iter", "n_starts", "depth", "rrs_error",
+              "model_error", "albedo", "k_min", "iod", "b_pct", "bottom_type"):
+        assert np.array_equal(gs[k], g1[k]), k
+
+
+@pytest.mark.parametrize("nbot,own_h", [(2, False), (3, True), (1, False)])
+def test_native_stack_against_oracle(nbot, own_h):
+    """Native-layout stacks (the reference's own vector over the valid dates, per-date view
+    angles, optionally one depth start per pixel) against the oracle's native multi-date
+    inversion with the reference's Nelder-Mead settings: status identical, every converged
+    stack within the bar (or with the same misfit where it is flat, or shown to differ by
+    the host C library's rounding alone on at most 0.5 % of them), at least 98 %
+    bit-identical."""
+    nd = 4
+    rrs, ss, valid, truth = O.gen_stack(61 + nbot, 384, n_dates=nd, k_min=2)
+    rng = np.random.default_rng(nbot)
+    sv = (1.0 / np.cos(np.radians(rng.uniform(0.0, 20.0, (nd, 1)))) * np.ones((1, rrs.shape[2]))).astype(np.float32)
+    h = rng.uniform(1.0, 20.0, rrs.shape[2]).astype(np.float32) if own_h else None
+    cfg = P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=nbot, reqmin=1e-2, konvge=100,
+                        kcount=5000, check_watchdog=1)
+    g = P.invert_stack(cfg, rrs, ss, valid, sec_view=sv, hstart=h)
+    o = O.invert_stack_native(rrs, ss, valid, n_bottoms=nbot, sec_view=sv, hstart_px=h)
+    assert np.array_equal(g["status"], o["status"])
+    v = o["status"] >= 0
+    both = (g["status"] == 0) & (o["status"] == 0)
+    dH = np.abs(g["x"][0] - o["x"][0])
+    rel = np.abs(g["x"][1:] - o["x"][1:]) / np.maximum(np.abs(o["x"][1:]), 1e-300)
+    within = (dH < 0.01) & np.all((rel < 1e-4) | (o["x"][1:] == 0), axis=0)
+    flat = np.abs(g["rrs_error"] - o["rrs_error"]) <= 1e-6 * np.abs(o["rrs_error"])
+    exact = np.all(g["x"] == o["x"], axis=0)
+    bad = both & ~(within | flat)
+    print("native %d bottoms%s: %d valid, %d converged, within %.4f, bit-identical %.4f, "
+          "outside %d" % (nbot, " (own depth starts)" if own_h else "", v.sum(), both.sum(),
+                          within[both].mean(), exact[v].mean(), bad.sum()))
+    assert bad.sum() <= 0.005 * both.sum()
+    assert exact[v].mean() > 0.98
+    sel = v & exact
+    for k in ("depth", "rrs_error", "k_min", "iod", "albedo", "bottom_type"):
+        assert np.allclose(g[k][sel], o[k][sel], rtol=1e-6, atol=1e-7), k
+    assert np.allclose(g["b_pct"][:, sel], o["b_pct"][:, sel], rtol=1e-6, atol=1e-6)
