@@ -244,6 +244,20 @@ __device__ PH_INLINE double ph_band_fast(double P, double lnP, double G, double 
 // objective: model/samodel.c:2432-2759 with one region
 // ---------------------------------------------------------------------------------------
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// The launch constants of a kernel whose first argument starts with its PhConst, read through
+// the kernel-argument segment pointer behind an empty asm the compiler cannot see through:
+// loads through it stay where they are used (scalar loads from the segment) instead of being
+// hoisted out of the persistent loop into scalar registers that then spill into vector lanes.
+__device__ __attribute__((always_inline)) inline const PhConst& ph_kernarg_const()
+{
+    typedef const __attribute__((address_space(4))) PhConst* PhConstP4;
+    PhConstP4 p = (PhConstP4)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const PhConst*)p;
+}
+#endif
+
 template <int NBOT>
 struct PhLayout {
     static constexpr int nbot = NBOT == 0 ? 2 : NBOT;
@@ -252,7 +266,9 @@ struct PhLayout {
     static constexpr int n() { return off + 3 * ND; }
 };
 
-template <typename FT, int ND, int NB, int NBOT, bool SIDE>
+// KARG: c is the kernel's own launch constants (ph_kernarg_const); each band then reads its
+// coefficients afresh from the kernel-argument segment.
+template <typename FT, int ND, int NB, int NBOT, bool SIDE, bool KARG = false>
 PH_INLINE double ph_objective(const double* x, const PhConst& c,
                                                const PhPixel<FT, ND, NB>& px,
                                                PhSide<ND, NB>* side)
@@ -296,15 +312,20 @@ PH_INLINE double ph_objective(const double* x, const PhConst& c,
         if constexpr (sizeof(FT) == 8) {
 #pragma unroll
             for (int b = 0; b < NB; b++) {
+#if defined(__HIP_DEVICE_COMPILE__)
+                const PhConst& cb = KARG ? ph_kernarg_const() : c;
+#else
+                const PhConst& cb = c;
+#endif
                 double rho = 0.0;
 #pragma unroll
                 for (int k = 0; k < nbot; k++) {
-                    int t = NBOT == 0 ? k : c.bottom_idx[k];
-                    rho += qb[k] * c.bot[t][b];
+                    int t = NBOT == 0 ? k : cb.bottom_idx[k];
+                    rho += qb[k] * cb.bot[t][b];
                 }
                 double K, rB, r;
-                double m = ph_band64(P, G, X, rho, H, c.a0[b], c.a1[b], c.aw[b], c.bbw[b],
-                                     c.eg[b], px.pw[s][b], px.ss[s], px.sv[s], &K, &rB, &r);
+                double m = ph_band64(P, G, X, rho, H, cb.a0[b], cb.a1[b], cb.aw[b], cb.bbw[b],
+                                     cb.eg[b], px.pw[s][b], px.ss[s], px.sv[s], &K, &rB, &r);
                 double d = m - px.rrs[s][b];
                 err += d * d;
                 Kd[s][b] = K;

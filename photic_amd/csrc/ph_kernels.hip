@@ -117,6 +117,10 @@ __device__ inline void ph_write_nodata(const PhInvertArgs& a, long pix, int n_pa
     if (a.n_starts) a.n_starts[pix] = 0;
 }
 
+// The objective reads the band coefficients through the kernel-argument segment pointer
+// (ph_kernarg_const): the launch constants must open the kernel's only argument.
+static_assert(offsetof(PhInvertArgs, c) == 0, "PhConst must start the kernel arguments");
+
 // Waves per SIMD the kernel is compiled for: two for the five-parameter mix (its lane state
 // fits 256 registers and its simplexes 80 KB of LDS per block, so two blocks share a CU), one
 // for the native layouts, whose larger simplexes fill the LDS with one block.
@@ -225,7 +229,8 @@ __global__ void __launch_bounds__(256, ph_invert_waves<NBOT>()) ph_invert_kernel
         {
             PhSide<1, NB> side;
             double f = 0.0;
-            if (S.phase != PH_IDLE) f = ph_objective<FT, 1, NB, NBOT, true>(S.xe, c, px, &side);
+            if (S.phase != PH_IDLE)
+                f = ph_objective<FT, 1, NB, NBOT, true, true>(S.xe, c, px, &side);
             if (S.phase != PH_IDLE) {
                 ph_nm_consume(S, c, f);
                 if (S.phase == PH_DONE) {
