@@ -45,6 +45,9 @@ def lib():
         _lib.orc_py_band_tables.argtypes = [v, i, v, v]
         _lib.orc_py_start_step.argtypes = [i, i, v, i, v, d, v, v]
         _lib.orc_py_objective_stack.argtypes = [v, i, i, v, v, v, v]
+        _lib.orc_py_objective_stack_ex.restype = d
+        _lib.orc_py_objective_stack_ex.argtypes = [v, i, i, i, i, v, v, v, v]
+        _lib.orc_py_start_step_stack.argtypes = [i, i, v, i, i, v, d, v, v]
         _lib.orc_py_objective_stack.restype = d
         _lib.orc_py_gen_stack.argtypes = [C.c_uint64, l, l, v, i, d, i, d, i, v, v, v, v]
         _lib.orc_py_invert_stack.restype = l
@@ -353,14 +356,30 @@ def hostsim_invert_stack(cfg_const, rrs, sec_sun, valid, seed=None, math="libm")
     return out
 
 
-def objective_stack(x, rrs, sec_sun, wl=WL4):
-    """Objective and E_Rrs of one mix-layout vector x [2 + 3k] over k dates (rrs [k][nb])."""
+def objective_stack(x, rrs, sec_sun, wl=WL4, n_bottoms=None):
+    """Objective and E_Rrs of one vector x over k dates (rrs [k][nb]): the mix layout
+    [2 + 3k], or the native one [1 + 2 n_bottoms + 3k] when n_bottoms is given."""
     x = np.ascontiguousarray(x, np.float64)
     r = np.ascontiguousarray(rrs, np.float32)
     s = np.ascontiguousarray(sec_sun, np.float32)
     k, nb = r.shape
     e = C.c_double()
     w = np.asarray(wl, np.float64)
-    f = lib().orc_py_objective_stack(w.ctypes.data, nb, k, x.ctypes.data, r.ctypes.data,
-                                     s.ctypes.data, C.byref(e))
+    if n_bottoms is None:
+        f = lib().orc_py_objective_stack(w.ctypes.data, nb, k, x.ctypes.data, r.ctypes.data,
+                                         s.ctypes.data, C.byref(e))
+    else:
+        f = lib().orc_py_objective_stack_ex(w.ctypes.data, nb, k, 0, n_bottoms, x.ctypes.data,
+                                            r.ctypes.data, s.ctypes.data, C.byref(e))
     return f, e.value
+
+
+def start_step_stack(rrs, h0, mode=1, n_bottoms=2, wl=WL4):
+    """Start point and steps of one depth start for a k-date pixel (rrs [k][nb])."""
+    r = np.ascontiguousarray(rrs, np.float64)
+    k, nb = r.shape
+    w = np.asarray(wl, np.float64)
+    start, step = np.zeros(64), np.zeros(64)
+    n = lib().orc_py_start_step_stack(mode, n_bottoms, w.ctypes.data, nb, k, r.ctypes.data, h0,
+                                      start.ctypes.data, step.ctypes.data)
+    return start[:n], step[:n]

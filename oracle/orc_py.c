@@ -426,8 +426,10 @@ long orc_py_invert_stack(const double *wl, int nb, int n_dates, long n, const fl
 
 /* Objective of one mix-layout vector over k dates: x [2 + 3k], rrs [k][nb], sec_sun [k];
  * returns the objective and E_Rrs in *rrs_error. */
-double orc_py_objective_stack(const double *wl, int nb, int k, const double *x,
-                              const float *rrs, const float *sec_sun, double *rrs_error)
+/* The objective of one k-date vector x (mode 1 mix, mode 0 native with n_bottoms bottoms). */
+double orc_py_objective_stack_ex(const double *wl, int nb, int k, int mode, int n_bottoms,
+                                 const double *x, const float *rrs, const float *sec_sun,
+                                 double *rrs_error)
 {
     orc_bands bands;
     orc_band_tables(wl, nb, &bands);
@@ -439,13 +441,34 @@ double orc_py_objective_stack(const double *wl, int nb, int k, const double *x,
         sv[s] = 1.0;
     }
     orc_pixel px;
-    orc_pixel_init(&px, ORC_MODE_MIX, 2, NULL, k, &bands, ss, sv, r);
-    for (int i = 0; i < 2 + 3 * k; i++)
+    orc_pixel_init(&px, mode, n_bottoms, NULL, k, &bands, ss, sv, r);
+    for (int i = 0; i < orc_n_params(&px); i++)
         xv[i] = x[i];
     double f = orc_objective(xv, &px);
     if (rrs_error)
         *rrs_error = px.Rrs_error;
     return f;
+}
+
+double orc_py_objective_stack(const double *wl, int nb, int k, const double *x,
+                              const float *rrs, const float *sec_sun, double *rrs_error)
+{
+    return orc_py_objective_stack_ex(wl, nb, k, ORC_MODE_MIX, 2, x, rrs, sec_sun, rrs_error);
+}
+
+/* Start point and steps of one depth start for a k-date pixel (rrs [k][nb]). */
+int orc_py_start_step_stack(int mode, int n_bottoms, const double *wl, int nb, int k,
+                            const double *rrs, double h0, double *start, double *step)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    orc_pixel px;
+    double ss[ORC_MAXD], sv[ORC_MAXD];
+    for (int s = 0; s < k; s++)
+        ss[s] = sv[s] = 1.0;
+    orc_pixel_init(&px, mode, n_bottoms, NULL, k, &bands, ss, sv, rrs);
+    orc_start_step(&px, h0, start, step);
+    return orc_n_params(&px);
 }
 
 /* The no-data disc mask at the pixel indices idx[m] (row * n_cols + col). */

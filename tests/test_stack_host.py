@@ -76,6 +76,32 @@ def test_stack_of_identical_dates_is_the_single_date_problem():
             assert np.isclose(ok, o1[0], rtol=1e-13)
 
 
+@pytest.mark.parametrize("nbot", [1, 2, 3])
+def test_native_stack_of_identical_dates_is_the_single_date_problem(nbot):
+    """The native layout [H, B_k, q_k, (P, G, X) per date]: k copies of one date give the
+    single-date misfit at the same parameters (no K penalty in play), and the same start point
+    -- amplitudes 100 * 4 * mean Rrs(490) over the dates, weights 1, each date's P, G, X from
+    its own Rrs -- as the single date (model/samodel.c:2249-2354, 2545-2579)."""
+    rrs1, ss1, _, _, _ = O.gen(19 + nbot, 6)
+    bq = [30.0 + 10 * j for j in range(nbot)] + [1.0 + 0.5 * j for j in range(nbot)]
+    x1 = np.array([8.0] + bq + [3.0, 2.0, 1.0])
+    for k in (2, 3):
+        xk = np.concatenate([[8.0], bq, np.tile([3.0, 2.0, 1.0], k)])
+        for p in range(6):
+            r = np.repeat(rrs1[:, p][None, :], k, axis=0)
+            o1, e1 = O.objective_stack(x1, rrs1[:, p][None, :], ss1[p:p + 1], n_bottoms=nbot)
+            ok, ek = O.objective_stack(xk, r, np.repeat(ss1[p:p + 1], k), n_bottoms=nbot)
+            assert np.isclose(ek, e1, rtol=1e-13) and np.isclose(ok, o1, rtol=1e-13)
+            s1, st1 = O.start_step_stack(rrs1[:, p][None, :].astype(np.float64), 15.0, mode=0,
+                                         n_bottoms=nbot)
+            sk, stk = O.start_step_stack(r.astype(np.float64), 15.0, mode=0, n_bottoms=nbot)
+            off = 1 + 2 * nbot
+            assert np.allclose(sk[:off], s1[:off], rtol=1e-15, atol=0)
+            assert np.allclose(stk[:off], st1[:off], rtol=1e-15, atol=0)
+            assert np.array_equal(sk[off:], np.tile(s1[off:], k))
+            assert np.array_equal(stk[off:], np.tile(st1[off:], k))
+
+
 def test_stack_generator_draws():
     rrs, ss, valid, truth = O.gen_stack(11, 4000, n_dates=8, p_cloud=0.3, k_min=3)
     k = valid.sum(0)
