@@ -70,6 +70,17 @@ def lib():
         _lib.orc_py_mlp_nparam.restype = i
         _lib.orc_py_set_threads.restype = i
         _lib.orc_py_set_threads.argtypes = [i]
+        _lib.orc_py_invert_regions.restype = l
+        _lib.orc_py_invert_regions.argtypes = [v, i, i, i, i, v, v, v, v, i, i, v, d, i, i, i, v,
+                                               v, l, v, i, v, v, C.POINTER(PyOut)]
+        _lib.orc_py_region_objective.restype = d
+        _lib.orc_py_region_objective.argtypes = [v, i, i, i, i, v, v, v, i, v, v, v]
+        _lib.orc_py_region_start_step.restype = i
+        _lib.orc_py_region_start_step.argtypes = [v, i, i, i, v, i, d, v, v]
+        _lib.orc_py_region_optimise.restype = i
+        _lib.orc_py_region_optimise.argtypes = [v, i, i, i, i, v, v, v, i, v, d, i, i, i, v, v, v]
+        _lib.orc_py_region_gather.restype = i
+        _lib.orc_py_region_gather.argtypes = [v, v, i, i, i, i, i, i, i, v, v]
     return _lib
 
 
@@ -383,3 +394,112 @@ def start_step_stack(rrs, h0, mode=1, n_bottoms=2, wl=WL4):
     n = lib().orc_py_start_step_stack(mode, n_bottoms, w.ctypes.data, nb, k, r.ctypes.data, h0,
                                       start.ctypes.data, step.ctypes.data)
     return start[:n], step[:n]
+
+
+HSTART8 = (40.0, 30.0, 20.0, 15.0, 10.0, 7.5, 2.5, 1.0)
+
+
+def _region_out(npar, n):
+    return {
+        "x": np.zeros((npar, n)), "depth": np.zeros(n, np.float32),
+        "rrs_error": np.zeros(n, np.float32), "model_error": np.zeros(n, np.float32),
+        "k_min": np.zeros(n, np.float32), "iod": np.zeros(n, np.float32),
+        "albedo": np.zeros(n, np.float32), "bottom_type": np.zeros(n, np.float32),
+        "b_pct": np.zeros((3, n), np.float32), "n_iter": np.zeros(n, np.int32),
+        "converged": np.zeros(n, np.int32), "n_evals": np.zeros(n, np.int32),
+        "n_starts": np.zeros(n, np.int32), "status": np.zeros(n, np.int32),
+        "misfit": np.zeros(n),
+    }
+
+
+def invert_regions(rrs, n_rows, n_cols, idx, nodata, sec_sun, sec_view, n_spatial=2,
+                   n_bottoms=3, bottom_idx=(0, 1, 2), hstart_px=None, reqmin=1e-2, konvge=100,
+                   kcount=5000, hstart=HSTART8, wl=WL4):
+    """Oracle inversion of the pixels idx (row * n_cols + col) of a scene rrs [nd][nb][n_rows *
+    n_cols] with their (2 n_spatial - 1)^2 regions; nodata [nd][nb]; sec_sun, sec_view [nd].
+    x is [R_max (1 + 2 n_bottoms) + 3 nd][n] in the reference's layout, zero-padded."""
+    rrs = np.ascontiguousarray(rrs, np.float32)
+    nd, nb, npix = rrs.shape
+    assert npix == n_rows * n_cols
+    ix = np.ascontiguousarray(idx, np.int64)
+    n = len(ix)
+    rmax = (2 * max(n_spatial, 1) - 1) ** 2
+    npar = rmax * (1 + 2 * n_bottoms) + 3 * nd
+    out = _region_out(npar, n)
+    out["n_regions"] = np.zeros(n, np.int32)
+    out["origin"] = np.zeros(n, np.int32)
+    o = PyOut(**{k: v.ctypes.data for k, v in out.items() if k not in ("n_regions", "origin")})
+    w = np.asarray(wl, np.float64)
+    hs = np.asarray(hstart, np.float64)
+    bi = np.asarray(bottom_idx, np.int32)
+    nod = np.ascontiguousarray(np.broadcast_to(np.asarray(nodata, np.float32), (nd, nb)))
+    ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (nd,)))
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (nd,)))
+    hp = None if hstart_px is None else np.ascontiguousarray(hstart_px, np.float32)
+    lib().orc_py_invert_regions(w.ctypes.data, nb, nd, n_rows, n_cols, rrs.ctypes.data,
+                                nod.ctypes.data, ss.ctypes.data, sv.ctypes.data, n_spatial,
+                                n_bottoms, bi.ctypes.data, reqmin, konvge, kcount, len(hs),
+                                hs.ctypes.data, _p(hp), n, ix.ctypes.data, npar,
+                                out["n_regions"].ctypes.data, out["origin"].ctypes.data,
+                                C.byref(o))
+    return out
+
+
+def region_objective(x, rrs, origin=0, sec_sun=1.0, sec_view=1.0, n_bottoms=3,
+                     bottom_idx=(0, 1, 2), wl=WL4):
+    """Region objective of one vector x for region data rrs [R][nd][nb]; returns (E, terms)
+    with terms = E_Rrs, depth penalty, bottom penalty, K penalty, albedo, model error."""
+    r = np.ascontiguousarray(rrs, np.float64)
+    R, nd, nb = r.shape
+    xv = np.ascontiguousarray(x, np.float64)
+    w = np.asarray(wl, np.float64)
+    bi = np.asarray(bottom_idx, np.int32)
+    ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (nd,)))
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (nd,)))
+    terms = np.zeros(6)
+    f = lib().orc_py_region_objective(w.ctypes.data, nb, nd, R, origin, r.ctypes.data,
+                                      ss.ctypes.data, sv.ctypes.data, n_bottoms, bi.ctypes.data,
+                                      xv.ctypes.data, terms.ctypes.data)
+    return f, terms
+
+
+def region_start_step(rrs, h0, n_bottoms=3, wl=WL4):
+    """Start point and steps of one depth start for region data rrs [R][nd][nb]."""
+    r = np.ascontiguousarray(rrs, np.float64)
+    R, nd, nb = r.shape
+    w = np.asarray(wl, np.float64)
+    start, step = np.zeros(256), np.zeros(256)
+    n = lib().orc_py_region_start_step(w.ctypes.data, nb, nd, R, r.ctypes.data, n_bottoms, h0,
+                                       start.ctypes.data, step.ctypes.data)
+    return start[:n], step[:n]
+
+
+def region_optimise(rrs, origin=0, sec_sun=1.0, sec_view=1.0, n_bottoms=3, bottom_idx=(0, 1, 2),
+                    reqmin=1e-2, konvge=100, kcount=5000, hstart=HSTART8, wl=WL4):
+    """One pixel's region inversion from region data rrs [R][nd][nb]: (x, outputs) with
+    outputs = depth, E_Rrs, model error, K_min, albedo, IOD, bottom type, evaluations."""
+    r = np.ascontiguousarray(rrs, np.float64)
+    R, nd, nb = r.shape
+    w = np.asarray(wl, np.float64)
+    bi = np.asarray(bottom_idx, np.int32)
+    hs = np.asarray(hstart, np.float64)
+    ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (nd,)))
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (nd,)))
+    x, out = np.zeros(256), np.zeros(8)
+    n = lib().orc_py_region_optimise(w.ctypes.data, nb, nd, R, origin, r.ctypes.data,
+                                     ss.ctypes.data, sv.ctypes.data, n_bottoms, bi.ctypes.data,
+                                     reqmin, konvge, kcount, len(hs), hs.ctypes.data,
+                                     x.ctypes.data, out.ctypes.data)
+    return x[:n], out
+
+
+def region_gather(rrs, n_rows, n_cols, i, j, n_spatial, nodata):
+    """The regions of pixel (i, j): (rrs_regions [R][nd][nb], origin)."""
+    g = np.ascontiguousarray(rrs, np.float32)
+    nd, nb, npix = g.shape
+    nod = np.ascontiguousarray(np.broadcast_to(np.asarray(nodata, np.float32), (nd, nb)))
+    out = np.zeros((25, nd, nb))
+    org = C.c_int(0)
+    R = lib().orc_py_region_gather(g.ctypes.data, nod.ctypes.data, nd, nb, n_rows, n_cols, i, j,
+                                   n_spatial, C.byref(org), out.ctypes.data)
+    return out[:R], org.value

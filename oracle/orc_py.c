@@ -7,6 +7,7 @@
  * oracle/_ref (same calling convention).
  */
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "photic_oracle.h"
@@ -479,4 +480,186 @@ void orc_py_disc_mask(uint64_t seed, long n_rows, long n_cols, int n_discs, doub
     for (long k = 0; k < m; k++)
         mask[k] = (uint8_t)orc_in_nodata_disc(seed, idx[k] / n_cols, idx[k] % n_cols, n_rows,
                                               n_cols, n_discs, r_min, r_max);
+}
+
+/* ---- spatial regions (orc_region.c) ---- */
+
+/* Invert the pixels idx[0..n-1] (row * n_cols + col) of an n_rows x n_cols scene with their
+ * (2 n_spatial - 1)^2 neighbourhoods: rrs [nd][nb][n_rows * n_cols] (already smoothed),
+ * nodata [nd * nb] (one value per band grid), sec_sun / sec_view [nd] per scene, hstart_px [n]
+ * one depth start per pixel or NULL (the list).  Pixels with no region left get status -1.
+ * x [stride][n] (stride >= the largest parameter count) receives |best| in the reference's
+ * layout; n_regions [n] and origin [n] the gather's result.  Other outputs as orc_py_invert. */
+long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_cols,
+                           const float *rrs, const float *nodata, const double *sec_sun,
+                           const double *sec_view, int n_spatial, int n_bottoms,
+                           const int *bottom_idx, double reqmin, int konvge, int kcount,
+                           int n_hstarts, const double *hstart, const float *hstart_px,
+                           long n, const long *idx, int x_stride, int *n_regions,
+                           int *origin, orc_py_out *o)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    long done = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : done)
+    for (long p = 0; p < n; p++) {
+        orc_region_pixel *px = (orc_region_pixel *)calloc(1, sizeof(orc_region_pixel));
+        orc_region_result *res = (orc_region_result *)calloc(1, sizeof(orc_region_result));
+        const int i = (int)(idx[p] / n_cols), j = (int)(idx[p] % n_cols);
+        int R = orc_region_gather(px, rrs, nodata, nd, nb, n_rows, n_cols, i, j, n_spatial);
+        if (n_regions) n_regions[p] = R;
+        if (origin) origin[p] = px->origin;
+        if (o->x)
+            for (int k = 0; k < x_stride; k++) o->x[(long)k * n + p] = 0.0;
+        if (R == 0) {
+            if (o->status) o->status[p] = -1;
+            if (o->depth) o->depth[p] = 0.0f;
+            free(px);
+            free(res);
+            continue;
+        }
+        orc_region_init(px, n_bottoms, bottom_idx, nd, &bands, sec_sun, sec_view);
+        orc_nm_cfg cfg;
+        orc_default_cfg(&cfg, nd, reqmin, konvge, kcount);
+        cfg.n_hstarts = n_hstarts;
+        for (int h = 0; h < n_hstarts && h < 16; h++)
+            cfg.hstart[h] = hstart[h];
+        if (hstart_px) {
+            cfg.n_hstarts = 1;
+            cfg.hstart[0] = (double)hstart_px[p];
+        }
+        orc_region_optimise(px, &cfg, NULL, res);
+        if (o->x)
+            for (int k = 0; k < res->n_params && k < x_stride; k++)
+                o->x[(long)k * n + p] = res->x[k];
+        if (o->depth) o->depth[p] = (float)res->depth;
+        if (o->rrs_error) o->rrs_error[p] = (float)res->Rrs_error;
+        if (o->model_error) o->model_error[p] = (float)res->model_error;
+        if (o->k_min) o->k_min[p] = (float)res->K_min;
+        if (o->iod) o->iod[p] = (float)res->iod;
+        if (o->albedo) o->albedo[p] = (float)res->bottom_albedo;
+        if (o->bottom_type) o->bottom_type[p] = (float)res->bottom_type;
+        if (o->b_pct)
+            for (int q = 0; q < 3; q++) o->b_pct[(long)q * n + p] = (float)res->B_percent[q];
+        if (o->n_iter) o->n_iter[p] = res->n_iterations;
+        if (o->converged) o->converged[p] = res->converged;
+        if (o->n_evals) o->n_evals[p] = (int)res->n_evals;
+        if (o->n_starts) o->n_starts[p] = res->n_starts;
+        if (o->status) o->status[p] = res->converged ? 0 : 2;
+        if (o->misfit) o->misfit[p] = res->model_error;
+        free(px);
+        free(res);
+        done++;
+    }
+    return done;
+}
+
+static orc_region_pixel *orc_py_region_px(const double *wl, int nb, int nd, int R, int origin,
+                                          const double *rrs /*[R][nd][nb]*/,
+                                          const double *sec_sun, const double *sec_view,
+                                          int n_bottoms, const int *bottom_idx,
+                                          orc_bands *bands)
+{
+    orc_band_tables(wl, nb, bands);
+    orc_region_pixel *px = (orc_region_pixel *)calloc(1, sizeof(orc_region_pixel));
+    px->n_regions = R;
+    px->origin = origin;
+    for (int r = 0; r < R; r++)
+        for (int s = 0; s < nd; s++)
+            for (int b = 0; b < nb; b++)
+                px->rrs[r][s][b] = rrs[((long)r * nd + s) * nb + b];
+    orc_region_init(px, n_bottoms, bottom_idx, nd, bands, sec_sun, sec_view);
+    return px;
+}
+
+/* The region objective of one vector x for one pixel's region data rrs [R][nd][nb]; terms
+ * [6] = E_Rrs, depth penalty, bottom penalty, K penalty, albedo (the last region's), model
+ * error. */
+double orc_py_region_objective(const double *wl, int nb, int nd, int R, int origin,
+                               const double *rrs, const double *sec_sun,
+                               const double *sec_view, int n_bottoms, const int *bottom_idx,
+                               const double *x, double *terms)
+{
+    orc_bands bands;
+    orc_region_pixel *px = orc_py_region_px(wl, nb, nd, R, origin, rrs, sec_sun, sec_view,
+                                            n_bottoms, bottom_idx, &bands);
+    double xv[ORC_RMAXN];
+    for (int i = 0; i < orc_region_n_params(px); i++)
+        xv[i] = x[i];
+    double f = orc_region_objective(xv, px);
+    if (terms) {
+        terms[0] = px->Rrs_error;
+        terms[1] = px->depth_error;
+        terms[2] = px->bottom_error;
+        terms[3] = px->K_error;
+        terms[4] = px->bottom_albedo;
+        terms[5] = px->model_error;
+    }
+    free(px);
+    return f;
+}
+
+/* Start point and steps of one depth start for one pixel's region data. */
+int orc_py_region_start_step(const double *wl, int nb, int nd, int R, const double *rrs,
+                             int n_bottoms, double h0, double *start, double *step)
+{
+    orc_bands bands;
+    double ss[ORC_RMAXD], sv[ORC_RMAXD];
+    for (int s = 0; s < nd; s++)
+        ss[s] = sv[s] = 1.0;
+    orc_region_pixel *px = orc_py_region_px(wl, nb, nd, R, 0, rrs, ss, sv, n_bottoms, NULL,
+                                            &bands);
+    orc_region_start_step(px, h0, start, step);
+    int n = orc_region_n_params(px);
+    free(px);
+    return n;
+}
+
+/* One pixel's multi-start inversion from its region data; out [8] = depth, E_Rrs, model
+ * error, K_min, albedo, IOD, bottom type, evaluations; x receives |best|.  Returns n_params. */
+int orc_py_region_optimise(const double *wl, int nb, int nd, int R, int origin,
+                           const double *rrs, const double *sec_sun, const double *sec_view,
+                           int n_bottoms, const int *bottom_idx, double reqmin, int konvge,
+                           int kcount, int n_hstarts, const double *hstart, double *x,
+                           double *out)
+{
+    orc_bands bands;
+    orc_region_pixel *px = orc_py_region_px(wl, nb, nd, R, origin, rrs, sec_sun, sec_view,
+                                            n_bottoms, bottom_idx, &bands);
+    orc_nm_cfg cfg;
+    orc_default_cfg(&cfg, nd, reqmin, konvge, kcount);
+    cfg.n_hstarts = n_hstarts;
+    for (int h = 0; h < n_hstarts && h < 16; h++)
+        cfg.hstart[h] = hstart[h];
+    orc_region_result *res = (orc_region_result *)calloc(1, sizeof(orc_region_result));
+    orc_region_optimise(px, &cfg, NULL, res);
+    for (int i = 0; i < res->n_params; i++)
+        x[i] = res->x[i];
+    out[0] = res->depth;
+    out[1] = res->Rrs_error;
+    out[2] = res->model_error;
+    out[3] = res->K_min;
+    out[4] = res->bottom_albedo;
+    out[5] = res->iod;
+    out[6] = res->bottom_type;
+    out[7] = (double)res->n_evals;
+    int n = res->n_params;
+    free(res);
+    free(px);
+    return n;
+}
+
+/* The gather of pixel (i, j): returns the region count, *origin, and rrs_out [25][nd][nb]. */
+int orc_py_region_gather(const float *rrs, const float *nodata, int nd, int nb, int n_rows,
+                         int n_cols, int i, int j, int n_spatial, int *origin, double *rrs_out)
+{
+    orc_region_pixel *px = (orc_region_pixel *)calloc(1, sizeof(orc_region_pixel));
+    int R = orc_region_gather(px, rrs, nodata, nd, nb, n_rows, n_cols, i, j, n_spatial);
+    *origin = px->origin;
+    for (int r = 0; r < R; r++)
+        for (int s = 0; s < nd; s++)
+            for (int b = 0; b < nb; b++)
+                rrs_out[((long)r * nd + s) * nb + b] = px->rrs[r][s][b];
+    free(px);
+    return R;
 }

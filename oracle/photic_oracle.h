@@ -28,6 +28,9 @@
  *                            nelmin plugged in it gives the same results as with orc_nelmin.
  *   - generator              Philox4x32-10 known-answer vectors (Salmon et al. 2011).
  *   - MLP                    finite differences on the FP64 forward.
+ *   - orc_region_*           with one region, bit-identical to orc_objective, orc_start_step
+ *                            and orc_optimise; continuity penalties against hand-computed
+ *                            values; the gather on hand-built grids (tests/test_oracle_region.py).
  * samodel.c itself cannot be built in this image (it needs netcdf.h and cpgplot.h), so the
  * forward model and objective are pinned by the closed forms above, not by linking.
  */
@@ -177,6 +180,49 @@ long orc_invert_scene(const orc_scene_cfg *cfg, long n_pix, const float *rrs,
                       const float *sec_sun, const uint8_t *valid, int threads,
                       float *depth, float *rrs_error, float *model_error, int *status,
                       int *n_evals, double *x_out, int x_stride);
+
+/* --- spatial regions (n_spatial > 1), orc_region.c --- */
+#define ORC_MAXR 25    /* regions: (2 n_spatial - 1)^2 for n_spatial <= 3 */
+#define ORC_RMAXD 8    /* scenes */
+#define ORC_RMAXBOT 3  /* bottoms */
+#define ORC_RMAXN (ORC_MAXR * (1 + 2 * ORC_RMAXBOT) + 3 * ORC_RMAXD)
+
+/* One pixel's region problem (model_data with N_r regions, model/common.h:116-188). */
+typedef struct {
+    int n_regions, origin;
+    int n_bottoms, bottom_idx[ORC_RMAXBOT];
+    int n_dates;
+    const orc_bands *bands;
+    double sec_sun[ORC_RMAXD], sec_view[ORC_RMAXD];
+    double rrs[ORC_MAXR][ORC_RMAXD][ORC_MAXB]; /* measured, [region][scene][band] */
+    double r440[ORC_MAXR][ORC_RMAXD], r490[ORC_MAXR][ORC_RMAXD], r550[ORC_MAXR][ORC_RMAXD];
+    double r640[ORC_MAXR][ORC_RMAXD], r750[ORC_MAXR][ORC_RMAXD];
+    /* side effects of the last evaluation */
+    double K[ORC_RMAXD][ORC_MAXB];             /* the last region's, as the reference keeps it */
+    double rrs_mod[ORC_MAXR][ORC_RMAXD][ORC_MAXB], rrs_bot[ORC_MAXR][ORC_RMAXD][ORC_MAXB];
+    double bottom_albedo, Rrs_error, depth_error, bottom_error, K_error, model_error;
+    long n_evals;
+} orc_region_pixel;
+
+typedef struct {
+    double x[ORC_RMAXN]; /* |best|: [H_r], [B_{r,k}], [q_{r,k}], [(P, G, X)_s] */
+    int n_params, n_regions;
+    double model_error, Rrs_error, depth, K_min, bottom_albedo;
+    double B_percent[ORC_NBOTTOM_TYPES];
+    double bottom_type, iod;
+    int n_iterations, converged, n_starts;
+    long n_evals;
+} orc_region_result;
+
+int orc_region_n_params(const orc_region_pixel *px);
+int orc_region_gather(orc_region_pixel *px, const float *rrs, const float *nodata, int nd,
+                      int nb, int n_rows, int n_cols, int i, int j, int n_spatial);
+void orc_region_init(orc_region_pixel *px, int n_bottoms, const int *bottom_idx, int n_dates,
+                     const orc_bands *bands, const double *sec_sun, const double *sec_view);
+double orc_region_objective(double *x, void *px);
+void orc_region_start_step(const orc_region_pixel *px, double h0, double *start, double *step);
+void orc_region_optimise(orc_region_pixel *px, const orc_nm_cfg *cfg, orc_nelmin_fn nm,
+                         orc_region_result *res);
 
 /* --- synthetic scenes (Philox4x32-10, Salmon et al. 2011) --- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
