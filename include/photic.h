@@ -136,6 +136,33 @@ int photic_invert_stack_h(const photic_cfg *cfg, long n_pix, int n_dates, const 
                           const unsigned char *valid, const float *hstart,
                           const photic_outputs *out, void *stream);
 
+/* Spatial regions: the reference's n_spatial > 1 run, replacing the per-pixel loop of
+ * model/samodel.c:901-1128 with extract_Rrs_data (:2957-3027), samodel_error over every region
+ * (:2432-2759, with the depth and bottom continuity penalties) and the region outputs of
+ * samodel_optimise (:1992-2079).  Pixel idx[q] (row * n_cols + col) of an n_rows x n_cols scene
+ * is inverted jointly with its (2 n_spatial - 1)^2 neighbours, clamped at the edges, a
+ * neighbour with any no-data band dropped: one depth, n_bottoms amplitudes and n_bottoms
+ * weights per region, one P, G, X per scene; early exit at 2.5 n_scenes (cfg->early_exit > 0:
+ * that value times n_scenes).  rrs [n_scenes][4][n_rows * n_cols] (device; already smoothed
+ * when the caller smooths), nodata [4 n_scenes] (host) the no-data value of each band grid,
+ * sec_sun and sec_view [n_scenes] (host) each scene's subsurface secants.  idx [n] (device) or
+ * NULL for every pixel (n = n_rows * n_cols), each then screened by the reference's rule (any
+ * band no-data or negative: not modelled, model/samodel.c:933-947).  hstart [n] (device): one
+ * depth start per listed pixel (an empirical depth), or NULL for the configured list.
+ * Outputs per listed pixel as photic_invert: depth is the regions' depths weighted 1 and the
+ * centre region sqrt(N_r), bottom percentages and type are the centre region's, IOD is over
+ * every region; out->x is [R_max (1 + 2 n_bottoms) + 3 n_scenes][n] with R_max = (2 n_spatial
+ * - 1)^2, in the reference's layout [H_r], [B_{r,k}], [q_{r,k}], [(P, G, X)_s] and zero past
+ * the pixel's own parameter count.  n_regions [n] (device) or NULL: each pixel's region count
+ * (0: not modelled, status -1).  PHOTIC_MODE_NATIVE with 1..3 bottoms, 4 bands, FP64, at most
+ * 128 parameters (n_spatial 2 with up to 3 bottoms and 7 scenes; n_spatial 3 with one bottom
+ * and up to 7 scenes). */
+int photic_invert_regions(const photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
+                          const float *rrs, const float *nodata, const double *sec_sun,
+                          const double *sec_view, int n_spatial, long n, const int64_t *idx,
+                          const float *hstart, const photic_outputs *out, int *n_regions,
+                          void *stream);
+
 /* Synthetic stack scene (Philox as photic_gen_scene, counter word 3 = date): H ~ U(0.5, 25),
  * B ~ U(0, 1) per pixel; per date sun zenith ~ U(20, 50) deg and log-normal P, G, X; a
  * Bernoulli(p_cloud) cloud mask with at least k_min clear dates; a date whose noisy Rrs goes

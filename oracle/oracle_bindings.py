@@ -72,7 +72,7 @@ def lib():
         _lib.orc_py_set_threads.argtypes = [i]
         _lib.orc_py_invert_regions.restype = l
         _lib.orc_py_invert_regions.argtypes = [v, i, i, i, i, v, v, v, v, i, i, v, d, i, i, i, v,
-                                               v, l, v, i, v, v, C.POINTER(PyOut)]
+                                               v, l, v, i, i, v, v, C.POINTER(PyOut)]
         _lib.orc_py_region_objective.restype = d
         _lib.orc_py_region_objective.argtypes = [v, i, i, i, i, v, v, v, i, v, v, v]
         _lib.orc_py_region_start_step.restype = i
@@ -414,10 +414,11 @@ def _region_out(npar, n):
 
 def invert_regions(rrs, n_rows, n_cols, idx, nodata, sec_sun, sec_view, n_spatial=2,
                    n_bottoms=3, bottom_idx=(0, 1, 2), hstart_px=None, reqmin=1e-2, konvge=100,
-                   kcount=5000, hstart=HSTART8, wl=WL4):
+                   kcount=5000, hstart=HSTART8, screen=False, wl=WL4):
     """Oracle inversion of the pixels idx (row * n_cols + col) of a scene rrs [nd][nb][n_rows *
-    n_cols] with their (2 n_spatial - 1)^2 regions; nodata [nd][nb]; sec_sun, sec_view [nd].
-    x is [R_max (1 + 2 n_bottoms) + 3 nd][n] in the reference's layout, zero-padded."""
+    n_cols] with their (2 n_spatial - 1)^2 regions; nodata [nd][nb]; sec_sun, sec_view [nd];
+    screen: apply the reference's own-pixel no-data rule first.  x is [R_max (1 + 2 n_bottoms)
+    + 3 nd][n] in the reference's layout, zero-padded."""
     rrs = np.ascontiguousarray(rrs, np.float32)
     nd, nb, npix = rrs.shape
     assert npix == n_rows * n_cols
@@ -439,7 +440,7 @@ def invert_regions(rrs, n_rows, n_cols, idx, nodata, sec_sun, sec_view, n_spatia
     lib().orc_py_invert_regions(w.ctypes.data, nb, nd, n_rows, n_cols, rrs.ctypes.data,
                                 nod.ctypes.data, ss.ctypes.data, sv.ctypes.data, n_spatial,
                                 n_bottoms, bi.ctypes.data, reqmin, konvge, kcount, len(hs),
-                                hs.ctypes.data, _p(hp), n, ix.ctypes.data, npar,
+                                hs.ctypes.data, _p(hp), n, ix.ctypes.data, int(screen), npar,
                                 out["n_regions"].ctypes.data, out["origin"].ctypes.data,
                                 C.byref(o))
     return out

@@ -487,7 +487,9 @@ void orc_py_disc_mask(uint64_t seed, long n_rows, long n_cols, int n_discs, doub
 /* Invert the pixels idx[0..n-1] (row * n_cols + col) of an n_rows x n_cols scene with their
  * (2 n_spatial - 1)^2 neighbourhoods: rrs [nd][nb][n_rows * n_cols] (already smoothed),
  * nodata [nd * nb] (one value per band grid), sec_sun / sec_view [nd] per scene, hstart_px [n]
- * one depth start per pixel or NULL (the list).  Pixels with no region left get status -1.
+ * one depth start per pixel or NULL (the list).  screen: first apply the reference's per-pixel
+ * rule to the pixel itself (any band of any scene no-data or negative: not modelled,
+ * model/samodel.c:933-947).  Pixels not modelled or with no region left get status -1.
  * x [stride][n] (stride >= the largest parameter count) receives |best| in the reference's
  * layout; n_regions [n] and origin [n] the gather's result.  Other outputs as orc_py_invert. */
 long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_cols,
@@ -495,7 +497,7 @@ long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_c
                            const double *sec_view, int n_spatial, int n_bottoms,
                            const int *bottom_idx, double reqmin, int konvge, int kcount,
                            int n_hstarts, const double *hstart, const float *hstart_px,
-                           long n, const long *idx, int x_stride, int *n_regions,
+                           long n, const long *idx, int screen, int x_stride, int *n_regions,
                            int *origin, orc_py_out *o)
 {
     orc_bands bands;
@@ -506,7 +508,14 @@ long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_c
         orc_region_pixel *px = (orc_region_pixel *)calloc(1, sizeof(orc_region_pixel));
         orc_region_result *res = (orc_region_result *)calloc(1, sizeof(orc_region_result));
         const int i = (int)(idx[p] / n_cols), j = (int)(idx[p] % n_cols);
-        int R = orc_region_gather(px, rrs, nodata, nd, nb, n_rows, n_cols, i, j, n_spatial);
+        int bad = 0;
+        for (int t = 0; screen && t < nd * nb; t++) {
+            float v = rrs[(long)t * n_rows * n_cols + idx[p]];
+            if (orc_approx_equal(v, nodata[t], 1.0e-6f) || v < 0.0f)
+                bad = 1;
+        }
+        int R = bad ? 0 : orc_region_gather(px, rrs, nodata, nd, nb, n_rows, n_cols, i, j,
+                                            n_spatial);
         if (n_regions) n_regions[p] = R;
         if (origin) origin[p] = px->origin;
         if (o->x)

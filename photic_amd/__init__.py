@@ -132,6 +132,8 @@ def _load():
                                     vp]),
         "photic_invert_stack_h": (i, [C.POINTER(Cfg), l, i, vp, vp, vp, vp, vp,
                                       C.POINTER(Outputs), vp]),
+        "photic_invert_regions": (i, [C.POINTER(Cfg), i, i, i, vp, vp, vp, vp, i, l, vp, vp,
+                                      C.POINTER(Outputs), vp, vp]),
         "photic_autotune": (i, [C.POINTER(Cfg), l, vp, vp, vp, C.POINTER(Tuning), vp]),
         "photic_autotune_stack": (i, [C.POINTER(Cfg), l, i, vp, vp, vp, vp, C.POINTER(Tuning),
                                       vp]),
@@ -348,6 +350,64 @@ def invert_stack(cfg, rrs, sec_sun, valid, seed=None, fields=None, sec_view=None
     else:
         invert_stack_device(cfg, n, nd, d_rrs.value, d_ss.value, d_v.value,
                             d_seed.value if d_seed is not None else None, outs)
+    check(lib.photic_synchronize())
+    return outs.fetch()
+
+
+class RegionOutputs(DeviceOutputs):
+    """Device output buffers for n pixels of a region run (x is [R_max (1 + 2 n_bottoms) +
+    3 n_scenes][n], R_max = (2 n_spatial - 1)^2), plus the region count per pixel."""
+
+    def __init__(self, cfg, n, n_scenes, n_spatial, fields=None):
+        self.n = n
+        ns = max(n_spatial, 1)
+        self.np = (2 * ns - 1) ** 2 * (1 + 2 * cfg.n_bottoms) + 3 * n_scenes
+        self.bufs = {}
+        self.struct = Outputs()
+        for name, (dt, rows) in _OUT_FIELDS.items():
+            if fields is not None and name not in fields:
+                continue
+            rows = self.np if rows is None else rows
+            b = DeviceBuffer(np.dtype(dt).itemsize * rows * n)
+            self.bufs[name] = (b, dt, rows)
+            setattr(self.struct, name, b.value)
+        b = DeviceBuffer(4 * n)
+        self.bufs["n_regions"] = (b, np.int32, 1)
+        self.n_regions = b
+
+
+def invert_regions_device(cfg, n_rows, n_cols, n_scenes, d_rrs, nodata, sec_sun, sec_view,
+                          n_spatial, n, d_idx=None, d_hstart=None, outs=None, stream=None):
+    """photic_invert_regions on device pointers (ints or None) and host nodata [4 n_scenes],
+    sec_sun, sec_view [n_scenes]; returns the RegionOutputs."""
+    if outs is None:
+        outs = RegionOutputs(cfg, n, n_scenes, n_spatial)
+    nod = np.ascontiguousarray(np.broadcast_to(np.asarray(nodata, np.float32), (n_scenes, 4)))
+    ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (n_scenes,)))
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (n_scenes,)))
+    check(lib.photic_invert_regions(C.byref(cfg), n_rows, n_cols, n_scenes, d_rrs,
+                                    nod.ctypes.data, ss.ctypes.data, sv.ctypes.data, n_spatial,
+                                    n, d_idx, d_hstart, C.byref(outs.struct),
+                                    outs.n_regions.value, stream))
+    return outs
+
+
+def invert_regions(cfg, rrs, n_rows, n_cols, nodata, sec_sun, sec_view, n_spatial=2, idx=None,
+                   hstart=None, fields=None):
+    """Invert host arrays: rrs [n_scenes][4][n_rows * n_cols] float32, nodata [n_scenes][4],
+    sec_sun / sec_view [n_scenes]; idx [n] int64 pixel indices or None (every pixel, screened);
+    hstart [n] or None.  Returns host outputs, with n_regions."""
+    rrs = np.ascontiguousarray(rrs, dtype=np.float32)
+    nd = rrs.shape[0]
+    assert rrs.shape == (nd, 4, n_rows * n_cols)
+    n = n_rows * n_cols if idx is None else len(idx)
+    d_rrs = DeviceBuffer.from_array(rrs)
+    d_idx = DeviceBuffer.from_array(np.asarray(idx, np.int64)) if idx is not None else None
+    d_h = DeviceBuffer.from_array(np.asarray(hstart, np.float32)) if hstart is not None else None
+    outs = RegionOutputs(cfg, n, nd, n_spatial, fields)
+    invert_regions_device(cfg, n_rows, n_cols, nd, d_rrs.value, nodata, sec_sun, sec_view,
+                          n_spatial, n, d_idx.value if d_idx is not None else None,
+                          d_h.value if d_h is not None else None, outs)
     check(lib.photic_synchronize())
     return outs.fetch()
 

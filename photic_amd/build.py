@@ -26,7 +26,8 @@ CC_FLAGS = [
     "-std=c11", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-D__HIP_PLATFORM_AMD__",
     "-I" + os.path.join(ROCM, "include"), "-I" + CSRC, "-I" + os.path.join(ROOT, "include"),
 ]
-HIP_SOURCES = ["ph_kernels.hip", "ph_mlp.hip", "ph_stack.hip", "ph_stack_native.hip"]
+HIP_SOURCES = ["ph_kernels.hip", "ph_mlp.hip", "ph_stack.hip", "ph_stack_native.hip",
+               "ph_region.hip"]
 C_SOURCES = ["photic_capi.c", "photic_samodel.c"]
 
 

@@ -84,6 +84,35 @@ typedef struct PhStackArgs {
     int *status, *n_evals, *n_iter, *n_starts;
 } PhStackArgs;
 
+/* One launch of the spatial-region inversion (ph_region.hip): the pixels idx[q] (or q) of an
+ * n_rows x n_cols scene, each inverted with its (2 n_spatial - 1)^2 neighbourhood, one
+ * wavefront per pixel.  A wave's LDS block (lds_doubles doubles) holds the simplex
+ * [(n_max + 1) n_max] from offset 0, then the arrays at the o_* offsets (in doubles):
+ * o_y vertex values [n_max + 1], o_x the trial point [n_max], o_d squared residual per pair,
+ * o_q rB / r per pair, o_rm measured Rrs per pair, o_pw (440 / lambda)^Y per pair
+ * [np_max each], o_k the last region's K [4 n_scenes], o_t K-penalty term per scene and the
+ * last scene's Kmin [PH_MAXD + 1], o_bq bottom fraction per (region, bottom) [r_max n_bottoms],
+ * o_m Rrs at 490 / 550 / 640 nm per (region, scene) [3 r_max n_scenes], o_pix the regions'
+ * pixel indices [r_max]. */
+typedef struct PhRegionArgs {
+    PhConst c;                     /* c.n_dates = n_scenes, c.n_bottoms native bottoms */
+    int n_rows, n_cols, n_spatial, n_scenes;
+    int r_max, n_max, np_max;      /* regions, parameters, (region, scene, band) pairs */
+    int lds_doubles;
+    int o_y, o_x, o_d, o_q, o_rm, o_pw, o_k, o_t, o_bq, o_m, o_pix;
+    float nodata[PH_MAXD * 4];     /* no-data value of each (scene, band) grid */
+    double sec_sun[PH_MAXD], sec_view[PH_MAXD];
+    long n;                        /* pixels to invert */
+    const float *rrs;              /* [n_scenes][4][n_rows * n_cols] */
+    const int64_t *idx;            /* [n] pixel indices, or NULL: pixel q is q, screened */
+    const float *hstart;           /* [n]: one depth start per pixel, or NULL: c.hstart */
+    unsigned long long *queue;     /* [0] work counter, [1] watchdog flag; zero at launch */
+    unsigned long long max_ticks;
+    float *depth, *rrs_error, *model_error, *albedo, *k_min, *iod, *b_pct, *bottom_type;
+    double *x_out;                 /* [n_max][n] */
+    int *status, *n_evals, *n_iter, *n_starts, *n_regions;
+} PhRegionArgs;
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -116,6 +145,10 @@ int ph_stack_occupancy(int k, int nbot, int seeded, int block, int *blocks_per_c
 int ph_launch_gen_stack(const PhConst *c, uint64_t seed, long pix0, long n, int n_dates,
                         double sigma, double p_cloud, int k_min, float *rrs, float *sec_sun,
                         unsigned char *valid, float *truth, void *stream);
+
+/* spatial regions (ph_region.hip) */
+int ph_launch_region(const PhRegionArgs *a, int grid, void *stream);
+int ph_region_occupancy(const PhRegionArgs *a, int *blocks_per_cu);
 
 /* the warm-start network (ph_mlp.hip) */
 int ph_mlp_nparam(void);

@@ -1,0 +1,881 @@
+// photic_amd -- spatial regions on gfx950: the reference's n_spatial > 1 run, in which each
+// pixel is inverted jointly with its (2 n_spatial - 1)^2 neighbours (model/samodel.c:
+// 2957-3027): one depth, bottom amplitudes and bottom weights per region, one P, G, X per
+// scene shared by every region, and a misfit over every (region, scene, band) plus the depth
+// and bottom continuity penalties and the K penalty (model/samodel.c:2432-2759).
+//
+// With up to 25 regions the problem has up to ~100 parameters and 800 band evaluations per
+// objective, so one wavefront runs one pixel:
+//
+//   lane l          owns coordinates l and l + 64 (MC slots): start point, centroid, trial
+//                   point, step and best value in registers, its columns of the simplex in
+//                   LDS ([vertex][coordinate], stride N);
+//   lane l          evaluates pairs l, l + 64, ... (pair j = (region, scene, band) in the
+//                   reference's loop order) of every objective evaluation;
+//   every lane      runs the modified AS 47 Nelder-Mead state machine (model/asa047.c:10-502)
+//                   and the multi-start (model/samodel.c:2119-2427) on identical values.
+//
+// The trial point goes through LDS once per evaluation; the pairs' squared residuals, the last
+// region's attenuations and the per-region bottom fractions come back through LDS, and every
+// lane re-forms the misfit from them in the reference's order, so the result is the oracle's
+// bit for bit (FP64, correctly rounded transcendentals, no contraction).  Pixels come from one
+// global counter, one per wavefront.
+#include <hip/hip_runtime.h>
+
+#include "ph_grpengine.h"
+#include "ph_launch.h"
+
+namespace {
+
+PH_DEV int ph_uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <int MC>
+struct PhReg {
+    double* g;  // the wave's LDS block
+    int o_y;    // its vertex values
+    int li, R, N, NP, nd, origin;
+    double rq;  // reqmin * N
+    double xs[MC], xb[MC], xe[MC], step[MC], best[MC], xs0[MC], step0[MC];
+    double h0, tot;
+    double ylo, ystar, ynewlo, del, lowest, yr;
+    int ilo, ihi, icount, jcount, numres, ifault, phase, sub, kstart, nstart;
+    int best_icount, best_conv, evals;
+
+    PH_DEV int cidx(int m) const { return li + 64 * m; }
+    PH_DEV bool co(int m) const { return li + 64 * m < N; }
+    PH_DEV double p(int k, int m) const { return g[k * N + li + 64 * m]; }
+    PH_DEV void set_p(int k, int m, double v) const { g[k * N + li + 64 * m] = v; }
+    PH_DEV double y(int k) const { return g[o_y + k]; }
+    PH_DEV void set_y(int k, double v) const { g[o_y + k] = v; }
+};
+
+struct PhRegSide {
+    double k_min, iod_sum, nobs, Rrs_error, model_error, albedo;
+};
+
+// A band coefficient for a lane-varying band b.
+#define PH_BSEL(arr, b) ph_band_sel(arr, b)
+
+// The region misfit of the point held in the lanes' xe (model/samodel.c:2432-2759).
+template <int NBOT, int MC>
+PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const PhConst& c,
+                               bool side_on, PhRegSide* side)
+{
+    double* g = S.g;
+    const int li = S.li, R = S.R, N = S.N, NP = S.NP, nd = S.nd;
+    const int off = R + 2 * NBOT * R;
+    ph_wave_sync(); // the last evaluation's readers are done with the buffers
+#pragma unroll
+    for (int m = 0; m < MC; m++)
+        if (S.co(m)) g[a.o_x + S.cidx(m)] = S.xe[m];
+    ph_wave_sync();
+    const double* X = g + a.o_x;
+    for (int j = li; j < NP; j += 64) {
+        const int r = j / (4 * nd);
+        const int s = (j >> 2) - r * nd;
+        const int b = j & 3;
+        const double H = fabs(X[r]);
+        double Bk[NBOT], q[NBOT], qsum = 0.0;
+#pragma unroll
+        for (int k = 0; k < NBOT; k++) Bk[k] = 0.01 * fabs(X[R + r * NBOT + k]);
+#pragma unroll
+        for (int k = 0; k < NBOT; k++) {
+            q[k] = fabs(X[R + R * NBOT + r * NBOT + k]);
+            qsum += q[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NBOT; k++) q[k] /= qsum;
+        double rho = 0.0;
+#pragma unroll
+        for (int k = 0; k < NBOT; k++) rho += q[k] * Bk[k] * PH_BSEL(c.bot[c.bottom_idx[k]], b);
+        const double P = 0.01 * fabs(X[off + 3 * s]);
+        const double Gc = 0.01 * fabs(X[off + 1 + 3 * s]);
+        const double Xc = 0.01 * fabs(X[off + 2 + 3 * s]);
+        double ss = a.sec_sun[0], sv = a.sec_view[0];
+        for (int t = 1; t < PH_MAXD; t++) {
+            ss = ph_pick(s == t, a.sec_sun[t], ss);
+            sv = ph_pick(s == t, a.sec_view[t], sv);
+        }
+        double K, rB, rr;
+        const double m = ph_band64(P, Gc, Xc, rho, H, PH_BSEL(c.a0, b), PH_BSEL(c.a1, b),
+                                   PH_BSEL(c.aw, b), PH_BSEL(c.bbw, b), PH_BSEL(c.eg, b),
+                                   g[a.o_pw + j], ss, sv, &K, &rB, &rr);
+        const double d = m - g[a.o_rm + j];
+        g[a.o_d + j] = d * d;
+        if (r == R - 1) g[a.o_k + 4 * s + b] = K; // the reference keeps the last region's K
+        if (side_on) g[a.o_q + j] = rB / rr;
+    }
+    // each region's bottom fractions |B_k| |q_k| / sum |q| (model/samodel.c:2648-2686)
+    if (li < R) {
+        double qs = 0.0;
+#pragma unroll
+        for (int kb = 0; kb < NBOT; kb++) qs += fabs(X[R + R * NBOT + li * NBOT + kb]);
+#pragma unroll
+        for (int k = 0; k < NBOT; k++)
+            g[a.o_bq + li * NBOT + k] =
+                fabs(X[R + li * NBOT + k]) * fabs(X[R + R * NBOT + li * NBOT + k]) / qs;
+    }
+    ph_wave_sync();
+    // lane s < nd: scene s's Kmin over the last region's bands and its K-penalty term, at the
+    // origin's depth (model/samodel.c:2696-2732)
+    if (li < nd) {
+        const double H = fabs(X[S.origin]);
+        const double kmm = 0.275, kmn = 0.185;
+        double Kmin = 1.0e4;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const double K = g[a.o_k + 4 * li + b];
+            if (!ph_approx_equal((float)K, 0.0f, 1.0e-6f) && K < Kmin) Kmin = K;
+        }
+        double thr = -1.0;
+        if (H < 1.0 && Kmin < kmm) thr = 0.01 + kmn;
+        else if (H < 2.0 && Kmin < 0.5 * (1.5 * kmn + 0.5 * kmm)) thr = 0.01 + 0.5 * (1.5 * kmn + 0.5 * kmm);
+        else if (H < 3.0 && Kmin < 0.5 * (1.25 * kmn + 0.75 * kmm)) thr = 0.01 + 0.5 * (1.25 * kmn + 0.75 * kmm);
+        else if (H < 4.0 && Kmin < 0.5 * (1.5 * kmn + 0.5 * kmm)) thr = 0.01 + 0.5 * (1.5 * kmn + 0.5 * kmm);
+        else if (H < 5.0 && Kmin < 0.5 * (1.75 * kmn + 0.25 * kmm)) thr = 0.01 + 0.5 * (1.75 * kmn + 0.25 * kmm);
+        // the sum below adds +0.0 for a scene without a penalty: exact, since the running
+        // penalty is never -0.0
+        double term = 0.0;
+        if (thr > 0.0) {
+            const double t = 1.0 / (0.01 + Kmin) - 1.0 / thr;
+            term = 100.0 * (t * t);
+        }
+        g[a.o_t + li] = term;
+        if (li == nd - 1) g[a.o_t + PH_MAXD] = Kmin;
+    }
+    ph_wave_sync();
+
+    // ---- every lane: the misfit in the reference's order ----
+    double err = 0.0;
+    for (int j = 0; j < NP; j++) err += g[a.o_d + j];
+    const double nt = (double)NP;
+    const double mean = S.tot / nt;
+    const double e_rrs = 100.0 * sqrt(err / nt) / mean;
+    // depth continuity (model/samodel.c:2599-2630)
+    double dm = 0.0;
+    for (int r = 0; r < R; r++) dm += fabs(X[r]);
+    dm /= (double)R;
+    double thr;
+    if (dm < 4.0) thr = 0.4;
+    else if (dm < 8.0) thr = 0.2;
+    else if (dm < 12.0) thr = 0.1;
+    else thr = 0.05;
+    double ed = 0.0, nout = 0.0;
+    for (int r = 0; r < R; r++) {
+        const double h = fabs(X[r]);
+        if (h < (1.0 - thr) * dm || h > (1.0 + thr) * dm) {
+            ed += (h - dm) * (h - dm);
+            nout += 1.0;
+        }
+    }
+    if (nout > 0.5) ed = 100.0 * sqrt(ed / nout) / dm;
+    // bottom continuity (model/samodel.c:2632-2692)
+    double bthr;
+    if (dm < 5.0) bthr = 0.25;
+    else if (dm < 10.0) bthr = 0.1;
+    else if (dm < 15.0) bthr = 0.05;
+    else bthr = 0.01;
+    double eb = 0.0, btot = 0.0;
+    nout = 0.0;
+#pragma unroll
+    for (int k = 0; k < NBOT; k++) {
+        double bm = 0.0;
+        for (int r = 0; r < R; r++) bm += g[a.o_bq + r * NBOT + k];
+        bm /= (double)R;
+        btot += bm;
+        for (int r = 0; r < R; r++) {
+            const double bo = g[a.o_bq + r * NBOT + k];
+            if (bo < (1.0 - bthr) * bm || bo > (1.0 + bthr) * bm) {
+                eb += (bo - bm) * (bo - bm);
+                nout += 1.0;
+            }
+        }
+    }
+    if (nout > 0.5) {
+        const double bm = btot / (double)NBOT;
+        eb = 100.0 * sqrt(eb / nout) / bm;
+    }
+    double Kerr = 0.0;
+    for (int s = 0; s < nd; s++) Kerr += g[a.o_t + s];
+    const double Kmin = g[a.o_t + PH_MAXD];
+    if (Kmin > 0.7) {
+        const double t = 4.0 * (Kmin - 0.7);
+        Kerr += 100.0 * (t * t);
+    }
+    double e = 80.0 * e_rrs + 15.0 * ed + 10.0 * eb + 15.0 * Kerr;
+    e /= 120.0;
+    if (side_on) {
+        double kmin_all = 1.0e10;
+        for (int j = 0; j < 4 * nd; j++) {
+            const double K = g[a.o_k + j];
+            if (!ph_approx_equal((float)K, 0.0f, 1.0e-4f) && K < kmin_all) kmin_all = K;
+        }
+        // IOD in (scene, region, band) order (model/samodel.c:2051-2064)
+        double iod = 0.0, nobs = 0.0;
+        for (int s = 0; s < nd; s++)
+            for (int r = 0; r < R; r++)
+                for (int b = 0; b < 4; b++) {
+                    iod += g[a.o_q + (r * nd + s) * 4 + b];
+                    nobs += 1.0;
+                }
+        // the albedo of the last band evaluated: region R - 1's sum q_k B_k
+        double Bk[NBOT], q[NBOT], qsum = 0.0, alb = 0.0;
+#pragma unroll
+        for (int k = 0; k < NBOT; k++) Bk[k] = 0.01 * fabs(X[R + (R - 1) * NBOT + k]);
+#pragma unroll
+        for (int k = 0; k < NBOT; k++) {
+            q[k] = fabs(X[R + R * NBOT + (R - 1) * NBOT + k]);
+            qsum += q[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NBOT; k++) q[k] /= qsum;
+#pragma unroll
+        for (int k = 0; k < NBOT; k++) alb += q[k] * Bk[k];
+        side->k_min = kmin_all == 1.0e10 ? 0.0 : kmin_all;
+        side->iod_sum = iod;
+        side->nobs = nobs;
+        side->Rrs_error = e_rrs;
+        side->model_error = e;
+        side->albedo = alb;
+    }
+    return e;
+}
+
+// Start point and steps of depth start k (model/samodel.c:2246-2354): every region's depth at
+// the start depth (the pixel's own when it has one), step 1.25 H; the rest from xs0 / step0.
+template <int MC>
+PH_DEV void ph_reg_start_step(PhReg<MC>& S, const PhConst& c, int k)
+{
+    double h0 = c.hstart[0];
+    ph_for<PH_MAXH - 1>([&](auto m0) PH_BODY {
+        constexpr int m = m0 + 1;
+        h0 = ph_pick(k == m, c.hstart[m], h0);
+    });
+    if (S.h0 >= 0.0) h0 = S.h0;
+#pragma unroll
+    for (int m = 0; m < MC; m++) {
+        const bool hc = S.cidx(m) < S.R;
+        S.xs[m] = hc ? h0 : S.xs0[m];
+        S.step[m] = hc ? 1.25 * h0 : S.step0[m];
+    }
+}
+
+template <int MC>
+PH_DEV void ph_reg_nm_begin(PhReg<MC>& S, const PhConst& c)
+{
+    S.icount = 0;
+    S.numres = 0;
+    S.jcount = c.konvge;
+    S.del = 1.0;
+    S.phase = PH_BUILD;
+    S.sub = S.N;
+#pragma unroll
+    for (int m = 0; m < MC; m++) S.xe[m] = S.xs[m];
+}
+
+// One step of the state machine (the group engine's, with runtime N and MC coordinates per
+// lane): consume f, the value of the point in xe in phase S.phase, and leave the next point to
+// evaluate in xe (or PH_START / PH_DONE).
+template <int MC>
+PH_DEV void ph_reg_consume(PhReg<MC>& S, const PhConst& c, double f)
+{
+    const int N = S.N;
+    const double dn = (double)N, dnn = (double)(N + 1);
+    const double eps = 1.0e-6;
+    enum {
+        A_NONE,
+        A_BUILD_NEXT,
+        A_AFTER_BUILD,
+        A_MOVE,
+        A_SHRINK_NEXT,
+        A_ITER_END,
+        A_FACT_FAIL,
+        A_FACT_MINUS,
+        A_FACT_NEXT,
+        A_RUN_DONE
+    };
+    S.evals++;
+    const int ph = S.phase;
+    if (ph == PH_FINAL) {
+        S.phase = PH_DONE;
+        return;
+    }
+    if (ph < PH_BUILD || ph > PH_FACT_MINUS) return;
+    S.icount++;
+
+    // ---- classify (asa047.c:176-393, 455-479) ----
+    bool wr = false, wr_xs = false;
+    int wj = 0;
+    double wf = f;
+    int act = A_NONE;
+    bool mv_phi = false;
+    double mv_coef = 0.5;
+    int mv_phase = PH_EXPAND;
+    if (ph == PH_BUILD) {
+        wr = true;
+        wj = S.sub;
+        const int next = (S.sub == N) ? 0 : S.sub + 1;
+        S.sub = next;
+        act = (next < N) ? A_BUILD_NEXT : A_AFTER_BUILD;
+    } else if (ph == PH_REFLECT) {
+        S.ystar = f;
+        if (f < S.ylo) {
+            act = A_MOVE;
+            mv_coef = 2.0;
+            mv_phase = PH_EXPAND;
+        } else {
+            int l = 0;
+            for (int i = 0; i <= N; i++) l += (f < S.y(i)) ? 1 : 0;
+            if (l > 1) {
+                wr = true;
+                wj = S.ihi;
+                wr_xs = true;
+                wf = f;
+                act = A_ITER_END;
+            } else if (l == 0) {
+                act = A_MOVE;
+                mv_phi = true;
+                mv_phase = PH_CONTRACT_IN;
+            } else {
+                act = A_MOVE;
+                mv_phase = PH_CONTRACT_OUT;
+            }
+        }
+    } else if (ph == PH_EXPAND) {
+        wr = true;
+        wj = S.ihi;
+        wr_xs = S.ystar < f;
+        wf = ph_pick(wr_xs, S.ystar, f);
+        act = A_ITER_END;
+    } else if (ph == PH_CONTRACT_IN) {
+        if (S.y(S.ihi) < f) {
+            S.sub = 0;
+            act = A_SHRINK_NEXT;
+        } else {
+            wr = true;
+            wj = S.ihi;
+            act = A_ITER_END;
+        }
+    } else if (ph == PH_CONTRACT_OUT) {
+        wr = true;
+        wj = S.ihi;
+        wr_xs = !(f <= S.ystar);
+        wf = ph_pick(wr_xs, S.ystar, f);
+        act = A_ITER_END;
+    } else if (ph == PH_SHRINK) {
+        wr = true;
+        wj = S.sub;
+        S.sub = S.sub + 1;
+        act = (S.sub <= N) ? A_SHRINK_NEXT : A_AFTER_BUILD;
+    } else if (ph == PH_FACT_PLUS) {
+        act = (ph_trunc10(f) < S.yr) ? A_FACT_FAIL : A_FACT_MINUS;
+    } else { // PH_FACT_MINUS
+        act = (ph_trunc10(f) < S.yr) ? A_FACT_FAIL : A_FACT_NEXT;
+    }
+
+    // ---- the vertex write ----
+    if (wr) {
+#pragma unroll
+        for (int m = 0; m < MC; m++)
+            if (S.co(m)) S.set_p(wj, m, ph_pick(wr_xs, S.xs[m], S.xe[m]));
+        S.set_y(wj, wf);
+    }
+
+    // ---- follow-up ----
+    if (act == A_BUILD_NEXT) {
+        const int j = S.sub;
+#pragma unroll
+        for (int m = 0; m < MC; m++)
+            S.xe[m] = (j == S.cidx(m)) ? S.xs[m] + S.step[m] * S.del : +S.xs[m];
+        S.phase = PH_BUILD;
+        return;
+    }
+    if (act == A_MOVE) {
+#pragma unroll
+        for (int m = 0; m < MC; m++)
+            if (S.co(m)) {
+                const double v = mv_phi ? S.p(S.ihi, m) : +S.xs[m];
+                const double b = S.xb[m];
+                S.xe[m] = b + mv_coef * (v - b);
+            }
+        S.phase = mv_phase;
+        return;
+    }
+    if (act == A_SHRINK_NEXT) {
+#pragma unroll
+        for (int m = 0; m < MC; m++)
+            if (S.co(m)) S.xe[m] = (S.p(S.sub, m) + S.p(S.ilo, m)) * 0.5;
+        S.phase = PH_SHRINK;
+        return;
+    }
+    if (act == A_FACT_FAIL) {
+        // asa047.c:488-493: restart from the probe point with a simplex of relative size eps
+        S.ifault = 2;
+#pragma unroll
+        for (int m = 0; m < MC; m++) {
+            S.xs[m] = S.xb[m];
+            S.xe[m] = S.xb[m];
+        }
+        S.del = eps;
+        S.numres++;
+        S.phase = PH_BUILD;
+        S.sub = N;
+        return;
+    }
+    if (act == A_FACT_MINUS || act == A_FACT_NEXT) {
+        const int i0 = S.sub;
+        const bool minus = (act == A_FACT_MINUS);
+        const int i1 = minus ? -1 : i0 + 1;
+#pragma unroll
+        for (int m = 0; m < MC; m++) {
+            const double d = S.step[m] * eps;
+            double v = S.xb[m];
+            if (i0 == S.cidx(m)) v = minus ? v - d - d : v + d;
+            if (i1 == S.cidx(m)) v = v + d;
+            S.xb[m] = v;
+            S.xe[m] = v;
+        }
+        if (minus) {
+            S.phase = PH_FACT_MINUS;
+            return;
+        }
+        if (i1 < N) {
+            S.sub = i1;
+            S.phase = PH_FACT_PLUS;
+            return;
+        }
+        S.ifault = 0;
+        act = A_RUN_DONE;
+    }
+
+    bool iter_begin = false, nm_finish = false;
+    if (act == A_AFTER_BUILD) {
+        S.ylo = S.y(0);
+        S.ilo = 0;
+        for (int i = 1; i <= N; i++) {
+            const double yi = S.y(i);
+            if (yi < S.ylo) {
+                S.ylo = yi;
+                S.ilo = i;
+            }
+        }
+        iter_begin = true;
+    }
+    // ---- end of an iteration: asa047.c:397-434 ----
+    if (act == A_ITER_END) {
+        const double yhi = S.y(S.ihi);
+        if (yhi < S.ylo) {
+            S.ylo = yhi;
+            S.ilo = S.ihi;
+        }
+        S.jcount--;
+        if (S.jcount > 0) {
+            iter_begin = true;
+        } else if (S.icount <= c.kcount) {
+            S.jcount = c.konvge;
+            double z = 0.0;
+            for (int i = 0; i <= N; i++) z = z + S.y(i);
+            const double xm = z / dnn;
+            z = 0.0;
+            for (int i = 0; i <= N; i++) {
+                const double yi = S.y(i);
+                z = z + (yi - xm) * (yi - xm);
+            }
+            if (z <= S.rq) nm_finish = true;
+            else iter_begin = true;
+        } else {
+            iter_begin = true;
+        }
+    }
+    // ---- start of an iteration: asa047.c:215-253 ----
+    if (iter_begin) {
+        if (c.kcount <= S.icount) {
+            nm_finish = true;
+        } else {
+            S.ynewlo = S.y(0);
+            S.ihi = 0;
+            for (int i = 1; i <= N; i++) {
+                const double yi = S.y(i);
+                if (S.ynewlo < yi) {
+                    S.ynewlo = yi;
+                    S.ihi = i;
+                }
+            }
+            // centroid without the worst vertex: the sum over vertices 0..N in order, less
+            // the worst vertex's coordinate
+            const int hi = S.ihi;
+#pragma unroll
+            for (int m = 0; m < MC; m++)
+                if (S.co(m)) {
+                    double z = 0.0;
+                    for (int k = 0; k <= N; k++) z = z + S.p(k, m);
+                    const double phi = S.p(hi, m);
+                    const double b = (z - phi) / dn;
+                    const double r = b + 1.0 * (b - phi);
+                    S.xb[m] = b;
+                    S.xs[m] = r;
+                    S.xe[m] = r;
+                }
+            S.phase = PH_REFLECT;
+            return;
+        }
+    }
+    // ---- convergence or evaluations exhausted: asa047.c:440-454 ----
+    if (nm_finish) {
+        const int lo = S.ilo;
+#pragma unroll
+        for (int m = 0; m < MC; m++)
+            if (S.co(m)) S.xb[m] = S.p(lo, m);
+        S.ynewlo = S.y(lo);
+        S.yr = ph_trunc10(S.ynewlo);
+        if (c.kcount < S.icount) {
+            S.ifault = 2;
+            act = A_RUN_DONE;
+        } else {
+            S.ifault = 0;
+            S.sub = 0;
+            if (S.li == 0) S.xb[0] = S.xb[0] + S.step[0] * eps;
+#pragma unroll
+            for (int m = 0; m < MC; m++) S.xe[m] = S.xb[m];
+            S.phase = PH_FACT_PLUS;
+            return;
+        }
+    }
+    // ---- a run ended; the multi-start bookkeeping, model/samodel.c:2385-2405 ----
+    if (act == A_RUN_DONE) {
+        S.nstart++;
+        if (S.ynewlo < S.lowest) {
+            S.lowest = S.ynewlo;
+#pragma unroll
+            for (int m = 0; m < MC; m++) S.best[m] = S.xb[m];
+            S.best_icount = S.icount;
+            S.best_conv = (S.ifault == 0) ? 1 : 0;
+            if (S.lowest < c.early_exit) S.kstart = c.n_hstarts;
+        }
+        S.kstart++;
+        if (S.kstart < c.n_hstarts) {
+            S.phase = PH_START;
+            return;
+        }
+#pragma unroll
+        for (int m = 0; m < MC; m++) S.xe[m] = S.best[m];
+        S.phase = PH_FINAL;
+    }
+}
+
+__device__ void ph_reg_write_nodata(const PhRegionArgs& a, long q, int n_regions)
+{
+    // the reference's defaults for pixels it does not model (model/samodel.c:819-841)
+    if (a.depth) a.depth[q] = 0.0f;
+    if (a.rrs_error) a.rrs_error[q] = 0.0f;
+    if (a.model_error) a.model_error[q] = 0.0f;
+    if (a.albedo) a.albedo[q] = 0.0f;
+    if (a.k_min) a.k_min[q] = 0.0f;
+    if (a.iod) a.iod[q] = 0.0f;
+    if (a.b_pct)
+        for (int m = 0; m < 3; m++) a.b_pct[(long)m * a.n + q] = -9999.0f;
+    if (a.bottom_type) a.bottom_type[q] = -9999.0f;
+    if (a.status) a.status[q] = -1;
+    if (a.n_evals) a.n_evals[q] = 0;
+    if (a.n_iter) a.n_iter[q] = 0;
+    if (a.n_starts) a.n_starts[q] = 0;
+    if (a.n_regions) a.n_regions[q] = n_regions;
+}
+
+} // namespace
+
+// One wavefront per pixel (block = 64 lanes), dynamic LDS a.lds_doubles doubles per block.
+template <int NBOT, int MC>
+__global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
+{
+    extern __shared__ double ph_reg_lds[];
+    const PhConst& c = a.c;
+    PhReg<MC> S;
+    S.g = ph_reg_lds;
+    S.o_y = a.o_y;
+    S.li = (int)__lane_id();
+    const int li = S.li;
+    const int nd = a.n_scenes;
+    S.nd = nd;
+    const long npix = (long)a.n_rows * a.n_cols;
+    const unsigned long long t0 = wall_clock64();
+    double* g = S.g;
+
+    for (;;) {
+        if (wall_clock64() - t0 > a.max_ticks) {
+            if (li == 0) atomicOr(a.queue + 1, 1ull);
+            break;
+        }
+        unsigned long long qq = 0;
+        if (li == 0) qq = atomicAdd(a.queue, 1ull);
+        const long q = (long)__shfl(qq, 0);
+        if (q >= a.n) break;
+        const long p = a.idx ? a.idx[q] : q;
+        const int pi = (int)(p / a.n_cols), pj = (int)(p % a.n_cols);
+
+        // without a pixel list, the pixel's own screen: any band of any scene no-data or
+        // negative (model/samodel.c:933-947)
+        if (!a.idx) {
+            bool bad = false;
+            if (li < 4 * nd) {
+                const float v = a.rrs[(long)li * npix + p];
+                bad = ph_approx_equal(v, a.nodata[li], 1.0e-6f) || v < 0.0f;
+            }
+            if (__ballot(bad) != 0ull) {
+                if (li == 0) ph_reg_write_nodata(a, q, 0);
+                if (a.x_out)
+                    for (int i = li; i < a.n_max; i += 64) a.x_out[(long)i * a.n + q] = 0.0;
+                continue;
+            }
+        }
+
+        // ---- the regions (model/samodel.c:2957-3027): candidate li of the (2 ns - 1)^2
+        // block, row-major, clamped; dropped when any band is no-data ----
+        const int ns = a.n_spatial == 0 ? 1 : a.n_spatial;
+        const int w = 2 * ns - 1;
+        bool ok = false, centre = false;
+        long cpix = 0;
+        if (li < w * w) {
+            const int ii = li / w - (ns - 1), jj = li % w - (ns - 1);
+            const int iii = pi + ii < 0 ? 0 : (pi + ii > a.n_rows - 1 ? a.n_rows - 1 : pi + ii);
+            const int jjj = pj + jj < 0 ? 0 : (pj + jj > a.n_cols - 1 ? a.n_cols - 1 : pj + jj);
+            cpix = (long)iii * a.n_cols + jjj;
+            centre = (iii == pi && jjj == pj);
+            ok = true;
+            for (int t = 0; t < 4 * nd; t++)
+                if (ph_approx_equal(a.rrs[(long)t * npix + cpix], a.nodata[t], 1.0e-6f)) ok = false;
+        }
+        const unsigned long long okm = __ballot(ok), cm = __ballot(centre);
+        const int R = ph_uni(__popcll(okm));
+        const int tc = 63 - __clzll((long long)cm); // the last visit of the centre
+        const int origin = ph_uni(__popcll(okm & ((1ull << tc) - 1ull)));
+        if (R == 0) {
+            if (li == 0) ph_reg_write_nodata(a, q, 0);
+            if (a.x_out)
+                for (int i = li; i < a.n_max; i += 64) a.x_out[(long)i * a.n + q] = 0.0;
+            continue;
+        }
+        const int N = ph_uni(R * (1 + 2 * NBOT) + 3 * nd);
+        const int NP = ph_uni(R * nd * 4);
+        S.R = R;
+        S.N = N;
+        S.NP = NP;
+        S.origin = origin;
+        S.rq = c.reqmin * (double)N;
+
+        ph_wave_sync(); // the last pixel's readers are done with the buffers
+        if (ok) g[a.o_pix + __popcll(okm & ((1ull << li) - 1ull))] = (double)cpix;
+        ph_wave_sync();
+
+        // ---- pairs: measured Rrs and (440 / lambda)^Y of the region's scene (Y from the
+        // region's 440/490 ratio, model/samodel.c:2898-2903); per (region, scene) Rrs at
+        // 490, 550 and 640 nm for the starts ----
+        for (int j = li; j < NP; j += 64) {
+            const int r = j / (4 * nd);
+            const int s = (j >> 2) - r * nd;
+            const int b = j & 3;
+            const long rp = (long)g[a.o_pix + r];
+            double r4[4];
+#pragma unroll
+            for (int bb = 0; bb < 4; bb++) r4[bb] = (double)a.rrs[((long)s * 4 + bb) * npix + rp];
+            g[a.o_rm + j] = ph_band_sel(r4, b);
+            double r440 = ph_rrs_at(c, r4, 0);
+            const double r490 = ph_rrs_at(c, r4, 1);
+            if (r440 < 0.0) r440 = 0.0001;
+            const double chi = r440 / r490;
+            double Y = 3.44 * (1.0 - 3.17 * PH_EXP(-2.01 * chi));
+            if (Y < 0.0) Y = 0.0;
+            if (Y > 2.5) Y = 2.5;
+            g[a.o_pw + j] = PH_POW(440.0 / ph_band_sel(c.wl, b), Y);
+            if (b == 0) {
+                const int t = r * nd + s;
+                g[a.o_m + 3 * t] = r490;
+                g[a.o_m + 3 * t + 1] = ph_rrs_at(c, r4, 2);
+                g[a.o_m + 3 * t + 2] = ph_rrs_at(c, r4, 3);
+            }
+        }
+        ph_wave_sync();
+        double tot = 0.0;
+        for (int j = 0; j < NP; j++) tot += g[a.o_rm + j];
+        S.tot = tot;
+
+        // ---- coordinate starts other than depth (model/samodel.c:2246-2354) ----
+        double m490 = 0.0;
+        for (int t = 0; t < R * nd; t++) m490 += g[a.o_m + 3 * t];
+        m490 /= (double)R * nd;
+        const int off = R + 2 * NBOT * R;
+#pragma unroll
+        for (int m = 0; m < MC; m++) {
+            const int cc = S.cidx(m);
+            double v = 0.0, st = 0.0;
+            if (cc >= R && cc < R + R * NBOT) {
+                v = 100.0 * 4.0 * m490;
+                st = 1.5 * v;
+            } else if (cc >= R + R * NBOT && cc < off) {
+                v = 1.0;
+                st = 0.5 * v;
+            } else if (cc >= off && cc < N) {
+                const int s = (cc - off) / 3, comp = (cc - off) % 3;
+                double a4 = 0.0, b5 = 0.0, c6 = 0.0;
+                for (int r = 0; r < R; r++) {
+                    a4 += g[a.o_m + 3 * (r * nd + s)];
+                    b5 += g[a.o_m + 3 * (r * nd + s) + 1];
+                    c6 += g[a.o_m + 3 * (r * nd + s) + 2];
+                }
+                a4 /= (double)R;
+                b5 /= (double)R;
+                c6 /= (double)R;
+                const double P0 = 100.0 * 0.072 * PH_POW(a4 / b5, -1.7);
+                const double G0 = 1.5 * P0;
+                const double X0 = 100.0 * 30.0 * c.aw640 * c6;
+                v = comp == 0 ? P0 : (comp == 1 ? G0 : X0);
+                st = 2.0 * v;
+            }
+            S.xs0[m] = v;
+            S.step0[m] = st;
+            S.best[m] = 0.0;
+        }
+        S.h0 = a.hstart ? (double)a.hstart[q] : -1.0;
+        S.kstart = 0;
+        S.lowest = 1.0e4;
+        S.nstart = 0;
+        S.evals = 0;
+        S.best_icount = 0;
+        S.best_conv = 0;
+        S.phase = PH_START;
+
+        // ---- the multi-start Nelder-Mead, one evaluation per pass ----
+        PhRegSide side;
+        bool timed_out = false;
+        for (;;) {
+            if (S.phase == PH_START) {
+                ph_reg_start_step<MC>(S, c, S.kstart);
+                ph_reg_nm_begin<MC>(S, c);
+            }
+            const bool fin = S.phase == PH_FINAL;
+            const double f = ph_reg_objective<NBOT, MC>(S, a, c, fin, &side);
+            ph_reg_consume<MC>(S, c, f);
+            if (S.phase == PH_DONE) break;
+            if ((S.evals & 255) == 0 && wall_clock64() - t0 > a.max_ticks) {
+                timed_out = true;
+                break;
+            }
+        }
+        if (timed_out) {
+            if (li == 0) atomicOr(a.queue + 1, 1ull);
+            break;
+        }
+
+        // ---- outputs (model/samodel.c:1992-2079) ----
+        if (a.x_out) {
+#pragma unroll
+            for (int m = 0; m < MC; m++)
+                if (S.co(m)) a.x_out[(long)S.cidx(m) * a.n + q] = fabs(S.best[m]);
+            for (int i = N + li; i < a.n_max; i += 64) a.x_out[(long)i * a.n + q] = 0.0;
+        }
+        ph_wave_sync();
+#pragma unroll
+        for (int m = 0; m < MC; m++)
+            if (S.co(m)) g[a.o_x + S.cidx(m)] = fabs(S.best[m]);
+        ph_wave_sync();
+        if (li == 0) {
+            const double* X = g + a.o_x;
+            // depth: regions weighted 1, the origin sqrt(N_r) (model/samodel.c:2000-2017)
+            const double ow = sqrt((double)R);
+            double d = 0.0;
+            for (int r = 0; r < R; r++) {
+                const double H = X[r];
+                if (r == origin) d += ow * H;
+                else d += H;
+            }
+            if (a.depth) a.depth[q] = (float)(d / (ow + ((double)R) - 1.0));
+            if (a.rrs_error) a.rrs_error[q] = (float)side.Rrs_error;
+            if (a.model_error) a.model_error[q] = (float)side.model_error;
+            if (a.albedo) a.albedo[q] = (float)side.albedo;
+            if (a.k_min) a.k_min[q] = (float)side.k_min;
+            if (a.iod) a.iod[q] = (float)(100.0 * side.iod_sum / side.nobs);
+            if (a.b_pct || a.bottom_type) {
+                // the origin's bottom percentages, dominant bottom 1-based
+                double qsum = 0.0, pct[PH_NBOT];
+                for (int k = 0; k < NBOT; k++) qsum += X[R + R * NBOT + origin * NBOT + k];
+                for (int m = 0; m < PH_NBOT; m++) pct[m] = 0.0;
+                for (int k = 0; k < NBOT; k++) {
+                    const int t = c.bottom_idx[k];
+                    const double v = 100.0 * X[R + R * NBOT + origin * NBOT + k] / qsum;
+                    for (int m = 0; m < PH_NBOT; m++) pct[m] = ph_pick(m == t, v, pct[m]);
+                }
+                double largest = 0.0, bt = 0.0;
+                for (int k = 0; k < NBOT; k++) {
+                    const int t = c.bottom_idx[k];
+                    double v = 0.0;
+                    for (int m = 0; m < PH_NBOT; m++) v = ph_pick(m == t, pct[m], v);
+                    if (v > largest) {
+                        largest = v;
+                        bt = 1.0 + t;
+                    }
+                }
+                if (a.b_pct)
+                    for (int m = 0; m < 3; m++) a.b_pct[(long)m * a.n + q] = (float)pct[m];
+                if (a.bottom_type) a.bottom_type[q] = (float)bt;
+            }
+            if (a.status) a.status[q] = S.best_conv ? 0 : 2;
+            if (a.n_evals) a.n_evals[q] = S.evals;
+            if (a.n_iter) a.n_iter[q] = S.best_icount;
+            if (a.n_starts) a.n_starts[q] = S.nstart;
+            if (a.n_regions) a.n_regions[q] = R;
+        }
+    }
+}
+
+template <int NBOT, int MC>
+static int ph_region_launch_t(const PhRegionArgs* a, int grid, void* stream)
+{
+    const size_t lds = sizeof(double) * (size_t)a->lds_doubles;
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void*)ph_region_kernel<NBOT, MC>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return (int)e;
+    }
+    ph_region_kernel<NBOT, MC><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+    return (int)hipGetLastError();
+}
+
+template <int NBOT, int MC>
+static int ph_region_occ_t(const PhRegionArgs* a, int* bpc)
+{
+    const size_t lds = sizeof(double) * (size_t)a->lds_doubles;
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        bpc, (const void*)ph_region_kernel<NBOT, MC>, 64, lds);
+}
+
+template <typename Fn>
+static int ph_region_dispatch(int nbot, int mc, Fn&& fn)
+{
+    if (mc == 1) {
+        if (nbot == 1) return fn(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
+        if (nbot == 2) return fn(std::integral_constant<int, 2>{}, std::integral_constant<int, 1>{});
+        if (nbot == 3) return fn(std::integral_constant<int, 3>{}, std::integral_constant<int, 1>{});
+    } else if (mc == 2) {
+        if (nbot == 1) return fn(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{});
+        if (nbot == 2) return fn(std::integral_constant<int, 2>{}, std::integral_constant<int, 2>{});
+        if (nbot == 3) return fn(std::integral_constant<int, 3>{}, std::integral_constant<int, 2>{});
+    }
+    return PH_LAUNCH_UNSUPPORTED;
+}
+
+extern "C" int ph_launch_region(const PhRegionArgs* a, int grid, void* stream)
+{
+    const int mc = a->n_max <= 64 ? 1 : (a->n_max <= 128 ? 2 : 0);
+    return ph_region_dispatch(a->c.n_bottoms, mc, [&](auto nb, auto m) {
+        return ph_region_launch_t<decltype(nb)::value, decltype(m)::value>(a, grid, stream);
+    });
+}
+
+extern "C" int ph_region_occupancy(const PhRegionArgs* a, int* blocks_per_cu)
+{
+    const int mc = a->n_max <= 64 ? 1 : (a->n_max <= 128 ? 2 : 0);
+    return ph_region_dispatch(a->c.n_bottoms, mc, [&](auto nb, auto m) {
+        return ph_region_occ_t<decltype(nb)::value, decltype(m)::value>(a, blocks_per_cu);
+    });
+}

@@ -598,6 +598,144 @@ int photic_invert_stack_h(const photic_cfg *cfg, long n_pix, int n_dates, const 
                            out, stream);
 }
 
+/* ------------------------------------------------------------------------------------ */
+/* spatial regions                                                                        */
+/* ------------------------------------------------------------------------------------ */
+
+int photic_invert_regions(const photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
+                          const float *rrs, const float *nodata, const double *sec_sun,
+                          const double *sec_view, int n_spatial, long n, const int64_t *idx,
+                          const float *hstart, const photic_outputs *out, int *n_regions,
+                          void *stream)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    if (!cfg || !out || !nodata || !sec_sun || !sec_view)
+        return ph_fail(PHOTIC_EINVAL, "cfg, out, nodata, sec_sun and sec_view are required");
+    if (n_scenes < 1 || n_scenes > PH_MAXD)
+        return ph_fail(PHOTIC_EINVAL, "n_scenes %d outside 1..%d", n_scenes, PH_MAXD);
+    if (n_rows < 1 || n_cols < 1 || n < 0 || n_spatial < 0)
+        return ph_fail(PHOTIC_EINVAL, "n_rows, n_cols >= 1, n >= 0 and n_spatial >= 0 required");
+    if ((long)n_rows * n_cols > 2147483647L)
+        return ph_fail(PHOTIC_EINVAL, "scene above 2^31 - 1 pixels");
+    if (!idx && n != (long)n_rows * n_cols)
+        return ph_fail(PHOTIC_EINVAL, "without idx, n must be n_rows * n_cols");
+    if (cfg->mode != PHOTIC_MODE_NATIVE || cfg->n_bands != 4 || cfg->precision != PHOTIC_FP64 ||
+        cfg->n_bottoms < 1 || cfg->n_bottoms > 3)
+        return ph_fail(PHOTIC_EUNSUPPORTED, "regions are compiled for the native layout with "
+                                            "1..3 bottoms, 4 bands, FP64");
+    const int ns = n_spatial == 0 ? 1 : n_spatial;
+    const int nbot = cfg->n_bottoms;
+    const int r_max = (2 * ns - 1) * (2 * ns - 1);
+    const int n_max = r_max * (1 + 2 * nbot) + 3 * n_scenes;
+    if (r_max > 64 || n_max > 128)
+        return ph_fail(PHOTIC_EUNSUPPORTED, "%d regions of %d bottoms and %d scenes need %d "
+                                            "parameters; the kernels hold at most 128",
+                       r_max, nbot, n_scenes, n_max);
+    if (n == 0) return PHOTIC_OK;
+    if (!rrs) return ph_fail(PHOTIC_EINVAL, "rrs is required");
+
+    PhRegionArgs a;
+    memset(&a, 0, sizeof(a));
+    rc = ph_make_const(cfg, n_scenes, 0, &a.c);
+    if (rc) return rc;
+    a.c.early_exit = (cfg->early_exit > 0.0 ? cfg->early_exit : 2.5) * (double)(float)n_scenes;
+    if (hstart) a.c.n_hstarts = 1; /* the pixel's own depth start replaces the list */
+    a.n_rows = n_rows;
+    a.n_cols = n_cols;
+    a.n_spatial = n_spatial;
+    a.n_scenes = n_scenes;
+    a.r_max = r_max;
+    a.n_max = n_max;
+    a.np_max = r_max * n_scenes * 4;
+    /* the wave's LDS block: the simplex, then each array at an even offset */
+    int off = (n_max + 1) * n_max;
+#define PH_TAKE(field, count)          \
+    do {                               \
+        a.field = off;                 \
+        off += ((count) + 1) & ~1;     \
+    } while (0)
+    PH_TAKE(o_y, n_max + 1);
+    PH_TAKE(o_x, n_max);
+    PH_TAKE(o_d, a.np_max);
+    PH_TAKE(o_q, a.np_max);
+    PH_TAKE(o_rm, a.np_max);
+    PH_TAKE(o_pw, a.np_max);
+    PH_TAKE(o_k, 4 * n_scenes);
+    PH_TAKE(o_t, PH_MAXD + 1);
+    PH_TAKE(o_bq, r_max * nbot);
+    PH_TAKE(o_m, 3 * r_max * n_scenes);
+    PH_TAKE(o_pix, r_max);
+#undef PH_TAKE
+    a.lds_doubles = off;
+    for (int k = 0; k < 4 * n_scenes; k++) a.nodata[k] = nodata[k];
+    for (int s = 0; s < n_scenes; s++) {
+        a.sec_sun[s] = sec_sun[s];
+        a.sec_view[s] = sec_view[s];
+    }
+    a.n = n;
+    a.rrs = rrs;
+    a.idx = idx;
+    a.hstart = hstart;
+    a.depth = out->depth;
+    a.rrs_error = out->rrs_error;
+    a.model_error = out->model_error;
+    a.albedo = out->albedo;
+    a.k_min = out->k_min;
+    a.iod = out->iod;
+    a.b_pct = out->b_pct;
+    a.bottom_type = out->bottom_type;
+    a.x_out = out->x;
+    a.status = out->status;
+    a.n_evals = out->n_evals;
+    a.n_iter = out->n_iter;
+    a.n_starts = out->n_starts;
+    a.n_regions = n_regions;
+
+    int dev = 0, wrate = 0, n_cu = 0, bpc = 0, max_lds = 0;
+    rc = ph_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess || max_lds <= 0)
+        max_lds = 65536;
+    if ((long)a.lds_doubles * 8 > max_lds)
+        return ph_fail(PHOTIC_EUNSUPPORTED, "%d regions need %d bytes of LDS per pixel, the "
+                                            "device has %d", r_max, a.lds_doubles * 8, max_lds);
+    if (hipDeviceGetAttribute(&wrate, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        wrate <= 0)
+        wrate = 100000; /* kHz */
+    const double limit_s = cfg->watchdog_s > 0.0 ? cfg->watchdog_s : 600.0;
+    a.max_ticks = (unsigned long long)(limit_s * 1000.0 * (double)wrate);
+    rc = ph_n_cu(&n_cu);
+    if (rc) return rc;
+    rc = ph_launch_status(ph_region_occupancy(&a, &bpc), "occupancy");
+    if (rc) return rc;
+    if (bpc < 1) return ph_fail(PHOTIC_EUNSUPPORTED, "region kernel does not fit a CU");
+    long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
+    if (grid > n) grid = n;
+
+    hipStream_t st = (hipStream_t)stream;
+    unsigned long long *queue = NULL;
+    rc = ph_hip(hipMallocAsync((void **)&queue, 2 * sizeof(unsigned long long), st),
+                "hipMallocAsync");
+    if (rc) return rc;
+    rc = ph_launch_status(ph_launch_zero_u64(queue, 2, st), "zero work counter");
+    if (rc == PHOTIC_OK) {
+        a.queue = queue;
+        rc = ph_launch_status(ph_launch_region(&a, (int)grid, st), "region kernel");
+    }
+    if (rc == PHOTIC_OK && cfg->check_watchdog) {
+        unsigned long long flag = 0;
+        rc = ph_hip(hipMemcpyAsync(&flag, queue + 1, sizeof(flag), hipMemcpyDeviceToHost, st),
+                    "watchdog read");
+        if (rc == PHOTIC_OK) rc = ph_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        if (rc == PHOTIC_OK && flag)
+            rc = ph_fail(PHOTIC_EDEVICE, "region kernel stopped by its %.0f s watchdog", limit_s);
+    }
+    int rc2 = ph_hip(hipFreeAsync(queue, st), "hipFreeAsync");
+    return rc ? rc : rc2;
+}
+
 int photic_gen_stack(const photic_cfg *cfg, uint64_t seed, long pix0, long n, int n_dates,
                      double sigma, double p_cloud, int k_min, float *rrs, float *sec_sun,
                      unsigned char *valid, float *truth, void *stream)

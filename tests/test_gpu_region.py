@@ -1,0 +1,107 @@
+"""Spatial regions (the reference's n_spatial > 1 run) on the GPU against the oracle: every
+pixel of a small scene inverted with its (2 n_spatial - 1)^2 neighbourhood, including no-data
+pixels (dropped as neighbours, not modelled as centres), negative neighbours (kept), clamped
+edges, per-pixel depth starts, several scenes, and the reference's own Nelder-Mead settings.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle_bindings as O  # noqa: E402
+import photic_amd as P  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def region_scene(seed, n_rows, n_cols, nd):
+    """A scene of n_rows x n_cols pixels and nd scenes (synthetic pixels laid out row-major),
+    with a no-data pixel, a pixel no-data on one band only, and a negative pixel."""
+    n = n_rows * n_cols
+    if nd == 1:
+        rrs = O.gen(seed, n)[0][None].copy()
+        sec_sun = np.array([np.float32(1.15)], np.float64)
+    else:
+        rrs, ss, valid, _ = O.gen_stack(seed, n, n_dates=nd, k_min=nd)
+        rrs[np.broadcast_to((valid == 0)[:, None, :], rrs.shape)] = -9999.0
+        sec_sun = np.array([np.float32(1.1 + 0.05 * s) for s in range(nd)], np.float64)
+    sec_view = np.array([np.float32(1.0 + 0.02 * s) for s in range(nd)], np.float64)
+    rrs[:, :, 1 * n_cols + 1] = -9999.0          # a no-data pixel
+    rrs[nd - 1, 2, (n_rows - 1) * n_cols + 2] = -9999.0  # no data on one band
+    rrs[0, 3, 2 * n_cols + (n_cols - 1)] = -0.0005  # negative: not a centre, kept as neighbour
+    return np.ascontiguousarray(rrs, np.float32), sec_sun, sec_view
+
+
+CASES = [
+    # nbot, n_spatial, nd, rows, cols, own depth starts
+    (3, 2, 1, 8, 8, False),  # the reference's defaults: 66 parameters, 36 pairs
+    (2, 2, 1, 7, 9, True),   # an empirical depth per pixel
+    (1, 3, 1, 6, 6, False),  # 25 regions, 78 parameters, 100 pairs
+    (2, 2, 2, 6, 7, False),  # two scenes
+    (1, 2, 1, 5, 5, False),  # 30 parameters: one coordinate per lane
+]
+
+
+@pytest.mark.parametrize("nbot,ns,nd,n_rows,n_cols,own_h", CASES)
+def test_regions_against_oracle(nbot, ns, nd, n_rows, n_cols, own_h):
+    """Status, region count and every output identical to the oracle's on at least 98 % of
+    the pixels (bit for bit in the parameter vector); the rest within the bar or with the same
+    misfit where it is flat."""
+    rrs, ss, sv = region_scene(200 + 10 * nbot + ns + nd, n_rows, n_cols, nd)
+    n = n_rows * n_cols
+    h = np.random.default_rng(nbot).uniform(1.0, 15.0, n).astype(np.float32) if own_h else None
+    cfg = P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=nbot, reqmin=1e-2, konvge=100,
+                        kcount=5000, check_watchdog=1)
+    print("regions: %d bottoms, n_spatial %d, %d scenes, %d x %d%s" %
+          (nbot, ns, nd, n_rows, n_cols, ", own depth starts" if own_h else ""), flush=True)
+    g = P.invert_regions(cfg, rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns, hstart=h)
+    o = O.invert_regions(rrs, n_rows, n_cols, np.arange(n), -9999.0, ss, sv, n_spatial=ns,
+                         n_bottoms=nbot, bottom_idx=(0, 1, 2), hstart_px=h, screen=True)
+    assert np.array_equal(g["status"], o["status"])
+    assert np.array_equal(g["n_regions"], o["n_regions"])
+    v = o["status"] >= 0
+    assert (~v).sum() >= 3  # the planted ones, and any the generator's noise made negative
+    assert g["x"].shape == o["x"].shape
+    exact = np.all(g["x"] == o["x"], axis=0)
+    flat = np.abs(g["model_error"] - o["model_error"]) <= 1e-6 * np.abs(o["model_error"])
+    print("  %d modelled, bit-identical %.4f, regions %s" %
+          (v.sum(), exact[v].mean(), np.bincount(o["n_regions"][v])), flush=True)
+    assert np.all(exact[v] | flat[v])
+    assert exact[v].mean() >= 0.98
+    sel = v & exact
+    for k in ("depth", "rrs_error", "model_error", "k_min", "iod", "albedo", "bottom_type",
+              "n_iter", "n_starts"):
+        assert np.array_equal(g[k][sel], o[k][sel]), k
+    assert np.array_equal(g["b_pct"][:, sel], o["b_pct"][:, sel])
+    # the oracle also counts the reference's evaluation of each start before the simplex is
+    # built, whose value is discarded; the kernel skips it
+    assert np.array_equal(g["n_evals"][sel], o["n_evals"][sel] - o["n_starts"][sel])
+    # pixels not modelled keep the reference's defaults
+    nv = ~v
+    assert np.all(g["depth"][nv] == 0.0) and np.all(g["bottom_type"][nv] == -9999.0)
+    assert np.all(g["b_pct"][:, nv] == -9999.0) and np.all(g["x"][:, nv] == 0.0)
+
+
+def test_regions_pixel_list_and_unsupported():
+    """A pixel list (the drop-in's form: screened by the caller) gives the same results as the
+    whole scene at those pixels; configurations beyond the compiled kernels are refused."""
+    nbot, ns, n_rows, n_cols = 2, 2, 6, 6
+    rrs, ss, sv = region_scene(77, n_rows, n_cols, 1)
+    cfg = P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=nbot, reqmin=1e-2, konvge=100,
+                        kcount=5000, check_watchdog=1)
+    full = P.invert_regions(cfg, rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns)
+    idx = np.flatnonzero(full["status"] >= 0)[::2].astype(np.int64)
+    part = P.invert_regions(cfg, rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns, idx=idx)
+    for k in ("x", "depth", "rrs_error", "status", "n_evals", "n_regions"):
+        a = full[k][..., idx]
+        assert np.array_equal(a, part[k]), k
+    # 5 x 5 regions of 3 bottoms: 178 parameters
+    with pytest.raises(P.PhoticError):
+        P.invert_regions(P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=3), rrs, n_rows, n_cols,
+                         -9999.0, ss, sv, n_spatial=3)
+    with pytest.raises(P.PhoticError):
+        P.invert_regions(P.cfg_default(), rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=2)
