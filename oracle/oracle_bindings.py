@@ -72,13 +72,16 @@ def lib():
         _lib.orc_py_set_threads.argtypes = [i]
         _lib.orc_py_invert_regions.restype = l
         _lib.orc_py_invert_regions.argtypes = [v, i, i, i, i, v, v, v, v, i, i, v, d, i, i, i, v,
-                                               v, l, v, i, i, v, v, C.POINTER(PyOut)]
+                                               v, l, v, v, i, v, v, C.POINTER(PyOut)]
+        _lib.orc_sec_deg.restype = d
+        _lib.orc_sec_deg.argtypes = [d]
         _lib.orc_py_region_objective.restype = d
         _lib.orc_py_region_objective.argtypes = [v, i, i, i, i, v, v, v, i, v, v, v]
         _lib.orc_py_region_start_step.restype = i
         _lib.orc_py_region_start_step.argtypes = [v, i, i, i, v, i, d, v, v]
         _lib.orc_py_region_optimise.restype = i
         _lib.orc_py_region_optimise.argtypes = [v, i, i, i, i, v, v, v, i, v, d, i, i, i, v, v, v]
+        _lib.orc_py_smooth_grid.argtypes = [v, i, i, i, C.c_float, v]
         _lib.orc_py_region_gather.restype = i
         _lib.orc_py_region_gather.argtypes = [v, v, i, i, i, i, i, i, i, v, v]
     return _lib
@@ -97,6 +100,9 @@ def ref():
         _ref.interp_1d.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double]
         _ref.approx_equal.restype = C.c_int
         _ref.approx_equal.argtypes = [C.c_float, C.c_float, C.c_float]
+        _ref.smoothed_index_2d.restype = C.c_float
+        _ref.smoothed_index_2d.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.c_int, C.c_float]
     return _ref
 
 
@@ -414,11 +420,12 @@ def _region_out(npar, n):
 
 def invert_regions(rrs, n_rows, n_cols, idx, nodata, sec_sun, sec_view, n_spatial=2,
                    n_bottoms=3, bottom_idx=(0, 1, 2), hstart_px=None, reqmin=1e-2, konvge=100,
-                   kcount=5000, hstart=HSTART8, screen=False, wl=WL4):
+                   kcount=5000, hstart=HSTART8, screen=False, screen_rrs=None, wl=WL4):
     """Oracle inversion of the pixels idx (row * n_cols + col) of a scene rrs [nd][nb][n_rows *
     n_cols] with their (2 n_spatial - 1)^2 regions; nodata [nd][nb]; sec_sun, sec_view [nd];
-    screen: apply the reference's own-pixel no-data rule first.  x is [R_max (1 + 2 n_bottoms)
-    + 3 nd][n] in the reference's layout, zero-padded."""
+    screen: apply the reference's own-pixel no-data rule first, to rrs or to screen_rrs (the
+    grids before smoothing) when given.  x is [R_max (1 + 2 n_bottoms) + 3 nd][n] in the
+    reference's layout, zero-padded."""
     rrs = np.ascontiguousarray(rrs, np.float32)
     nd, nb, npix = rrs.shape
     assert npix == n_rows * n_cols
@@ -437,10 +444,15 @@ def invert_regions(rrs, n_rows, n_cols, idx, nodata, sec_sun, sec_view, n_spatia
     ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (nd,)))
     sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (nd,)))
     hp = None if hstart_px is None else np.ascontiguousarray(hstart_px, np.float32)
+    sc = None
+    if screen_rrs is not None:
+        sc = np.ascontiguousarray(screen_rrs, np.float32).reshape(rrs.shape)
+    elif screen:
+        sc = rrs
     lib().orc_py_invert_regions(w.ctypes.data, nb, nd, n_rows, n_cols, rrs.ctypes.data,
                                 nod.ctypes.data, ss.ctypes.data, sv.ctypes.data, n_spatial,
                                 n_bottoms, bi.ctypes.data, reqmin, konvge, kcount, len(hs),
-                                hs.ctypes.data, _p(hp), n, ix.ctypes.data, int(screen), npar,
+                                hs.ctypes.data, _p(hp), n, ix.ctypes.data, _p(sc), npar,
                                 out["n_regions"].ctypes.data, out["origin"].ctypes.data,
                                 C.byref(o))
     return out
@@ -504,3 +516,23 @@ def region_gather(rrs, n_rows, n_cols, i, j, n_spatial, nodata):
     R = lib().orc_py_region_gather(g.ctypes.data, nod.ctypes.data, nd, nb, n_rows, n_cols, i, j,
                                    n_spatial, C.byref(org), out.ctypes.data)
     return out[:R], org.value
+
+
+def smooth_grid(grid, r, spval):
+    """The oracle's smoothing of every pixel of a 2-D float grid."""
+    g = np.ascontiguousarray(grid, np.float32)
+    out = np.empty_like(g)
+    lib().orc_py_smooth_grid(g.ctypes.data, g.shape[0], g.shape[1], r, spval, out.ctypes.data)
+    return out
+
+
+def ref_smoothed(grid, i, j, r, spval):
+    """The reference's own smoothed_index_2d on a row-pointer copy of grid."""
+    g = np.ascontiguousarray(grid, np.float32)
+    rows = (C.c_void_p * g.shape[0])(*[g.ctypes.data + k * g.strides[0] for k in range(g.shape[0])])
+    return ref().smoothed_index_2d(rows, i, j, g.shape[0], g.shape[1], r, spval)
+
+
+def sec_deg(theta_deg):
+    """1 / cos of an angle in degrees, as the reference converts scene angles."""
+    return lib().orc_sec_deg(float(theta_deg))

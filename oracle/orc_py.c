@@ -487,8 +487,9 @@ void orc_py_disc_mask(uint64_t seed, long n_rows, long n_cols, int n_discs, doub
 /* Invert the pixels idx[0..n-1] (row * n_cols + col) of an n_rows x n_cols scene with their
  * (2 n_spatial - 1)^2 neighbourhoods: rrs [nd][nb][n_rows * n_cols] (already smoothed),
  * nodata [nd * nb] (one value per band grid), sec_sun / sec_view [nd] per scene, hstart_px [n]
- * one depth start per pixel or NULL (the list).  screen: first apply the reference's per-pixel
- * rule to the pixel itself (any band of any scene no-data or negative: not modelled,
+ * one depth start per pixel or NULL (the list).  screen_rrs (same layout as rrs, e.g. the
+ * grids before smoothing) or NULL: first apply the reference's per-pixel rule to the pixel's
+ * own values there (any band of any scene no-data or negative: not modelled,
  * model/samodel.c:933-947).  Pixels not modelled or with no region left get status -1.
  * x [stride][n] (stride >= the largest parameter count) receives |best| in the reference's
  * layout; n_regions [n] and origin [n] the gather's result.  Other outputs as orc_py_invert. */
@@ -497,7 +498,8 @@ long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_c
                            const double *sec_view, int n_spatial, int n_bottoms,
                            const int *bottom_idx, double reqmin, int konvge, int kcount,
                            int n_hstarts, const double *hstart, const float *hstart_px,
-                           long n, const long *idx, int screen, int x_stride, int *n_regions,
+                           long n, const long *idx, const float *screen_rrs, int x_stride,
+                           int *n_regions,
                            int *origin, orc_py_out *o)
 {
     orc_bands bands;
@@ -509,8 +511,8 @@ long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_c
         orc_region_result *res = (orc_region_result *)calloc(1, sizeof(orc_region_result));
         const int i = (int)(idx[p] / n_cols), j = (int)(idx[p] % n_cols);
         int bad = 0;
-        for (int t = 0; screen && t < nd * nb; t++) {
-            float v = rrs[(long)t * n_rows * n_cols + idx[p]];
+        for (int t = 0; screen_rrs && t < nd * nb; t++) {
+            float v = screen_rrs[(long)t * n_rows * n_cols + idx[p]];
             if (orc_approx_equal(v, nodata[t], 1.0e-6f) || v < 0.0f)
                 bad = 1;
         }
@@ -671,4 +673,13 @@ int orc_py_region_gather(const float *rrs, const float *nodata, int nd, int nb, 
                 rrs_out[((long)r * nd + s) * nb + b] = px->rrs[r][s][b];
     free(px);
     return R;
+}
+
+/* Every pixel of a grid [n_rows * n_cols] smoothed with radius r. */
+void orc_py_smooth_grid(const float *grid, int n_rows, int n_cols, int r, float spval,
+                        float *out)
+{
+    for (int i = 0; i < n_rows; i++)
+        for (int j = 0; j < n_cols; j++)
+            out[(long)i * n_cols + j] = orc_smoothed_index_2d(grid, i, j, n_rows, n_cols, r, spval);
 }

@@ -6,6 +6,7 @@
  * with its (2 n_spatial - 1)^2 neighbours, one depth and one bottom per region and one set of
  * water constituents per scene shared by all regions (model/samodel.c:1808-1815 layout with
  * N_r > 1).  Restated here:
+ *   orc_smoothed_index_2d  smoothed_index_2d, model/common.c:232-276 (float arithmetic)
  *   orc_region_gather      extract_Rrs_data, model/samodel.c:2957-3027 (n_sigma = 0)
  *   orc_region_objective   samodel_error, model/samodel.c:2432-2759, every term
  *   orc_region_start_step  model/samodel.c:2246-2354 with region means
@@ -15,13 +16,35 @@
  * Pinning (tests/test_oracle_region.py): with one region every function reduces bit for bit
  * to the pinned single-region oracle (orc_objective, orc_start_step, orc_optimise); the
  * continuity penalties against hand-computed values; the gather on hand-built grids (edges,
- * clamped duplicates, no-data neighbours, the origin's index).
+ * clamped duplicates, no-data neighbours, the origin's index); the smoothing bit for bit
+ * against the reference's own compiled common.c (oracle/_ref).
  */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "photic_oracle.h"
+
+/* The (2r - 1)^2 box mean around (i, j) with clamp-to-edge indexing, no-data values skipped,
+ * all no-data giving the no-data value, in float as model/common.c:232-276 computes it;
+ * grid [n_rows * n_cols] row-major. */
+float orc_smoothed_index_2d(const float *grid, int i, int j, int n_rows, int n_cols, int r,
+                            float spval)
+{
+    float sum = 0.0f, nobs = 0.0f;
+    for (int ii = 1 - r; ii < r; ii++) {
+        int iii = i + ii < 0 ? 0 : (i + ii > n_rows - 1 ? n_rows - 1 : i + ii);
+        for (int jj = 1 - r; jj < r; jj++) {
+            int jjj = j + jj < 0 ? 0 : (j + jj > n_cols - 1 ? n_cols - 1 : j + jj);
+            float v = grid[(long)iii * n_cols + jjj];
+            if (!orc_approx_equal(v, spval, 1.0e-6f)) {
+                sum += v;
+                nobs += 1.0f;
+            }
+        }
+    }
+    return nobs < 0.5f ? spval : sum / nobs;
+}
 
 int orc_region_n_params(const orc_region_pixel *px)
 {

@@ -214,6 +214,8 @@ typedef struct {
     long n_evals;
 } orc_region_result;
 
+float orc_smoothed_index_2d(const float *grid, int i, int j, int n_rows, int n_cols, int r,
+                            float spval);
 int orc_region_n_params(const orc_region_pixel *px);
 int orc_region_gather(orc_region_pixel *px, const float *rrs, const float *nodata, int nd,
                       int nb, int n_rows, int n_cols, int i, int j, int n_spatial);

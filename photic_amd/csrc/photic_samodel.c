@@ -6,7 +6,9 @@
  * depth-start group, inverted on the GPU -- one scene by photic_invert_h, several (the
  * reference's N_s = nscenes run, model/samodel.c:1808-1815) by photic_invert_stack_h with
  * each scene's sun and view angles -- and the outputs scattered back into the caller's grids
- * with the reference's defaults and sign conventions. */
+ * with the reference's defaults and sign conventions.  With n_spatial > 1 each pixel is
+ * inverted jointly with its (2 n_spatial - 1)^2 neighbours (model/samodel.c:2957-3027) by
+ * photic_invert_regions, on the smoothed band grids held on the device. */
 #include "photic_samodel.h"
 
 #include <math.h>
@@ -131,6 +133,77 @@ static int sm_run_group(const photic_cfg *cfg, sm_group *g, int nd, const double
     return rc;
 }
 
+/* n_spatial > 1: the smoothed grids of every (scene, band) go to the device once; each depth-
+ * start group's pixel list is inverted with its neighbourhoods by photic_invert_regions. */
+static int sm_run_regions(const photic_cfg *cfg, const float *d_grid, const float *nodata,
+                          int nrows, int ncols, int nd, const double *sec_sun,
+                          const double *sec_view, int n_spatial, sm_group *g, float **depth,
+                          float **model_error, float **bottom_albedo, float **bottom_sand,
+                          float **bottom_seagrass, float **bottom_coral, float **K_min,
+                          float **bottom_type, float **index_optical_depth)
+{
+    if (g->n == 0) return PHOTIC_OK;
+    const long n = g->n;
+    int64_t *d_idx = NULL;
+    float *d_h = NULL, *d_out = NULL;
+    int *d_st = NULL;
+    int64_t *ix = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int rc = ix ? PHOTIC_OK : PHOTIC_ENOMEM;
+    for (long q = 0; !rc && q < n; q++) ix[q] = (int64_t)g->idx[q];
+    if (!rc) rc = photic_device_malloc((void **)&d_idx, sizeof(int64_t) * (size_t)n);
+    if (!rc && g->hstart) rc = photic_device_malloc((void **)&d_h, sizeof(float) * (size_t)n);
+    if (!rc) rc = photic_device_malloc((void **)&d_out, sizeof(float) * 9 * (size_t)n);
+    if (!rc) rc = photic_device_malloc((void **)&d_st, sizeof(int) * (size_t)n);
+    if (!rc) rc = photic_memcpy_htod(d_idx, ix, sizeof(int64_t) * (size_t)n);
+    if (!rc && d_h) rc = photic_memcpy_htod(d_h, g->hstart, sizeof(float) * (size_t)n);
+    if (!rc) {
+        photic_outputs o;
+        memset(&o, 0, sizeof(o));
+        o.depth = d_out;
+        o.rrs_error = d_out + n;
+        o.albedo = d_out + 2 * n;
+        o.k_min = d_out + 3 * n;
+        o.iod = d_out + 4 * n;
+        o.bottom_type = d_out + 5 * n;
+        o.b_pct = d_out + 6 * n;
+        o.status = d_st;
+        rc = photic_invert_regions(cfg, nrows, ncols, nd, d_grid, nodata, sec_sun, sec_view,
+                                   n_spatial, n, d_idx, d_h, &o, NULL, NULL);
+        if (!rc) rc = photic_synchronize();
+    }
+    float *h = NULL;
+    int *st = NULL;
+    if (!rc) {
+        h = (float *)malloc(sizeof(float) * 9 * (size_t)n);
+        st = (int *)malloc(sizeof(int) * (size_t)n);
+        if (!h || !st) rc = PHOTIC_ENOMEM;
+    }
+    if (!rc) rc = photic_memcpy_dtoh(h, d_out, sizeof(float) * 9 * (size_t)n);
+    if (!rc) rc = photic_memcpy_dtoh(st, d_st, sizeof(int) * (size_t)n);
+    for (long p = 0; !rc && p < n; p++) {
+        if (st[p] < 0) continue; /* no region left: the defaults stay */
+        long k = g->idx[p];
+        int i = (int)(k / ncols), j = (int)(k % ncols);
+        depth[i][j] = -h[p]; /* negative below the surface, model/samodel.c:1486-1490 */
+        model_error[i][j] = h[n + p];
+        bottom_albedo[i][j] = h[2 * n + p];
+        K_min[i][j] = h[3 * n + p];
+        index_optical_depth[i][j] = h[4 * n + p];
+        bottom_type[i][j] = h[5 * n + p];
+        bottom_sand[i][j] = h[6 * n + p];
+        bottom_seagrass[i][j] = h[7 * n + p];
+        bottom_coral[i][j] = h[8 * n + p];
+    }
+    free(h);
+    free(st);
+    free(ix);
+    if (d_idx) photic_device_free(d_idx);
+    if (d_h) photic_device_free(d_h);
+    if (d_out) photic_device_free(d_out);
+    if (d_st) photic_device_free(d_st);
+    return rc;
+}
+
 int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridded_data[],
                    int *scene_indexes, int nscenes, int empirical_depth_present,
                    struct photic_geogrid empirical_depths, int n_smoothing_radius,
@@ -164,10 +237,12 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
         for (int b = 0; same_bands && b < nb && b < 4; b++)
             if (t->wavelengths[b] != sc->wavelengths[b]) same_bands = 0;
     }
-    if (nscenes > 8 || n_spatial > 1 || nb != 4 || n_bottoms < 1 || n_bottoms > 3 ||
-        !same_bands) {
+    const int nsr = n_spatial < 1 ? 1 : n_spatial;
+    const int n_par = (2 * nsr - 1) * (2 * nsr - 1) * (1 + 2 * n_bottoms) + 3 * nscenes;
+    if (nscenes > 8 || n_spatial < 0 || nb != 4 || n_bottoms < 1 || n_bottoms > 3 ||
+        !same_bands || n_par > 128) {
         fprintf(stderr, "photic samodel: supported are 1..8 scenes of one grid with the same "
-                        "4 bands, one region (n_spatial <= 1) and 1..3 bottoms (got %d scenes, "
+                        "4 bands, 1..3 bottoms and at most 128 parameters (got %d scenes, "
                         "n_spatial %d, %d bands, %d bottoms%s); outputs left at their "
                         "defaults\n",
                 nscenes, n_spatial, nb, n_bottoms, same_bands ? "" : ", differing bands");
@@ -223,7 +298,9 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
                     if (sm_feq(refl, g->nodata_value, 1.0e-6f) || refl < 0.0f) bad = 1;
                     float s = r > 1 ? sm_smoothed(g->array, i, j, nrows, ncols, r, g->nodata_value)
                                     : refl;
-                    if (sm_feq(s, g->nodata_value, 1.0e-6f)) bad = 1; /* region dropped */
+                    /* one region: a smoothed no-data value drops it (with several, the
+                     * region gather on the device decides) */
+                    if (n_spatial <= 1 && sm_feq(s, g->nodata_value, 1.0e-6f)) bad = 1;
                     v[4 * d + b] = s;
                 }
             }
@@ -248,17 +325,45 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
             memmove(gr[k].rrs + (long)q * gr[k].n, gr[k].rrs + (long)q * npix,
                     sizeof(float) * (size_t)gr[k].n);
 
+    /* n_spatial > 1: every (scene, band) grid, smoothed, and its no-data value, for the
+     * neighbourhood gather on the device */
+    float *grid = NULL, *d_grid = NULL, nod[8 * 4];
+    if (!rc && n_spatial > 1) {
+        grid = (float *)malloc(sizeof(float) * 4 * (size_t)nd * (size_t)(npix ? npix : 1));
+        if (!grid) rc = PHOTIC_ENOMEM;
+        for (int d = 0; !rc && d < nd; d++)
+            for (int b = 0; b < 4; b++) {
+                const struct photic_geogrid *g =
+                    &gridded_data[scene_data[scene_indexes[d]].band_indexes[b]];
+                float *dst = grid + (long)(4 * d + b) * npix;
+                nod[4 * d + b] = g->nodata_value;
+                for (int i = 0; i < nrows; i++)
+                    for (int j = 0; j < ncols; j++)
+                        dst[(long)i * ncols + j] =
+                            r > 1 ? sm_smoothed(g->array, i, j, nrows, ncols, r, g->nodata_value)
+                                  : g->array[i][j];
+            }
+        if (!rc) rc = photic_device_malloc((void **)&d_grid, sizeof(float) * 4 * nd * npix);
+        if (!rc) rc = photic_memcpy_htod(d_grid, grid, sizeof(float) * 4 * nd * npix);
+    }
     for (int k = 0; k < 3 && !rc; k++) {
         photic_cfg c = cfg;
         if (k == 2) {
             c.n_bottoms = 1;
             c.bottom_idx[0] = 0; /* sand */
         }
-        rc = sm_run_group(&c, &gr[k], nd, sec_sun, sec_view, ncols, depth, model_error,
-                          bottom_albedo, bottom_sand,
-                          bottom_seagrass, bottom_coral, K_min, bottom_type,
-                          index_optical_depth);
+        if (n_spatial > 1)
+            rc = sm_run_regions(&c, d_grid, nod, nrows, ncols, nd, sec_sun, sec_view, n_spatial,
+                                &gr[k], depth, model_error, bottom_albedo, bottom_sand,
+                                bottom_seagrass, bottom_coral, K_min, bottom_type,
+                                index_optical_depth);
+        else
+            rc = sm_run_group(&c, &gr[k], nd, sec_sun, sec_view, ncols, depth, model_error,
+                              bottom_albedo, bottom_sand, bottom_seagrass, bottom_coral, K_min,
+                              bottom_type, index_optical_depth);
     }
+    free(grid);
+    if (d_grid) photic_device_free(d_grid);
     for (int k = 0; k < 3; k++) {
         free(gr[k].idx);
         free(gr[k].rrs);

@@ -120,7 +120,7 @@ def test_samodel_rejects_unsupported():
 
 
 def _call_scenes(bands, theta_w, theta_v, n_bottoms, nodata=-9999.0, emp=None,
-                 emp_nodata=-99999.0):
+                 emp_nodata=-99999.0, n_spatial=1, smoothing=1):
     """samodel() over several scenes: bands [nd][4][nrows][ncols], one grid per (scene, band)."""
     nd, nb, nrows, ncols = bands.shape
     grids = [RowGrid(bands[d, b]) for d in range(nd) for b in range(nb)]
@@ -145,8 +145,8 @@ def _call_scenes(bands, theta_w, theta_v, n_bottoms, nodata=-9999.0, emp=None,
     outs = {k: RowGrid(np.zeros((nrows, ncols))) for k in
             ("depth", "sigma", "err", "alb", "sand", "grass", "coral", "kmin", "btype", "iod")}
     P.lib.photic_samodel.restype = C.c_int
-    rc = P.lib.photic_samodel(scenes, gd, idx, nd, int(emp is not None), empg, 1, 1,
-                              n_bottoms, *[outs[k].pp for k in ("depth", "sigma", "err", "alb",
+    rc = P.lib.photic_samodel(scenes, gd, idx, nd, int(emp is not None), empg, smoothing,
+                              n_spatial, n_bottoms, *[outs[k].pp for k in ("depth", "sigma", "err", "alb",
                                                                  "sand", "grass", "coral", "kmin",
                                                                  "btype", "iod")])
     return rc, {k: v.array() for k, v in outs.items()}
@@ -202,3 +202,76 @@ def test_samodel_dropin_several_scenes(nbot, with_emp):
         nd, nbot, " + empirical depths" if with_emp else "", ok.sum(), within.mean(),
         (depth[ok] == o_depth[ok]).mean()))
     assert (within | flat).mean() > 0.995 and within.mean() > 0.98
+
+
+@pytest.mark.parametrize("nbot,smoothing,with_emp,nd", [(3, 1, False, 1), (2, 2, True, 1),
+                                                        (1, 1, False, 2)])
+def test_samodel_dropin_regions(nbot, smoothing, with_emp, nd):
+    """samodel() with n_spatial = 2, the reference's default: every pixel inverted with its
+    3 x 3 neighbourhood of smoothed grids (model/samodel.c:2957-3027), screened on its own raw
+    values, optionally an empirical depth grid (its own depth start, sand only beyond 8 m) --
+    against the oracle's region inversion of the oracle-smoothed grids, pixel by pixel."""
+    nrows, ncols = 10, 12
+    n = nrows * ncols
+    if nd == 1:
+        rrs = O.gen(90 + nbot, n)[0][None]
+    else:
+        rrs = O.gen_stack(91 + nbot, n, n_dates=nd, p_cloud=0.0, k_min=nd)[0]
+    bands = np.ascontiguousarray(rrs.reshape(nd, 4, nrows, ncols), np.float32).copy()
+    bands[0, :, 2, 3] = -9999.0         # no data on every band of a scene
+    bands[nd - 1, 1, 4, 5] = -0.0004    # negative on one band
+    bands[0, 2, 6, 0] = -9999.0         # no data on one band, at the edge
+    tw, tv = (30.0, 41.0)[:nd], (0.0, 6.0)[:nd]
+    emp = None
+    if with_emp:
+        emp = np.random.default_rng(6).uniform(-15.0, -0.5, (nrows, ncols)).astype(np.float32)
+        emp[0, :6] = -99999.0           # no empirical depth here: all depth starts
+        emp[1, :3] = -0.5               # "very shallow": start at 1 m
+    rc, out = _call_scenes(bands, tw, tv, nbot, emp=emp, n_spatial=2, smoothing=smoothing)
+    assert rc == 0
+    ss = np.array([O.sec_deg(t) for t in tw])
+    sv = np.array([O.sec_deg(t) for t in tv])
+    raw = bands.reshape(nd, 4, n)
+    sm = np.stack([np.stack([O.smooth_grid(bands[d, b], smoothing, -9999.0) for b in range(4)])
+                   for d in range(nd)]).reshape(nd, 4, n)
+    groups = [(np.arange(n), None, nbot)]
+    if with_emp:
+        e = emp.reshape(-1)
+        has = e != -99999.0
+        h = np.where(e > -1.0, 1.0, np.abs(e)).astype(np.float32)
+        groups = [(np.nonzero(~has)[0], None, nbot), (np.nonzero(has & (h <= 8.0))[0], h, nbot),
+                  (np.nonzero(has & (h > 8.0))[0], h, 1)]
+    o_depth = np.zeros(n, np.float32)
+    o_err = np.zeros(n, np.float32)
+    o_status = np.full(n, -1, np.int32)
+    o_btype = np.full(n, -9999.0, np.float32)
+    o_kmin = np.zeros(n, np.float32)
+    o_iod = np.zeros(n, np.float32)
+    o_alb = np.zeros(n, np.float32)
+    o_sand = np.full(n, -9999.0, np.float32)
+    for idx, h, nb_ in groups:
+        if len(idx) == 0:
+            continue
+        o = O.invert_regions(sm, nrows, ncols, idx, -9999.0, ss, sv, n_spatial=2, n_bottoms=nb_,
+                             bottom_idx=(0, 1, 2), hstart_px=None if h is None else h[idx],
+                             screen_rrs=raw)
+        v = o["status"] >= 0
+        sel = idx[v]
+        o_status[idx] = o["status"]
+        o_depth[sel], o_err[sel], o_btype[sel] = o["depth"][v], o["rrs_error"][v], o["bottom_type"][v]
+        o_kmin[sel], o_iod[sel], o_alb[sel] = o["k_min"][v], o["iod"][v], o["albedo"][v]
+        o_sand[sel] = o["b_pct"][0][v]
+    ok = o_status >= 0
+    assert not ok[2 * ncols + 3] and not ok[4 * ncols + 5] and not ok[6 * ncols + 0]
+    assert np.array_equal(out["btype"].reshape(-1) != -9999.0, ok)
+    depth = -out["depth"].reshape(-1)
+    same = depth[ok] == o_depth[ok]
+    err = out["err"].reshape(-1)[ok]
+    flat = np.abs(err - o_err[ok]) <= 1e-6 * np.abs(o_err[ok])
+    print("regions drop-in: %d bottoms, radius %d, %d scenes%s: %d modelled, identical depth "
+          "%.4f" % (nbot, smoothing, nd, ", empirical depths" if with_emp else "", ok.sum(),
+                    same.mean()))
+    assert np.all(same | flat) and same.mean() >= 0.98
+    for a, b in ((out["err"], o_err), (out["kmin"], o_kmin), (out["iod"], o_iod),
+                 (out["alb"], o_alb), (out["btype"], o_btype), (out["sand"], o_sand)):
+        assert np.array_equal(a.reshape(-1)[ok][same], b[ok][same])

@@ -228,3 +228,22 @@ def test_region_start_step_by_hand():
     assert np.array_equal(s[6:10], [1.0] * 4) and np.array_equal(t[6:10], [0.5] * 4)
     assert t[2:6] == pytest.approx(1.5 * s[2:6], rel=1e-15)
     assert np.array_equal(t[10:], 2.0 * s[10:])
+
+
+@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("r", [1, 2, 3])
+def test_smoothing_matches_reference(r):
+    """The oracle's smoothing against the reference's own compiled smoothed_index_2d
+    (model/common.c:232-276) at every pixel of a grid with no-data values, bit for bit."""
+    rng = np.random.default_rng(r)
+    g = rng.uniform(0.0005, 0.03, (9, 11)).astype(np.float32)
+    g[rng.random(g.shape) < 0.2] = -9999.0
+    g[0:3, 0:3] = -9999.0  # a corner with no data at all in radius 2
+    s = O.smooth_grid(g, r, -9999.0)
+    for i in range(g.shape[0]):
+        for j in range(g.shape[1]):
+            assert s[i, j] == O.ref_smoothed(g, i, j, r, -9999.0), (i, j)
+    if r == 1:
+        assert np.array_equal(s, g)
+    if r == 2:
+        assert s[0, 0] == -9999.0
