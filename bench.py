@@ -23,6 +23,10 @@ Two further workloads, each with its own JSON line (not the headline):
   --workload stream  12662 x 16077 pixels on 4 dates, 2-4 valid: MLP-seeded stack inversion of
                      a host-resident scene streamed through HBM in 2048-row tiles (pinned
                      double-buffered copies on a side stream); wall time includes the copies.
+  --workload regions 512 x 512 pixels of a spatially coherent scene (smooth parameter fields,
+                     one sun zenith), each pixel inverted with its 3 x 3 neighbourhood: the
+                     reference's default n_spatial = 2 with 3 bottoms and its own Nelder-Mead
+                     settings (reqmin 1e-2, konvge 100, kcount 5000), inputs resident in HBM.
 """
 import argparse
 import ctypes as C
@@ -98,7 +102,12 @@ def main():
     ap.add_argument("--equal-misfit-pixels", type=int, default=1 << 22)
     ap.add_argument("--train-samples", type=int, default=1 << 24)
     ap.add_argument("--train-epochs", type=int, default=2)
-    ap.add_argument("--workload", choices=("scene", "stacks", "stream"), default="scene")
+    ap.add_argument("--workload", choices=("scene", "stacks", "stream", "regions"),
+                    default="scene")
+    ap.add_argument("--region-side", type=int, default=512)
+    ap.add_argument("--region-spatial", type=int, default=2)
+    ap.add_argument("--region-bottoms", type=int, default=3)
+    ap.add_argument("--region-cpu-sample", type=int, default=4096)
     ap.add_argument("--stack-side", type=int, default=2048)
     ap.add_argument("--stack-cpu-sample", type=int, default=32768)
     ap.add_argument("--stream-cpu-sample", type=int, default=65536)
@@ -119,6 +128,8 @@ def main():
         return bench_stacks(args, P)
     if args.workload == "stream":
         return bench_stream(args, P)
+    if args.workload == "regions":
+        return bench_regions(args, P)
     if args.copy_calibration:
         nbytes = args.copy_calibration
         src, dst = P.DeviceBuffer(nbytes), P.DeviceBuffer(nbytes)
@@ -477,6 +488,132 @@ def bench_stacks(args, P):
                                 "kind": "port",
                                 "sample": "first %d stacks of the same scene, FP64 oracle, "
                                           "OpenMP dynamic schedule, %.1f s" % (ns, dt)}
+        line["gpu_over_cpu"] = line["value"] / cpx
+    P.lib.photic_stream_destroy(stream)
+    print(json.dumps(line))
+
+
+def _region_flops(n_regions, n_evals, nd=1):
+    """Algorithmic FLOPs of a region inversion: evaluations x (R nd 46 n_b + 15 nd + 6) per
+    pixel of R regions, n_b = 4 bands (the band evaluations and the per-scene terms; the
+    continuity penalties' O(R n_bottoms) comparisons are not counted, like the simplex's
+    bookkeeping)."""
+    R = n_regions.astype(np.int64)
+    return float((n_evals.astype(np.int64) * (R * nd * 46 * 4 + 15 * nd + 6)).sum())
+
+
+def bench_regions(args, P):
+    """Spatial regions: every pixel of a side x side coherent scene with its neighbourhood."""
+    side, ns, nbot = args.region_side, args.region_spatial, args.region_bottoms
+    n = side * side
+    theta = 35.0
+    sec_sun = 1.0 / np.cos(np.radians(theta))
+    cfg = P.cfg_default(WL, mode=P.MODE_NATIVE, n_bottoms=nbot, reqmin=1e-2, konvge=100,
+                        kcount=5000, watchdog_s=900.0)
+    stream = C.c_void_p()
+    P.check(P.lib.photic_stream_create(C.byref(stream)))
+    d_rrs, d_tr = P.DeviceBuffer(16 * n), P.DeviceBuffer(24 * n)
+    P.check(P.lib.photic_gen_field(C.byref(cfg), 1, side, side, theta, 2e-4, d_rrs.value,
+                                   d_tr.value, stream))
+    P.check(P.lib.photic_synchronize())
+    truth_H = d_tr.to_array(np.float32, (6, n))[0]
+    del d_tr
+    fields = ("depth", "rrs_error", "status", "n_evals", "n_starts", "bottom_type")
+
+    def step():
+        outs = P.RegionOutputs(cfg, n, 1, ns, fields=fields)
+        P.check(P.lib.photic_synchronize())
+        e0, e1 = _events()
+        t0 = time.perf_counter()
+        P.check(P.lib.photic_event_record(e0, stream))
+        P.invert_regions_device(cfg, side, side, 1, d_rrs.value, -9999.0, sec_sun, 1.0, ns, n,
+                                None, None, outs, stream=stream)
+        P.check(P.lib.photic_event_record(e1, stream))
+        P.check(P.lib.photic_synchronize())
+        wall = 1e3 * (time.perf_counter() - t0)
+        ms = C.c_float()
+        P.check(P.lib.photic_event_elapsed_ms(e0, e1, C.byref(ms)))
+        P.lib.photic_event_destroy(e0)
+        P.lib.photic_event_destroy(e1)
+        return wall, float(ms.value), outs.fetch()
+
+    for _ in range(args.warmup):
+        step()
+    walls, kms, res = [], [], None
+    for _ in range(args.steps):
+        w, k_ms, res = step()
+        walls.append(w)
+        kms.append(k_ms)
+        print("regions step: %.1f ms" % w, file=sys.stderr, flush=True)
+    ms_step, k_ms = float(np.mean(walls)), float(np.mean(kms))
+    v = res["status"] >= 0
+    dH = res["depth"][v].astype(np.float64) - truth_H[v]
+    flops = _region_flops(res["n_regions"][v], res["n_evals"][v])
+    achieved = flops / (k_ms / 1e3) / 1e12
+
+    # the same scene with one region per pixel (n_spatial = 1), for what the neighbourhood buys
+    one = P.DeviceOutputs(cfg, n, fields=("depth", "rrs_error", "status", "n_evals"))
+    e0, e1 = _events()
+    P.check(P.lib.photic_event_record(e0, stream))
+    cfg1 = P.cfg_default(WL, mode=P.MODE_NATIVE, n_bottoms=nbot, reqmin=1e-2, konvge=100,
+                         kcount=5000, watchdog_s=900.0, sec_sun=float(sec_sun))
+    P.invert_device(cfg1, n, d_rrs.value, None, None, one, stream=stream)
+    P.check(P.lib.photic_event_record(e1, stream))
+    P.check(P.lib.photic_synchronize())
+    ms1 = C.c_float()
+    P.check(P.lib.photic_event_elapsed_ms(e0, e1, C.byref(ms1)))
+    r1 = one.fetch()
+    v1 = r1["status"] >= 0
+    dH1 = r1["depth"][v1].astype(np.float64) - truth_H[v1]
+
+    def quality(depth, err, dh, evals):
+        return {"evals_per_pixel": float(evals.mean()),
+                "median_rrs_error_pct": float(np.median(err)),
+                "depth_median_abs_error_m": float(np.median(np.abs(dh))),
+                "depth_runaway_frac": float((depth > RUNAWAY_M).mean()),
+                "depth_rmse_bounded_m": float(np.sqrt(np.mean(dh[depth <= RUNAWAY_M] ** 2)))}
+
+    line = {
+        "metric": "pixels/sec inverted, spatial regions (n_spatial %d, %d bottoms), 1 MI355X"
+                  % (ns, nbot),
+        "value": n / (ms_step / 1e3), "unit": "pixels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Philox-seeded smooth parameter fields, no real imagery)",
+        "config": {"workload": "region inversion, %dx%d pixels, one scene, %dx%d neighbourhood, "
+                               "%d bottoms, reqmin 1e-2 konvge 100 kcount 5000"
+                               % (side, side, 2 * ns - 1, 2 * ns - 1, nbot),
+                   "pixels": n, "seed": 1, "noise_sigma": 2e-4, "theta_sun_deg": theta,
+                   "parallelism": "1 GPU, one wavefront per pixel from a global counter"},
+        "regions": dict(quality(res["depth"][v], res["rrs_error"][v], dH, res["n_evals"][v]),
+                        region_histogram=np.bincount(res["n_regions"][v]).tolist()),
+        "one_region": dict(quality(r1["depth"][v1], r1["rrs_error"][v1], dH1, r1["n_evals"][v1]),
+                           pixels_per_s=n / (ms1.value / 1e3)),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "ph_region_kernel<%d, %d>" % (nbot, 1 if (2 * ns - 1) ** 2 *
+                                                              (1 + 2 * nbot) + 3 <= 64 else 2),
+                     "kernel_ms": k_ms,
+                     "flop_per_eval": "R (46 x 4) + 15 + 6 per pixel of R regions",
+                     "flop_per_launch": flops},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_bindings as O
+        threads = min(16, os.cpu_count() or 1)
+        used = O.set_threads(threads)
+        ns_cpu = min(args.region_cpu_sample, n)
+        rrs_h = d_rrs.to_array(np.float32, (1, 4, n))
+        t = time.perf_counter()
+        O.invert_regions(rrs_h, side, side, np.arange(ns_cpu), -9999.0, [sec_sun], [1.0],
+                         n_spatial=ns, n_bottoms=nbot, screen=True)
+        dt = time.perf_counter() - t
+        cpx = ns_cpu / dt
+        line["cpu_baseline"] = {"value": cpx, "unit": "pixels/s", "cores": used, "kind": "port",
+                                "sample": "first %d pixels of the same scene (with their "
+                                          "neighbourhoods), FP64 oracle, OpenMP dynamic "
+                                          "schedule, %.1f s" % (ns_cpu, dt)}
         line["gpu_over_cpu"] = line["value"] / cpx
     P.lib.photic_stream_destroy(stream)
     print(json.dumps(line))

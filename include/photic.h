@@ -181,6 +181,17 @@ int photic_gen_stack(const photic_cfg *cfg, uint64_t seed, long pix0, long n, in
 int photic_gen_scene(const photic_cfg *cfg, uint64_t seed, long pix0, long n, double sigma,
                      double deep_frac, float *rrs, float *sec_sun, float *truth, void *stream);
 
+/* Spatially coherent synthetic scene of n_rows x n_cols pixels (pixel index row * n_cols + col)
+ * for the region workload: each of H, B, ln P, ln G, ln X is a smooth field over the grid (0.5
+ * plus two plane waves of amplitude 0.25, wavelengths 64-256 pixels, random directions and
+ * phases from seed) mapped into its range (H 0.5-25 m, B 0-1, P, G, X between the single-date
+ * scene's bounds), one sun zenith theta_sun_deg for the scene, view at nadir, the FP64 forward
+ * model with self-consistent Y, and N(0, sigma) noise per pixel and band.  rrs [n_bands][n];
+ * truth [6][n] (H, P, G, X, B, theta) or NULL.  This build's test data; no reference
+ * counterpart. */
+int photic_gen_field(const photic_cfg *cfg, uint64_t seed, long n_rows, long n_cols,
+                     double theta_sun_deg, double sigma, float *rrs, float *truth, void *stream);
+
 /* No-data blobs for a synthetic scene of n_rows x n_cols pixels (pixel index row * n_cols +
  * col): n_discs discs keyed by seed (Philox, counter = disc index), disc i centred at
  * (u0 n_cols, u1 n_rows) with radius r_min + (r_max - r_min) u2; every band of a pixel whose

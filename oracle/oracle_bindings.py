@@ -82,6 +82,7 @@ def lib():
         _lib.orc_py_region_optimise.restype = i
         _lib.orc_py_region_optimise.argtypes = [v, i, i, i, i, v, v, v, i, v, d, i, i, i, v, v, v]
         _lib.orc_py_smooth_grid.argtypes = [v, i, i, i, C.c_float, v]
+        _lib.orc_py_gen_field.argtypes = [C.c_uint64, l, l, d, v, i, d, v, v, v]
         _lib.orc_py_region_gather.restype = i
         _lib.orc_py_region_gather.argtypes = [v, v, i, i, i, i, i, i, i, v, v]
     return _lib
@@ -536,3 +537,17 @@ def ref_smoothed(grid, i, j, r, spval):
 def sec_deg(theta_deg):
     """1 / cos of an angle in degrees, as the reference converts scene angles."""
     return lib().orc_sec_deg(float(theta_deg))
+
+
+def gen_field(seed, n_rows, n_cols, theta=35.0, sigma=2e-4, wl=WL4):
+    """The region workload's coherent scene: rrs [nb][n] float32 (negative bands kept as
+    drawn), truth [6][n], nodata [n] uint8."""
+    nb = len(wl)
+    n = n_rows * n_cols
+    rrs = np.zeros((nb, n), np.float32)
+    truth = np.zeros((6, n))
+    nod = np.zeros(n, np.uint8)
+    w = np.asarray(wl, np.float64)
+    lib().orc_py_gen_field(seed, n_rows, n_cols, theta, w.ctypes.data, nb, sigma, rrs.ctypes.data,
+                           truth.ctypes.data, nod.ctypes.data)
+    return rrs, truth, nod

@@ -138,6 +138,48 @@ int orc_gen_pixel(uint64_t seed, uint64_t pix, const orc_bands *bands, double no
     return nodata;
 }
 
+/* A smooth field in [0, 1] for the region workload: 0.5 plus two plane waves of amplitude
+ * 0.25; wave m of field q takes direction 2 pi u0, wavelength 64 + 192 u1 pixels and phase
+ * 2 pi u2 from blocks 200 + 4q + 2m and 201 + 4q + 2m of pixel 0. */
+static double orc_field(uint64_t seed, int q, long i, long j)
+{
+    double s = 0.5;
+    for (int m = 0; m < 2; m++) {
+        double u0, u1, u2, u3;
+        orc_block_uniforms(seed, 0, 200 + 4 * (uint32_t)q + 2 * (uint32_t)m, 0, &u0, &u1);
+        orc_block_uniforms(seed, 0, 201 + 4 * (uint32_t)q + 2 * (uint32_t)m, 0, &u2, &u3);
+        double a = 6.283185307179586 * u0, L = 64.0 + 192.0 * u1, ph = 6.283185307179586 * u2;
+        s += 0.25 * sin(6.283185307179586 * ((double)i * cos(a) + (double)j * sin(a)) / L + ph);
+    }
+    return s;
+}
+
+int orc_gen_field_pixel(uint64_t seed, long i, long j, long n_cols, double theta,
+                        const orc_bands *bands, double noise_sigma, orc_truth *truth,
+                        float *rrs_out)
+{
+    const uint64_t pix = (uint64_t)(i * n_cols + j);
+    orc_truth t;
+    t.H = 0.5 + 24.5 * orc_field(seed, 0, i, j);
+    t.B = orc_field(seed, 1, i, j);
+    t.P = exp(log(0.005) + orc_field(seed, 2, i, j) * (log(0.2) - log(0.005)));
+    t.G = exp(log(0.002) + orc_field(seed, 3, i, j) * (log(0.2) - log(0.002)));
+    t.X = exp(log(0.001) + orc_field(seed, 4, i, j) * (log(0.05) - log(0.001)));
+    t.theta_sun = theta;
+    if (truth)
+        *truth = t;
+    double rrs[ORC_MAXB];
+    orc_fwd_truth(&t, bands, 20, rrs, NULL);
+    int nodata = 0;
+    for (int b = 0; b < bands->n_bands; b++) {
+        float f = (float)(rrs[b] + noise_sigma * orc_block_normal(seed, pix, 112 + (uint32_t)b, 0));
+        if (f < 0.0f)
+            nodata = 1;
+        rrs_out[b] = f;
+    }
+    return nodata;
+}
+
 /* Multi-date stacks.  Per pixel: H, B from block 0 of date 0 (as the single-date scene); per
  * date d: sun zenith (block 1), P, G, X (blocks 16+a, 48+a, 80+a) and noise (112+b) with
  * counter word 3 = d; cloud mask from block 3 of date d (clear when u >= p_cloud).  A mask

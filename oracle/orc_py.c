@@ -683,3 +683,30 @@ void orc_py_smooth_grid(const float *grid, int n_rows, int n_cols, int r, float 
         for (int j = 0; j < n_cols; j++)
             out[(long)i * n_cols + j] = orc_smoothed_index_2d(grid, i, j, n_rows, n_cols, r, spval);
 }
+
+/* The region workload's coherent scene: rrs [nb][n], truth [6][n] (H, P, G, X, B, theta) or
+ * NULL, nodata [n] (1 where a band went negative), n = n_rows * n_cols. */
+void orc_py_gen_field(uint64_t seed, long n_rows, long n_cols, double theta, const double *wl,
+                      int nb, double sigma, float *rrs, double *truth, uint8_t *nodata)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    const long n = n_rows * n_cols;
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < n; q++) {
+        orc_truth t;
+        float r[ORC_MAXB];
+        int bad = orc_gen_field_pixel(seed, q / n_cols, q % n_cols, n_cols, theta, &bands, sigma,
+                                      &t, r);
+        for (int b = 0; b < nb; b++) rrs[(long)b * n + q] = r[b];
+        if (nodata) nodata[q] = (uint8_t)bad;
+        if (truth) {
+            truth[0 * n + q] = t.H;
+            truth[1 * n + q] = t.P;
+            truth[2 * n + q] = t.G;
+            truth[3 * n + q] = t.X;
+            truth[4 * n + q] = t.B;
+            truth[5 * n + q] = t.theta_sun;
+        }
+    }
+}

@@ -241,6 +241,12 @@ int orc_gen_pixel(uint64_t seed, uint64_t pix, const orc_bands *bands, double no
                   double deep_frac, orc_truth *truth, float *rrs_out, double *Y_out);
 void orc_fwd_truth(const orc_truth *t, const orc_bands *bands, int n_fixed_point,
                    double *rrs_out, double *Y_out);
+/* Pixel (i, j) of the spatially coherent scene of the region workload: H, B, ln P, ln G, ln X
+ * smooth fields over the grid (blocks 200-219 of pixel 0), one sun zenith theta, the noise of
+ * pixel i * n_cols + j; returns 1 if the pixel is nodata. */
+int orc_gen_field_pixel(uint64_t seed, long i, long j, long n_cols, double theta,
+                        const orc_bands *bands, double noise_sigma, orc_truth *truth,
+                        float *rrs_out);
 /* 1 when pixel (row, col) of an n_rows x n_cols scene lies inside one of its n_discs no-data
  * discs (Philox keyed by seed, counter = disc index). */
 int orc_in_nodata_disc(uint64_t seed, long row, long col, long n_rows, long n_cols, int n_discs,

@@ -132,6 +132,8 @@ def _load():
                                     vp]),
         "photic_invert_stack_h": (i, [C.POINTER(Cfg), l, i, vp, vp, vp, vp, vp,
                                       C.POINTER(Outputs), vp]),
+        "photic_gen_field": (i, [C.POINTER(Cfg), C.c_uint64, l, l, C.c_double, C.c_double, vp, vp,
+                                 vp]),
         "photic_invert_regions": (i, [C.POINTER(Cfg), i, i, i, vp, vp, vp, vp, i, l, vp, vp,
                                       C.POINTER(Outputs), vp, vp]),
         "photic_autotune": (i, [C.POINTER(Cfg), l, vp, vp, vp, C.POINTER(Tuning), vp]),
@@ -410,6 +412,17 @@ def invert_regions(cfg, rrs, n_rows, n_cols, nodata, sec_sun, sec_view, n_spatia
                           d_h.value if d_h is not None else None, outs)
     check(lib.photic_synchronize())
     return outs.fetch()
+
+
+def gen_field(cfg, seed, n_rows, n_cols, theta=35.0, sigma=2e-4):
+    """The region workload's coherent scene on the GPU: (rrs [nb][n], truth [6][n])."""
+    nb = cfg.n_bands
+    n = n_rows * n_cols
+    d_rrs, d_tr = DeviceBuffer(4 * nb * n), DeviceBuffer(4 * 6 * n)
+    check(lib.photic_gen_field(C.byref(cfg), seed, n_rows, n_cols, theta, sigma, d_rrs.value,
+                               d_tr.value, None))
+    check(lib.photic_synchronize())
+    return d_rrs.to_array(np.float32, (nb, n)), d_tr.to_array(np.float32, (6, n))
 
 
 def invert_stack_stream(cfg, n_rows, n_cols, rrs, sec_sun, valid, tile_rows=2048, net=None,

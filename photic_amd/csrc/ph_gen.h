@@ -14,6 +14,7 @@
 //   blocks 16+a / 48+a / 80+a: P / G / X attempt a (a < 32)  blocks 112+b: noise of band b
 //   stacks: cloud mask of date d from block 3 (redraw r from block 3 + r, r = 1..16)
 //   no-data discs: disc i (counter word 0 = i) centre from block 250, radius from block 251
+//   smooth fields: field q, wave m from blocks 200 + 4q + 2m and 201 + 4q + 2m (pixel 0)
 #pragma once
 
 #include "ph_engine.h"
@@ -121,6 +122,54 @@ PH_INLINE bool ph_gen_pixel(uint64_t seed, uint64_t pix, const PhConst& c,
     t.P = ph_trunc_lognormal(seed, pix, 16, 0, 0.005, 0.2);
     t.G = ph_trunc_lognormal(seed, pix, 48, 0, 0.002, 0.2);
     t.X = ph_trunc_lognormal(seed, pix, 80, 0, 0.001, 0.05);
+    *truth = t;
+    double rrs[PH_MAXB];
+    ph_truth_rrs(t, c, nb, rrs);
+    bool nodata = false;
+    for (int b = 0; b < nb; b++) {
+        float f = (float)(rrs[b] + sigma * ph_normal(seed, pix, 112 + (uint32_t)b, 0));
+        if (f < 0.0f) {
+            nodata = true;
+            f = (float)c.nodata;
+        }
+        rrs_out[b] = f;
+    }
+    return nodata;
+}
+
+// A smooth field in [0, 1] over the grid for the region workload: 0.5 plus two plane waves of
+// amplitude 0.25, wave m of field q with direction 2 pi u0, wavelength 64 + 192 u1 pixels and
+// phase 2 pi u2 drawn from the scene seed.
+PH_INLINE double ph_field(uint64_t seed, int q, long i, long j)
+{
+    double s = 0.5;
+    for (int m = 0; m < 2; m++) {
+        double u0, u1, u2, u3;
+        ph_uniforms(seed, 0, 200 + 4 * (uint32_t)q + 2 * (uint32_t)m, 0, &u0, &u1);
+        ph_uniforms(seed, 0, 201 + 4 * (uint32_t)q + 2 * (uint32_t)m, 0, &u2, &u3);
+        const double a = 6.283185307179586 * u0, L = 64.0 + 192.0 * u1;
+        const double ph = 6.283185307179586 * u2;
+        s += 0.25 * sin(6.283185307179586 * ((double)i * cos(a) + (double)j * sin(a)) / L + ph);
+    }
+    return s;
+}
+
+// Pixel (i, j) of a spatially coherent single-date scene: H = 0.5 + 24.5 s_H m, B = s_B,
+// P, G, X log-uniform over their bounds by s_P, s_G, s_X, one sun zenith for the scene, the
+// noise of pixel i * n_cols + j as in the single-date scene.  Returns true when a band went
+// negative after noise (written as the nodata value).
+PH_INLINE bool ph_gen_field_pixel(uint64_t seed, long i, long j, long n_cols, double theta,
+                                  const PhConst& c, int nb, double sigma, PhTruth* truth,
+                                  float* rrs_out)
+{
+    const uint64_t pix = (uint64_t)(i * n_cols + j);
+    PhTruth t;
+    t.H = 0.5 + 24.5 * ph_field(seed, 0, i, j);
+    t.B = ph_field(seed, 1, i, j);
+    t.P = exp(log(0.005) + ph_field(seed, 2, i, j) * (log(0.2) - log(0.005)));
+    t.G = exp(log(0.002) + ph_field(seed, 3, i, j) * (log(0.2) - log(0.002)));
+    t.X = exp(log(0.001) + ph_field(seed, 4, i, j) * (log(0.05) - log(0.001)));
+    t.theta = theta;
     *truth = t;
     double rrs[PH_MAXB];
     ph_truth_rrs(t, c, nb, rrs);

@@ -506,6 +506,39 @@ extern "C" int ph_launch_nodata_discs(uint64_t seed, long n_rows, long n_cols, i
     return (int)hipGetLastError();
 }
 
+__global__ void ph_gen_field_kernel(const PhConst c, uint64_t seed, long n_rows, long n_cols,
+                                    double theta, double sigma, float* rrs /*[NB][n]*/,
+                                    float* truth /*[6][n] or null*/)
+{
+    const long n = n_rows * n_cols;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    PhTruth t;
+    float r[PH_MAXB];
+    ph_gen_field_pixel(seed, q / n_cols, q % n_cols, n_cols, theta, c, c.n_bands, sigma, &t, r);
+    for (int b = 0; b < c.n_bands; b++) rrs[(long)b * n + q] = r[b];
+    if (truth) {
+        truth[0 * n + q] = (float)t.H;
+        truth[1 * n + q] = (float)t.P;
+        truth[2 * n + q] = (float)t.G;
+        truth[3 * n + q] = (float)t.X;
+        truth[4 * n + q] = (float)t.B;
+        truth[5 * n + q] = (float)t.theta;
+    }
+}
+
+extern "C" int ph_launch_gen_field(const PhConst* c, uint64_t seed, long n_rows, long n_cols,
+                                   double theta, double sigma, float* rrs, float* truth,
+                                   void* stream)
+{
+    const long n = n_rows * n_cols;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks == 0) return 0;
+    hipLaunchKernelGGL(ph_gen_field_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                       *c, seed, n_rows, n_cols, theta, sigma, rrs, truth);
+    return (int)hipGetLastError();
+}
+
 extern "C" int ph_launch_gen(const PhConst* c, uint64_t seed, long pix0, long n, double sigma,
                              double deep_frac, float* rrs, float* sec_sun, float* truth,
                              void* stream)

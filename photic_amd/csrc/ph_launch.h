@@ -130,6 +130,8 @@ int ph_launch_nodata_discs(uint64_t seed, long n_rows, long n_cols, int n_discs,
                            double r_max, int n_bands, float nodata, float *rrs, void *stream);
 int ph_launch_gen(const PhConst *c, uint64_t seed, long pix0, long n, double sigma,
                   double deep_frac, float *rrs, float *sec_sun, float *truth, void *stream);
+int ph_launch_gen_field(const PhConst *c, uint64_t seed, long n_rows, long n_cols, double theta,
+                        double sigma, float *rrs, float *truth, void *stream);
 int ph_kernel_attrs(int fp32, int n_bands, int n_bottoms, int *vgprs, int *sgprs,
                     int *scratch_bytes, int *max_threads);
 int ph_kernel_occupancy(int fp32, int n_bands, int n_bottoms, int seeded, int block,

@@ -29,13 +29,41 @@ namespace {
 
 PH_DEV int ph_uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Lane l's value of v, for a wave-uniform l.
+PH_DEV double ph_readlane(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// v[0] + v[1] + ... + v[n - 1] in that order, from LDS: the loads of a batch of eight go out
+// together, the additions stay in sequence.
+PH_DEV double ph_seq_sum(const double* v, int n)
+{
+    double s = 0.0;
+    int j = 0;
+    for (; j + 8 <= n; j += 8) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = v[j + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += t[u];
+    }
+    for (; j < n; j++) s += v[j];
+    return s;
+}
+
 template <int MC>
 struct PhReg {
-    double* g;  // the wave's LDS block
-    int o_y;    // its vertex values
+    // vertex value k lives in lane k % 64, slot k / 64
+    static constexpr int YC = MC + 1;
+    double* g; // the wave's LDS block: the simplex, [vertex][coordinate], stride N
     int li, R, N, NP, nd, origin;
-    double rq;  // reqmin * N
+    double rq; // reqmin * N
     double xs[MC], xb[MC], xe[MC], step[MC], best[MC], xs0[MC], step0[MC];
+    double yv[YC];
     double h0, tot;
     double ylo, ystar, ynewlo, del, lowest, yr;
     int ilo, ihi, icount, jcount, numres, ifault, phase, sub, kstart, nstart;
@@ -45,8 +73,49 @@ struct PhReg {
     PH_DEV bool co(int m) const { return li + 64 * m < N; }
     PH_DEV double p(int k, int m) const { return g[k * N + li + 64 * m]; }
     PH_DEV void set_p(int k, int m, double v) const { g[k * N + li + 64 * m] = v; }
-    PH_DEV double y(int k) const { return g[o_y + k]; }
-    PH_DEV void set_y(int k, double v) const { g[o_y + k] = v; }
+    // vertex value k (k wave-uniform)
+    PH_DEV double y(int k) const
+    {
+        double v = yv[0];
+#pragma unroll
+        for (int m = 1; m < YC; m++) v = ph_pick((k >> 6) == m, yv[m], v);
+        return ph_readlane(v, k & 63);
+    }
+    PH_DEV void set_y(int k, double v)
+    {
+#pragma unroll
+        for (int m = 0; m < YC; m++) yv[m] = ph_pick(k == li + 64 * m, v, yv[m]);
+    }
+    // The scan "v = y_0, at = 0; for i = 1..N: if (v < y_i) v = y_i, at = i"
+    // (asa047.c:223-231) as a wave reduction: no comparison with a NaN holds, so a NaN y_0
+    // stays; otherwise it ends at the first index of the largest value.  WANT_MAX false: the
+    // same with ">" (the lowest vertex, asa047.c:196-204).
+    template <bool WANT_MAX>
+    PH_DEV int scan(double* val) const
+    {
+        const double y0 = ph_readlane(yv[0], 0);
+        if (y0 != y0) {
+            *val = y0;
+            return 0;
+        }
+        double m = y0;
+#pragma unroll
+        for (int s = 0; s < YC; s++)
+            if (li + 64 * s <= N && (WANT_MAX ? yv[s] > m : yv[s] < m)) m = yv[s];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double o = __shfl_xor(m, off);
+            if (WANT_MAX ? o > m : o < m) m = o;
+        }
+        int at = 0;
+#pragma unroll
+        for (int s = YC - 1; s >= 0; s--) {
+            const unsigned long long b = __ballot(li + 64 * s <= N && yv[s] == m);
+            if (b) at = 64 * s + __ffsll((long long)b) - 1;
+        }
+        *val = y(at);
+        return at;
+    }
 };
 
 struct PhRegSide {
@@ -62,7 +131,7 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
                                bool side_on, PhRegSide* side)
 {
     double* g = S.g;
-    const int li = S.li, R = S.R, N = S.N, NP = S.NP, nd = S.nd;
+    const int li = S.li, R = S.R, NP = S.NP, nd = S.nd;
     const int off = R + 2 * NBOT * R;
     ph_wave_sync(); // the last evaluation's readers are done with the buffers
 #pragma unroll
@@ -105,69 +174,65 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
         if (r == R - 1) g[a.o_k + 4 * s + b] = K; // the reference keeps the last region's K
         if (side_on) g[a.o_q + j] = rB / rr;
     }
-    // each region's bottom fractions |B_k| |q_k| / sum |q| (model/samodel.c:2648-2686)
+    // lane r < R: region r's bottom fractions |B_k| |q_k| / sum |q| (model/samodel.c:2648-2686)
+    double bq[NBOT];
+#pragma unroll
+    for (int k = 0; k < NBOT; k++) bq[k] = 0.0;
     if (li < R) {
         double qs = 0.0;
 #pragma unroll
         for (int kb = 0; kb < NBOT; kb++) qs += fabs(X[R + R * NBOT + li * NBOT + kb]);
 #pragma unroll
         for (int k = 0; k < NBOT; k++)
-            g[a.o_bq + li * NBOT + k] =
-                fabs(X[R + li * NBOT + k]) * fabs(X[R + R * NBOT + li * NBOT + k]) / qs;
+            bq[k] = fabs(X[R + li * NBOT + k]) * fabs(X[R + R * NBOT + li * NBOT + k]) / qs;
     }
     ph_wave_sync();
     // lane s < nd: scene s's Kmin over the last region's bands and its K-penalty term, at the
     // origin's depth (model/samodel.c:2696-2732)
+    double kterm = 0.0, kmin_s = 1.0e4;
     if (li < nd) {
-        const double H = fabs(X[S.origin]);
+        const double H = fabs(ph_readlane(S.xe[0], S.origin));
         const double kmm = 0.275, kmn = 0.185;
-        double Kmin = 1.0e4;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const double K = g[a.o_k + 4 * li + b];
-            if (!ph_approx_equal((float)K, 0.0f, 1.0e-6f) && K < Kmin) Kmin = K;
+            if (!ph_approx_equal((float)K, 0.0f, 1.0e-6f) && K < kmin_s) kmin_s = K;
         }
         double thr = -1.0;
-        if (H < 1.0 && Kmin < kmm) thr = 0.01 + kmn;
-        else if (H < 2.0 && Kmin < 0.5 * (1.5 * kmn + 0.5 * kmm)) thr = 0.01 + 0.5 * (1.5 * kmn + 0.5 * kmm);
-        else if (H < 3.0 && Kmin < 0.5 * (1.25 * kmn + 0.75 * kmm)) thr = 0.01 + 0.5 * (1.25 * kmn + 0.75 * kmm);
-        else if (H < 4.0 && Kmin < 0.5 * (1.5 * kmn + 0.5 * kmm)) thr = 0.01 + 0.5 * (1.5 * kmn + 0.5 * kmm);
-        else if (H < 5.0 && Kmin < 0.5 * (1.75 * kmn + 0.25 * kmm)) thr = 0.01 + 0.5 * (1.75 * kmn + 0.25 * kmm);
+        if (H < 1.0 && kmin_s < kmm) thr = 0.01 + kmn;
+        else if (H < 2.0 && kmin_s < 0.5 * (1.5 * kmn + 0.5 * kmm)) thr = 0.01 + 0.5 * (1.5 * kmn + 0.5 * kmm);
+        else if (H < 3.0 && kmin_s < 0.5 * (1.25 * kmn + 0.75 * kmm)) thr = 0.01 + 0.5 * (1.25 * kmn + 0.75 * kmm);
+        else if (H < 4.0 && kmin_s < 0.5 * (1.5 * kmn + 0.5 * kmm)) thr = 0.01 + 0.5 * (1.5 * kmn + 0.5 * kmm);
+        else if (H < 5.0 && kmin_s < 0.5 * (1.75 * kmn + 0.25 * kmm)) thr = 0.01 + 0.5 * (1.75 * kmn + 0.25 * kmm);
         // the sum below adds +0.0 for a scene without a penalty: exact, since the running
         // penalty is never -0.0
-        double term = 0.0;
         if (thr > 0.0) {
-            const double t = 1.0 / (0.01 + Kmin) - 1.0 / thr;
-            term = 100.0 * (t * t);
+            const double t = 1.0 / (0.01 + kmin_s) - 1.0 / thr;
+            kterm = 100.0 * (t * t);
         }
-        g[a.o_t + li] = term;
-        if (li == nd - 1) g[a.o_t + PH_MAXD] = Kmin;
     }
-    ph_wave_sync();
 
-    // ---- every lane: the misfit in the reference's order ----
-    double err = 0.0;
-    for (int j = 0; j < NP; j++) err += g[a.o_d + j];
+    // ---- the misfit in the reference's order, every lane alike ----
+    const double err = ph_seq_sum(g + a.o_d, NP);
     const double nt = (double)NP;
     const double mean = S.tot / nt;
     const double e_rrs = 100.0 * sqrt(err / nt) / mean;
-    // depth continuity (model/samodel.c:2599-2630)
+    // depth continuity (model/samodel.c:2599-2630); depth r is coordinate r, lane r's xe[0].
+    // An in-band region adds +0.0 to the running sums below: exact, they are never -0.0.
+    const double hl = fabs(S.xe[0]);
     double dm = 0.0;
-    for (int r = 0; r < R; r++) dm += fabs(X[r]);
+    for (int r = 0; r < R; r++) dm += ph_readlane(hl, r);
     dm /= (double)R;
     double thr;
     if (dm < 4.0) thr = 0.4;
     else if (dm < 8.0) thr = 0.2;
     else if (dm < 12.0) thr = 0.1;
     else thr = 0.05;
-    double ed = 0.0, nout = 0.0;
-    for (int r = 0; r < R; r++) {
-        const double h = fabs(X[r]);
-        if (h < (1.0 - thr) * dm || h > (1.0 + thr) * dm) {
-            ed += (h - dm) * (h - dm);
-            nout += 1.0;
-        }
-    }
+    const bool hout = hl < (1.0 - thr) * dm || hl > (1.0 + thr) * dm;
+    const double hdev = hout ? (hl - dm) * (hl - dm) : 0.0;
+    double ed = 0.0;
+    for (int r = 0; r < R; r++) ed += ph_readlane(hdev, r);
+    double nout = (double)__popcll(__ballot(hout && li < R));
     if (nout > 0.5) ed = 100.0 * sqrt(ed / nout) / dm;
     // bottom continuity (model/samodel.c:2632-2692)
     double bthr;
@@ -180,24 +245,21 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
 #pragma unroll
     for (int k = 0; k < NBOT; k++) {
         double bm = 0.0;
-        for (int r = 0; r < R; r++) bm += g[a.o_bq + r * NBOT + k];
+        for (int r = 0; r < R; r++) bm += ph_readlane(bq[k], r);
         bm /= (double)R;
         btot += bm;
-        for (int r = 0; r < R; r++) {
-            const double bo = g[a.o_bq + r * NBOT + k];
-            if (bo < (1.0 - bthr) * bm || bo > (1.0 + bthr) * bm) {
-                eb += (bo - bm) * (bo - bm);
-                nout += 1.0;
-            }
-        }
+        const bool bout = bq[k] < (1.0 - bthr) * bm || bq[k] > (1.0 + bthr) * bm;
+        const double bdev = bout ? (bq[k] - bm) * (bq[k] - bm) : 0.0;
+        for (int r = 0; r < R; r++) eb += ph_readlane(bdev, r);
+        nout += (double)__popcll(__ballot(bout && li < R));
     }
     if (nout > 0.5) {
         const double bm = btot / (double)NBOT;
         eb = 100.0 * sqrt(eb / nout) / bm;
     }
     double Kerr = 0.0;
-    for (int s = 0; s < nd; s++) Kerr += g[a.o_t + s];
-    const double Kmin = g[a.o_t + PH_MAXD];
+    for (int s = 0; s < nd; s++) Kerr += ph_readlane(kterm, s);
+    const double Kmin = ph_readlane(kmin_s, nd - 1);
     if (Kmin > 0.7) {
         const double t = 4.0 * (Kmin - 0.7);
         Kerr += 100.0 * (t * t);
@@ -325,7 +387,9 @@ PH_DEV void ph_reg_consume(PhReg<MC>& S, const PhConst& c, double f)
             mv_phase = PH_EXPAND;
         } else {
             int l = 0;
-            for (int i = 0; i <= N; i++) l += (f < S.y(i)) ? 1 : 0;
+#pragma unroll
+            for (int m = 0; m < PhReg<MC>::YC; m++)
+                l += __popcll(__ballot(S.li + 64 * m <= N && f < S.yv[m]));
             if (l > 1) {
                 wr = true;
                 wj = S.ihi;
@@ -450,15 +514,7 @@ PH_DEV void ph_reg_consume(PhReg<MC>& S, const PhConst& c, double f)
 
     bool iter_begin = false, nm_finish = false;
     if (act == A_AFTER_BUILD) {
-        S.ylo = S.y(0);
-        S.ilo = 0;
-        for (int i = 1; i <= N; i++) {
-            const double yi = S.y(i);
-            if (yi < S.ylo) {
-                S.ylo = yi;
-                S.ilo = i;
-            }
-        }
+        S.ilo = S.template scan<false>(&S.ylo);
         iter_begin = true;
     }
     // ---- end of an iteration: asa047.c:397-434 ----
@@ -492,23 +548,24 @@ PH_DEV void ph_reg_consume(PhReg<MC>& S, const PhConst& c, double f)
         if (c.kcount <= S.icount) {
             nm_finish = true;
         } else {
-            S.ynewlo = S.y(0);
-            S.ihi = 0;
-            for (int i = 1; i <= N; i++) {
-                const double yi = S.y(i);
-                if (S.ynewlo < yi) {
-                    S.ynewlo = yi;
-                    S.ihi = i;
-                }
-            }
+            S.ihi = S.template scan<true>(&S.ynewlo);
             // centroid without the worst vertex: the sum over vertices 0..N in order, less
             // the worst vertex's coordinate
             const int hi = S.ihi;
 #pragma unroll
             for (int m = 0; m < MC; m++)
                 if (S.co(m)) {
+                    // the column's loads go out eight at a time, the sum stays in order
                     double z = 0.0;
-                    for (int k = 0; k <= N; k++) z = z + S.p(k, m);
+                    int k = 0;
+                    for (; k + 8 <= N + 1; k += 8) {
+                        double t[8];
+#pragma unroll
+                        for (int u = 0; u < 8; u++) t[u] = S.p(k + u, m);
+#pragma unroll
+                        for (int u = 0; u < 8; u++) z = z + t[u];
+                    }
+                    for (; k <= N; k++) z = z + S.p(k, m);
                     const double phi = S.p(hi, m);
                     const double b = (z - phi) / dn;
                     const double r = b + 1.0 * (b - phi);
@@ -592,7 +649,6 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
     const PhConst& c = a.c;
     PhReg<MC> S;
     S.g = ph_reg_lds;
-    S.o_y = a.o_y;
     S.li = (int)__lane_id();
     const int li = S.li;
     const int nd = a.n_scenes;
@@ -743,6 +799,8 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
         S.best_icount = 0;
         S.best_conv = 0;
         S.phase = PH_START;
+#pragma unroll
+        for (int m = 0; m < PhReg<MC>::YC; m++) S.yv[m] = 0.0;
 
         // ---- the multi-start Nelder-Mead, one evaluation per pass ----
         PhRegSide side;

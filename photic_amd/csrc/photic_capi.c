@@ -890,6 +890,21 @@ int photic_gen_scene(const photic_cfg *cfg, uint64_t seed, long pix0, long n, do
         "scene generator");
 }
 
+int photic_gen_field(const photic_cfg *cfg, uint64_t seed, long n_rows, long n_cols,
+                     double theta_sun_deg, double sigma, float *rrs, float *truth, void *stream)
+{
+    PhConst c;
+    int rc = ph_have_device();
+    if (rc) return rc;
+    rc = ph_make_const(cfg, 1, 0, &c);
+    if (rc) return rc;
+    if (n_rows < 0 || n_cols < 0) return ph_fail(PHOTIC_EINVAL, "negative scene size");
+    if (n_rows * n_cols > 0 && !rrs) return ph_fail(PHOTIC_EINVAL, "rrs is required");
+    return ph_launch_status(ph_launch_gen_field(&c, seed, n_rows, n_cols, theta_sun_deg, sigma,
+                                                rrs, truth, stream),
+                            "field generator");
+}
+
 int photic_gen_nodata_discs(uint64_t seed, long n_rows, long n_cols, int n_discs, double r_min,
                             double r_max, int n_bands, float nodata, float *rrs, void *stream)
 {

@@ -105,3 +105,21 @@ def test_regions_pixel_list_and_unsupported():
                          -9999.0, ss, sv, n_spatial=3)
     with pytest.raises(P.PhoticError):
         P.invert_regions(P.cfg_default(), rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=2)
+
+
+def test_field_generator_matches_oracle():
+    """The region workload's coherent scene: the GPU generator against the oracle's, the
+    parameters to float rounding of libm's sin / cos / exp / log, the Rrs within one float ulp,
+    negative bands written as no-data."""
+    cfg = P.cfg_default()
+    rrs, truth = P.gen_field(cfg, 9, 96, 80, theta=35.0)
+    orrs, otruth, onod = O.gen_field(9, 96, 80, theta=35.0)
+    for k in range(6):
+        assert np.allclose(truth[k], otruth[k].astype(np.float32), rtol=2e-7, atol=0), k
+    ok = onod == 0
+    assert np.array_equal(rrs[:, ~ok] == -9999.0, orrs[:, ~ok] < 0)
+    d = np.abs(rrs[:, ok].astype(np.float64) - orrs[:, ok])
+    assert np.all(d <= np.spacing(np.abs(orrs[:, ok]))), d.max()
+    # smooth: neighbouring depths differ by well under a metre
+    H = otruth[0].reshape(96, 80)
+    assert np.abs(np.diff(H, axis=0)).max() < 1.0 and np.abs(np.diff(H, axis=1)).max() < 1.0
