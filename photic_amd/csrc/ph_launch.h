@@ -88,18 +88,20 @@ typedef struct PhStackArgs {
  * n_rows x n_cols scene, each inverted with its (2 n_spatial - 1)^2 neighbourhood, one
  * wavefront per pixel.  A wave's LDS block (lds_doubles doubles) holds the simplex
  * [(n_max + 1) n_max] from offset 0, then the arrays at the o_* offsets (in doubles):
- * o_y vertex values [n_max + 1], o_x the trial point [n_max], o_d squared residual per pair,
+ * o_x the trial point [n_max], o_d squared residual per pair,
  * o_q rB / r per pair, o_rm measured Rrs per pair, o_pw (440 / lambda)^Y per pair
- * [np_max each], o_k the last region's K [4 n_scenes], o_t K-penalty term per scene and the
- * last scene's Kmin [PH_MAXD + 1], o_bq bottom fraction per (region, bottom) [r_max n_bottoms],
- * o_m Rrs at 490 / 550 / 640 nm per (region, scene) [3 r_max n_scenes], o_pix the regions'
- * pixel indices [r_max]. */
+ * [np_max each], o_k the last region's K [4 n_scenes], o_m Rrs at 490 / 550 / 640 nm per
+ * (region, scene) [3 r_max n_scenes], o_pix the regions' pixel indices [r_max], o_bc the band
+ * coefficients [4][8] (a0, a1, a_w, b_bw, exp(-0.015 (lambda - 440)), the bottoms'
+ * reflectances), o_geo each scene's sec(sun) then sec(view) [2 PH_MAXD], o_hd each region's
+ * depth-penalty term [r_max], o_bqv and o_bd each (bottom, region)'s fraction and penalty term
+ * [n_bottoms r_max each]. */
 typedef struct PhRegionArgs {
     PhConst c;                     /* c.n_dates = n_scenes, c.n_bottoms native bottoms */
     int n_rows, n_cols, n_spatial, n_scenes;
     int r_max, n_max, np_max;      /* regions, parameters, (region, scene, band) pairs */
     int lds_doubles;
-    int o_y, o_x, o_d, o_q, o_rm, o_pw, o_k, o_t, o_bq, o_m, o_pix;
+    int o_x, o_d, o_q, o_rm, o_pw, o_k, o_m, o_pix, o_bc, o_geo, o_hd, o_bqv, o_bd;
     float nodata[PH_MAXD * 4];     /* no-data value of each (scene, band) grid */
     double sec_sun[PH_MAXD], sec_view[PH_MAXD];
     long n;                        /* pixels to invert */

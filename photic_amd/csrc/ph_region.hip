@@ -61,7 +61,8 @@ struct PhReg {
     static constexpr int YC = MC + 1;
     double* g; // the wave's LDS block: the simplex, [vertex][coordinate], stride N
     int li, R, N, NP, nd, origin;
-    double rq; // reqmin * N
+    int nd_magic; // ceil(2^16 / nd): x / nd = (x nd_magic) >> 16 for x < 1024
+    double rq;    // reqmin * N
     double xs[MC], xb[MC], xe[MC], step[MC], best[MC], xs0[MC], step0[MC];
     double yv[YC];
     double h0, tot;
@@ -140,9 +141,10 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
     ph_wave_sync();
     const double* X = g + a.o_x;
     for (int j = li; j < NP; j += 64) {
-        const int r = j / (4 * nd);
+        const int r = ((j >> 2) * S.nd_magic) >> 16; // (j / 4) / nd, exact for j < 4096
         const int s = (j >> 2) - r * nd;
         const int b = j & 3;
+        const double* bc = g + a.o_bc + 8 * b; // a0, a1, aw, bbw, eg, bottoms of band b
         const double H = fabs(X[r]);
         double Bk[NBOT], q[NBOT], qsum = 0.0;
 #pragma unroll
@@ -156,42 +158,38 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
         for (int k = 0; k < NBOT; k++) q[k] /= qsum;
         double rho = 0.0;
 #pragma unroll
-        for (int k = 0; k < NBOT; k++) rho += q[k] * Bk[k] * PH_BSEL(c.bot[c.bottom_idx[k]], b);
+        for (int k = 0; k < NBOT; k++) rho += q[k] * Bk[k] * bc[5 + k];
         const double P = 0.01 * fabs(X[off + 3 * s]);
         const double Gc = 0.01 * fabs(X[off + 1 + 3 * s]);
         const double Xc = 0.01 * fabs(X[off + 2 + 3 * s]);
-        double ss = a.sec_sun[0], sv = a.sec_view[0];
-        for (int t = 1; t < PH_MAXD; t++) {
-            ss = ph_pick(s == t, a.sec_sun[t], ss);
-            sv = ph_pick(s == t, a.sec_view[t], sv);
-        }
+        const double ss = g[a.o_geo + s], sv = g[a.o_geo + PH_MAXD + s];
         double K, rB, rr;
-        const double m = ph_band64(P, Gc, Xc, rho, H, PH_BSEL(c.a0, b), PH_BSEL(c.a1, b),
-                                   PH_BSEL(c.aw, b), PH_BSEL(c.bbw, b), PH_BSEL(c.eg, b),
+        const double m = ph_band64(P, Gc, Xc, rho, H, bc[0], bc[1], bc[2], bc[3], bc[4],
                                    g[a.o_pw + j], ss, sv, &K, &rB, &rr);
         const double d = m - g[a.o_rm + j];
         g[a.o_d + j] = d * d;
         if (r == R - 1) g[a.o_k + 4 * s + b] = K; // the reference keeps the last region's K
         if (side_on) g[a.o_q + j] = rB / rr;
     }
-    // lane r < R: region r's bottom fractions |B_k| |q_k| / sum |q| (model/samodel.c:2648-2686)
+    // lane r < R: region r's bottom fractions |B_k| |q_k| / sum |q| (model/samodel.c:2648-2686),
+    // bottom-major in LDS
     double bq[NBOT];
-#pragma unroll
-    for (int k = 0; k < NBOT; k++) bq[k] = 0.0;
     if (li < R) {
         double qs = 0.0;
 #pragma unroll
         for (int kb = 0; kb < NBOT; kb++) qs += fabs(X[R + R * NBOT + li * NBOT + kb]);
 #pragma unroll
-        for (int k = 0; k < NBOT; k++)
+        for (int k = 0; k < NBOT; k++) {
             bq[k] = fabs(X[R + li * NBOT + k]) * fabs(X[R + R * NBOT + li * NBOT + k]) / qs;
+            g[a.o_bqv + k * R + li] = bq[k];
+        }
     }
     ph_wave_sync();
     // lane s < nd: scene s's Kmin over the last region's bands and its K-penalty term, at the
     // origin's depth (model/samodel.c:2696-2732)
     double kterm = 0.0, kmin_s = 1.0e4;
     if (li < nd) {
-        const double H = fabs(ph_readlane(S.xe[0], S.origin));
+        const double H = fabs(X[S.origin]);
         const double kmm = 0.275, kmn = 0.185;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
@@ -211,51 +209,68 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
             kterm = 100.0 * (t * t);
         }
     }
-
-    // ---- the misfit in the reference's order, every lane alike ----
-    const double err = ph_seq_sum(g + a.o_d, NP);
-    const double nt = (double)NP;
-    const double mean = S.tot / nt;
-    const double e_rrs = 100.0 * sqrt(err / nt) / mean;
-    // depth continuity (model/samodel.c:2599-2630); depth r is coordinate r, lane r's xe[0].
-    // An in-band region adds +0.0 to the running sums below: exact, they are never -0.0.
-    const double hl = fabs(S.xe[0]);
+    // depth continuity (model/samodel.c:2599-2630) and the bottoms' means.  An in-band region
+    // adds +0.0 to the running sums of squares below: exact, they are never -0.0.
     double dm = 0.0;
-    for (int r = 0; r < R; r++) dm += ph_readlane(hl, r);
+    {
+        int r = 0;
+        for (; r + 8 <= R; r += 8) {
+            double t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t[u] = X[r + u];
+#pragma unroll
+            for (int u = 0; u < 8; u++) dm += fabs(t[u]);
+        }
+        for (; r < R; r++) dm += fabs(X[r]);
+    }
     dm /= (double)R;
     double thr;
     if (dm < 4.0) thr = 0.4;
     else if (dm < 8.0) thr = 0.2;
     else if (dm < 12.0) thr = 0.1;
     else thr = 0.05;
-    const bool hout = hl < (1.0 - thr) * dm || hl > (1.0 + thr) * dm;
-    const double hdev = hout ? (hl - dm) * (hl - dm) : 0.0;
-    double ed = 0.0;
-    for (int r = 0; r < R; r++) ed += ph_readlane(hdev, r);
-    double nout = (double)__popcll(__ballot(hout && li < R));
-    if (nout > 0.5) ed = 100.0 * sqrt(ed / nout) / dm;
-    // bottom continuity (model/samodel.c:2632-2692)
     double bthr;
     if (dm < 5.0) bthr = 0.25;
     else if (dm < 10.0) bthr = 0.1;
     else if (dm < 15.0) bthr = 0.05;
     else bthr = 0.01;
-    double eb = 0.0, btot = 0.0;
-    nout = 0.0;
+    double bm[NBOT], btot = 0.0;
 #pragma unroll
     for (int k = 0; k < NBOT; k++) {
-        double bm = 0.0;
-        for (int r = 0; r < R; r++) bm += ph_readlane(bq[k], r);
-        bm /= (double)R;
-        btot += bm;
-        const bool bout = bq[k] < (1.0 - bthr) * bm || bq[k] > (1.0 + bthr) * bm;
-        const double bdev = bout ? (bq[k] - bm) * (bq[k] - bm) : 0.0;
-        for (int r = 0; r < R; r++) eb += ph_readlane(bdev, r);
-        nout += (double)__popcll(__ballot(bout && li < R));
+        bm[k] = ph_seq_sum(g + a.o_bqv + k * R, R) / (double)R;
+        btot += bm[k];
     }
-    if (nout > 0.5) {
-        const double bm = btot / (double)NBOT;
-        eb = 100.0 * sqrt(eb / nout) / bm;
+    bool hout = false, bout[NBOT];
+#pragma unroll
+    for (int k = 0; k < NBOT; k++) bout[k] = false;
+    if (li < R) {
+        const double hl = fabs(X[li]);
+        hout = hl < (1.0 - thr) * dm || hl > (1.0 + thr) * dm;
+        g[a.o_hd + li] = hout ? (hl - dm) * (hl - dm) : 0.0;
+#pragma unroll
+        for (int k = 0; k < NBOT; k++) {
+            bout[k] = bq[k] < (1.0 - bthr) * bm[k] || bq[k] > (1.0 + bthr) * bm[k];
+            g[a.o_bd + k * R + li] = bout[k] ? (bq[k] - bm[k]) * (bq[k] - bm[k]) : 0.0;
+        }
+    }
+    const double hcount = (double)__popcll(__ballot(hout));
+    int nb_out = 0;
+#pragma unroll
+    for (int k = 0; k < NBOT; k++) nb_out += __popcll(__ballot(bout[k]));
+    const double bcount = (double)nb_out;
+    ph_wave_sync();
+
+    // ---- the misfit in the reference's order, every lane alike ----
+    const double err = ph_seq_sum(g + a.o_d, NP);
+    const double nt = (double)NP;
+    const double mean = S.tot / nt;
+    const double e_rrs = 100.0 * sqrt(err / nt) / mean;
+    double ed = ph_seq_sum(g + a.o_hd, R);
+    if (hcount > 0.5) ed = 100.0 * sqrt(ed / hcount) / dm;
+    double eb = ph_seq_sum(g + a.o_bd, NBOT * R);
+    if (bcount > 0.5) {
+        const double bmean = btot / (double)NBOT;
+        eb = 100.0 * sqrt(eb / bcount) / bmean;
     }
     double Kerr = 0.0;
     for (int s = 0; s < nd; s++) Kerr += ph_readlane(kterm, s);
@@ -653,9 +668,32 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
     const int li = S.li;
     const int nd = a.n_scenes;
     S.nd = nd;
+    S.nd_magic = (65536 + nd - 1) / nd;
     const long npix = (long)a.n_rows * a.n_cols;
     const unsigned long long t0 = wall_clock64();
     double* g = S.g;
+    // the launch's band coefficients ([band][a0, a1, aw, bbw, eg, bottoms]) and each scene's
+    // geometry, once per wave
+    if (li < 32) {
+        const int b = li >> 3, w = li & 7;
+        double v = 0.0;
+        for (int bb = 0; bb < 4; bb++) {
+            if (bb != b) continue;
+            v = w == 0 ? c.a0[bb] : (w == 1 ? c.a1[bb] : (w == 2 ? c.aw[bb] : (w == 3 ? c.bbw[bb] : c.eg[bb])));
+#pragma unroll
+            for (int k = 0; k < NBOT; k++)
+                if (w == 5 + k) v = c.bot[c.bottom_idx[k]][bb];
+        }
+        g[a.o_bc + li] = v;
+    }
+    if (li < 2 * PH_MAXD) {
+        double v = 0.0;
+        for (int t = 0; t < PH_MAXD; t++) {
+            if (li == t) v = a.sec_sun[t];
+            if (li == PH_MAXD + t) v = a.sec_view[t];
+        }
+        g[a.o_geo + li] = v;
+    }
 
     for (;;) {
         if (wall_clock64() - t0 > a.max_ticks) {

@@ -655,17 +655,19 @@ int photic_invert_regions(const photic_cfg *cfg, int n_rows, int n_cols, int n_s
         a.field = off;                 \
         off += ((count) + 1) & ~1;     \
     } while (0)
-    PH_TAKE(o_y, n_max + 1);
     PH_TAKE(o_x, n_max);
     PH_TAKE(o_d, a.np_max);
     PH_TAKE(o_q, a.np_max);
     PH_TAKE(o_rm, a.np_max);
     PH_TAKE(o_pw, a.np_max);
     PH_TAKE(o_k, 4 * n_scenes);
-    PH_TAKE(o_t, PH_MAXD + 1);
-    PH_TAKE(o_bq, r_max * nbot);
     PH_TAKE(o_m, 3 * r_max * n_scenes);
     PH_TAKE(o_pix, r_max);
+    PH_TAKE(o_bc, 32);
+    PH_TAKE(o_geo, 2 * PH_MAXD);
+    PH_TAKE(o_hd, r_max);
+    PH_TAKE(o_bqv, r_max * nbot);
+    PH_TAKE(o_bd, r_max * nbot);
 #undef PH_TAKE
     a.lds_doubles = off;
     for (int k = 0; k < 4 * n_scenes; k++) a.nodata[k] = nodata[k];
