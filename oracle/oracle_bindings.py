@@ -82,6 +82,9 @@ def lib():
         _lib.orc_py_region_optimise.restype = i
         _lib.orc_py_region_optimise.argtypes = [v, i, i, i, i, v, v, v, i, v, d, i, i, i, v, v, v]
         _lib.orc_py_smooth_grid.argtypes = [v, i, i, i, C.c_float, v]
+        _lib.orc_py_depth_sigma.restype = i
+        _lib.orc_py_depth_sigma.argtypes = [v, i, i, i, i, v, v, v, v, v, v, i, i, v, C.c_float,
+                                            d, i, i, v, v]
         _lib.orc_py_gen_field.argtypes = [C.c_uint64, l, l, d, v, i, d, v, v, v]
         _lib.orc_py_region_gather.restype = i
         _lib.orc_py_region_gather.argtypes = [v, v, i, i, i, i, i, i, i, v, v]
@@ -551,3 +554,28 @@ def gen_field(seed, n_rows, n_cols, theta=35.0, sigma=2e-4, wl=WL4):
     lib().orc_py_gen_field(seed, n_rows, n_cols, theta, w.ctypes.data, nb, sigma, rrs.ctypes.data,
                            truth.ctypes.data, nod.ctypes.data)
     return rrs, truth, nod
+
+
+def depth_sigma(depth, grid, n_rows, n_cols, nodata, rsig, sec_sun, sec_view, n_spatial=1,
+                n_bottoms=3, emp=None, emp_nodata=-99999.0, reqmin=1e-2, konvge=100, kcount=5000,
+                wl=WL4):
+    """The oracle's depth-error pass: depth [npix] positive (0 unmodelled), grid [nd][nb][npix]
+    smoothed, nodata / rsig [nd][nb], sec_sun / sec_view [nd], emp [npix] or None.  Returns
+    (table [n_int], sigma [npix] float32)."""
+    g = np.ascontiguousarray(grid, np.float32)
+    nd, nb, npix = g.shape
+    dp = np.ascontiguousarray(depth, np.float32).reshape(-1)
+    nod = np.ascontiguousarray(np.broadcast_to(np.asarray(nodata, np.float32), (nd, nb)))
+    rs = np.ascontiguousarray(np.broadcast_to(np.asarray(rsig, np.float64), (nd, nb)))
+    ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (nd,)))
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (nd,)))
+    em = None if emp is None else np.ascontiguousarray(emp, np.float32).reshape(-1)
+    table = np.zeros(128)
+    sigma = np.zeros(npix, np.float32)
+    w = np.asarray(wl, np.float64)
+    n_int = lib().orc_py_depth_sigma(w.ctypes.data, nb, nd, n_rows, n_cols, dp.ctypes.data,
+                                     g.ctypes.data, nod.ctypes.data, rs.ctypes.data,
+                                     ss.ctypes.data, sv.ctypes.data, n_spatial, n_bottoms,
+                                     _p(em), emp_nodata, reqmin, konvge, kcount,
+                                     table.ctypes.data, sigma.ctypes.data)
+    return table[:n_int], sigma

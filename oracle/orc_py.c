@@ -517,7 +517,7 @@ long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_c
                 bad = 1;
         }
         int R = bad ? 0 : orc_region_gather(px, rrs, nodata, nd, nb, n_rows, n_cols, i, j,
-                                            n_spatial);
+                                            n_spatial, 0.0f, NULL);
         if (n_regions) n_regions[p] = R;
         if (origin) origin[p] = px->origin;
         if (o->x)
@@ -665,7 +665,8 @@ int orc_py_region_gather(const float *rrs, const float *nodata, int nd, int nb, 
                          int n_cols, int i, int j, int n_spatial, int *origin, double *rrs_out)
 {
     orc_region_pixel *px = (orc_region_pixel *)calloc(1, sizeof(orc_region_pixel));
-    int R = orc_region_gather(px, rrs, nodata, nd, nb, n_rows, n_cols, i, j, n_spatial);
+    int R = orc_region_gather(px, rrs, nodata, nd, nb, n_rows, n_cols, i, j, n_spatial, 0.0f,
+                              NULL);
     *origin = px->origin;
     for (int r = 0; r < R; r++)
         for (int s = 0; s < nd; s++)
@@ -709,4 +710,52 @@ void orc_py_gen_field(uint64_t seed, long n_rows, long n_cols, double theta, con
             truth[5 * n + q] = t.theta_sun;
         }
     }
+}
+
+/* The depth-error pass (orc_depth_sigma) with the reference's Nelder-Mead settings given:
+ * depth [npix] positive, grid [nd][nb][npix] smoothed, nodata / rsig [nd * nb], sec_sun /
+ * sec_view [nd], emp [npix] or NULL.  table [120], sigma [npix]; returns the interval count. */
+int orc_py_depth_sigma(const double *wl, int nb, int nd, int n_rows, int n_cols,
+                       const float *depth, const float *grid, const float *nodata,
+                       const double *rsig, const double *sec_sun, const double *sec_view,
+                       int n_spatial, int n_bottoms, const float *emp, float emp_nodata,
+                       double reqmin, int konvge, int kcount, double *table, float *sigma)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    orc_nm_cfg cfg;
+    orc_default_cfg(&cfg, nd, reqmin, konvge, kcount);
+    return orc_depth_sigma(depth, n_rows, n_cols, grid, nodata, rsig, nd, &bands, sec_sun,
+                           sec_view, n_spatial, n_bottoms, emp, emp_nodata, &cfg, table, sigma);
+}
+
+/* One perturbed region inversion (the depth-error pass's sample): pixel (i, j) of grid
+ * [nd][nb][n_rows * n_cols] with n_sigma R_sigma added; returns the depth in double (0 when
+ * no region is left), x [n_params] |best|, *n_evals. */
+double orc_py_region_sample(const double *wl, int nb, int nd, int n_rows, int n_cols,
+                            const float *grid, const float *nodata, const double *rsig,
+                            const double *sec_sun, const double *sec_view, int n_spatial,
+                            int n_bottoms, int i, int j, float n_sigma, double reqmin,
+                            int konvge, int kcount, double *x, long *n_evals)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    orc_nm_cfg cfg;
+    orc_default_cfg(&cfg, nd, reqmin, konvge, kcount);
+    orc_region_pixel *px = (orc_region_pixel *)calloc(1, sizeof(orc_region_pixel));
+    orc_region_result *res = (orc_region_result *)calloc(1, sizeof(orc_region_result));
+    const int bidx[3] = {0, 1, 2};
+    double depth = 0.0;
+    int R = orc_region_gather(px, grid, nodata, nd, nb, n_rows, n_cols, i, j, n_spatial, n_sigma,
+                              rsig);
+    if (R > 0) {
+        orc_region_init(px, n_bottoms, bidx, nd, &bands, sec_sun, sec_view);
+        orc_region_optimise(px, &cfg, NULL, res);
+        depth = res->depth;
+        for (int k = 0; x && k < res->n_params; k++) x[k] = res->x[k];
+        if (n_evals) *n_evals = res->n_evals;
+    }
+    free(px);
+    free(res);
+    return depth;
 }

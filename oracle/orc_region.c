@@ -54,10 +54,11 @@ int orc_region_n_params(const orc_region_pixel *px)
 /* extract_Rrs_data (model/samodel.c:2957-3027): candidates (ii, jj) over [1 - ns, ns - 1]^2
  * in row-major order, clamped to the grid; a candidate with any no-data band (approx_equal
  * only -- negative values are kept) is dropped; the origin is the index the centre candidate
- * had when it was visited (the last such candidate at a clamped edge). */
+ * had when it was visited (the last such candidate at a clamped edge).  rsig [nd * nb] or NULL:
+ * a perturbed gather adds n_sigma R_sigma to each kept value (model/samodel.c:3003-3007). */
 int orc_region_gather(orc_region_pixel *px, const float *rrs /*[nd][nb][n_rows*n_cols]*/,
                       const float *nodata /*[nd*nb]*/, int nd, int nb, int n_rows, int n_cols,
-                      int i, int j, int n_spatial)
+                      int i, int j, int n_spatial, float n_sigma, const double *rsig)
 {
     const long npix = (long)n_rows * n_cols;
     int ns = n_spatial == 0 ? 1 : n_spatial;
@@ -73,6 +74,8 @@ int orc_region_gather(orc_region_pixel *px, const float *rrs /*[nd][nb][n_rows*n
                     float v = rrs[((long)s * nb + b) * npix + (long)iii * n_cols + jjj];
                     if (!orc_approx_equal(v, nodata[s * nb + b], 1.0e-6f)) {
                         px->rrs[kr][s][b] = (double)v;
+                        if (rsig && !orc_approx_equal(n_sigma, 0.0f, 1.0e-6f))
+                            px->rrs[kr][s][b] += n_sigma * rsig[s * nb + b];
                     } else {
                         bad = 1;
                         break;
@@ -408,4 +411,124 @@ void orc_region_optimise(orc_region_pixel *px, const orc_nm_cfg *cfg, orc_nelmin
     orc_region_outputs(px, best, res);
     res->n_evals = px->n_evals - evals0;
     free(buf);
+}
+
+/* The depth-error pass of model/samodel.c:1376-1482 as the drop-in restates it: per 0.25 m
+ * interval up to the deepest depth (at most 30 m), 128 samples, each a pixel drawn with up to
+ * sqrt(n_rows n_cols) tries (Philox4x32-10, counter (interval * 128 + sample, try, 300, 0))
+ * whose depth lies strictly inside the interval, perturbed by n_sigma R_sigma with n_sigma
+ * = +-u (sign from u_0 < 0.5, counter (.., 0, 301, 0)), inverted again -- independently, from
+ * the main pass's start rules -- and the interval's standard deviation of the sampled depths
+ * (mean rounded to float, zeros skipped, as vec_mean2_double / vec_stddev_double).  depth
+ * [npix] the positive retrieved depths (0: unmodelled); emp [npix] the empirical depths or
+ * NULL.  Writes table [n_int] and sigma [npix]; returns n_int. */
+#define ORC_SIGMA_KEY 0x0000414D47495344ull
+
+static void orc_sigma_uniforms(uint32_t c0, uint32_t c1, uint32_t c2, double *u0, double *u1)
+{
+    const uint32_t ctr[4] = {c0, c1, c2, 0};
+    const uint32_t key[2] = {(uint32_t)ORC_SIGMA_KEY, (uint32_t)(ORC_SIGMA_KEY >> 32)};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    *u0 = (double)((((uint64_t)o[0] << 32) | o[1]) >> 11) * 0x1p-53;
+    *u1 = (double)((((uint64_t)o[2] << 32) | o[3]) >> 11) * 0x1p-53;
+}
+
+int orc_depth_sigma(const float *depth, int n_rows, int n_cols, const float *grid,
+                    const float *nodata, const double *rsig, int nd, const orc_bands *bands,
+                    const double *sec_sun, const double *sec_view, int n_spatial, int n_bottoms,
+                    const float *emp, float emp_nodata, const orc_nm_cfg *nm, double *table,
+                    float *sigma)
+{
+    const long npix = (long)n_rows * n_cols;
+    const int nb = bands->n_bands;
+    float maxf = -1.0e30f;
+    for (long p = 0; p < npix; p++) {
+        if (orc_approx_equal(depth[p], 0.0f, 1.0e-4f)) continue;
+        if (depth[p] > maxf) maxf = depth[p];
+    }
+    for (long p = 0; p < npix; p++) sigma[p] = 0.0f;
+    if (!(maxf > 0.0f)) return 0;
+    double maxd = maxf > 1.0e9f ? 1.0e9 : maxf;
+    maxd = 0.25 * ((int)maxd / 0.25);
+    if (maxd > 30.0) maxd = 30.0;
+    const int n_int = (int)((int)maxd / 0.25);
+    if (n_int <= 0) return 0;
+    const int n_trials = (int)sqrt((double)n_rows * n_cols);
+    const long n_s = (long)n_int * 128;
+    double *trial = (double *)calloc((size_t)n_s, sizeof(double));
+
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long q = 0; q < n_s; q++) {
+        const double d = 0.25 * (double)(q / 128);
+        long pix = -1;
+        for (int t = 0; t < n_trials; t++) {
+            double u0, u1;
+            orc_sigma_uniforms((uint32_t)q, (uint32_t)t, 300, &u0, &u1);
+            const int i = (int)(u0 * n_rows), j = (int)(u1 * n_cols);
+            const float v = depth[(long)i * n_cols + j];
+            if (v > d && v < d + 0.25) {
+                pix = (long)i * n_cols + j;
+                break;
+            }
+        }
+        if (pix < 0) continue;
+        double us, um;
+        orc_sigma_uniforms((uint32_t)q, 0, 301, &us, &um);
+        const float n_sigma = us < 0.5 ? (float)um : -(float)um;
+        orc_nm_cfg cfg = *nm;
+        int nbot = n_bottoms, bidx[3] = {0, 1, 2};
+        if (emp) {
+            const float e = emp[pix];
+            if (orc_approx_equal(e, emp_nodata, 1.0e-6f)) continue;
+            const float h = e > -1.0f ? 1.0f : fabsf(e);
+            cfg.n_hstarts = 1;
+            cfg.hstart[0] = (double)h;
+            if (h > 8.0) nbot = 1;
+        }
+        orc_region_pixel *px = (orc_region_pixel *)calloc(1, sizeof(orc_region_pixel));
+        orc_region_result *res = (orc_region_result *)calloc(1, sizeof(orc_region_result));
+        const int R = orc_region_gather(px, grid, nodata, nd, nb, n_rows, n_cols,
+                                        (int)(pix / n_cols), (int)(pix % n_cols), n_spatial,
+                                        n_sigma, rsig);
+        if (R > 0) {
+            orc_region_init(px, nbot, bidx, nd, bands, sec_sun, sec_view);
+            orc_region_optimise(px, &cfg, NULL, res);
+            trial[q] = res->depth;
+        }
+        free(px);
+        free(res);
+    }
+    for (int k = 0; k < n_int; k++) {
+        const double *v = trial + (long)k * 128;
+        double total = 0.0, sumdev = 0.0;
+        int count = 0;
+        for (int s = 0; s < 128; s++)
+            if (!orc_approx_equal((float)v[s], 0.0f, 1.0e-4f)) {
+                total += v[s];
+                count++;
+            }
+        const float mean = count == 0 ? 0.0f : (float)(total / (double)count);
+        count = 0;
+        for (int s = 0; s < 128; s++)
+            if (!orc_approx_equal((float)v[s], 0.0f, 1.0e-4f)) {
+                sumdev += (v[s] - (double)mean) * (v[s] - (double)mean);
+                count++;
+            }
+        table[k] = count == 0 ? 0.0 : sqrt(sumdev / (double)count);
+    }
+    for (long p = 0; p < npix; p++) {
+        const float v = depth[p];
+        if (!(v > 0.0f)) continue;
+        int k = 0;
+        for (double d = 0.0; d < maxd; d += 0.25) {
+            if (v > d && v <= d + 0.25) {
+                sigma[p] = (float)table[k];
+                break;
+            }
+            k++;
+        }
+    }
+    free(trial);
+    return n_int;
 }

@@ -218,13 +218,19 @@ float orc_smoothed_index_2d(const float *grid, int i, int j, int n_rows, int n_c
                             float spval);
 int orc_region_n_params(const orc_region_pixel *px);
 int orc_region_gather(orc_region_pixel *px, const float *rrs, const float *nodata, int nd,
-                      int nb, int n_rows, int n_cols, int i, int j, int n_spatial);
+                      int nb, int n_rows, int n_cols, int i, int j, int n_spatial,
+                      float n_sigma, const double *rsig);
 void orc_region_init(orc_region_pixel *px, int n_bottoms, const int *bottom_idx, int n_dates,
                      const orc_bands *bands, const double *sec_sun, const double *sec_view);
 double orc_region_objective(double *x, void *px);
 void orc_region_start_step(const orc_region_pixel *px, double h0, double *start, double *step);
 void orc_region_optimise(orc_region_pixel *px, const orc_nm_cfg *cfg, orc_nelmin_fn nm,
                          orc_region_result *res);
+int orc_depth_sigma(const float *depth, int n_rows, int n_cols, const float *grid,
+                    const float *nodata, const double *rsig, int nd, const orc_bands *bands,
+                    const double *sec_sun, const double *sec_view, int n_spatial, int n_bottoms,
+                    const float *emp, float emp_nodata, const orc_nm_cfg *nm, double *table,
+                    float *sigma);
 
 /* --- synthetic scenes (Philox4x32-10, Salmon et al. 2011) --- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

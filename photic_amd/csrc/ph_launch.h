@@ -108,12 +108,18 @@ typedef struct PhRegionArgs {
     const float *rrs;              /* [n_scenes][4][n_rows * n_cols] */
     const int64_t *idx;            /* [n] pixel indices, or NULL: pixel q is q, screened */
     const float *hstart;           /* [n]: one depth start per pixel, or NULL: c.hstart */
+    const float *pert;             /* [n]: n_sigma per pixel (R_sigma perturbation), or NULL */
+    double rsig[PH_MAXD * 4];      /* R_sigma of each (scene, band) */
     unsigned long long *queue;     /* [0] work counter, [1] watchdog flag; zero at launch */
     unsigned long long max_ticks;
     float *depth, *rrs_error, *model_error, *albedo, *k_min, *iod, *b_pct, *bottom_type;
     double *x_out;                 /* [n_max][n] */
+    double *depth64;               /* [n]: the depth before its rounding to float, or NULL */
     int *status, *n_evals, *n_iter, *n_starts, *n_regions;
 } PhRegionArgs;
+
+struct photic_cfg;
+struct photic_outputs;
 
 #ifdef __cplusplus
 extern "C" {
@@ -176,9 +182,19 @@ int ph_launch_mlp_stack_seed(const float *y, long n, int d, const unsigned char 
 int ph_launch_mlp_stack_mean(long n, int n_dates, const unsigned char *valid, float *seed,
                              void *stream);
 
+/* photic_invert_regions with a perturbation: pert [n] (device) n_sigma per listed pixel, rsig
+ * [4 n_scenes] (host) R_sigma per (scene, band); every region's values become
+ * Rrs + n_sigma R_sigma (model/samodel.c:3003-3007); depth64 [n] (device) or NULL receives each
+ * depth in double, as the reference keeps it.  For the drop-in's depth-error pass. */
+int ph_invert_regions_pert(const struct photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
+                           const float *rrs, const float *nodata, const double *sec_sun,
+                           const double *sec_view, int n_spatial, long n, const int64_t *idx,
+                           const float *hstart, const float *pert, const double *rsig,
+                           const struct photic_outputs *out, int *n_regions, double *depth64,
+                           void *stream);
+
 /* Host side (photic_capi.c): the launch constants for a photic_cfg; exported for the test
  * suite's host build of the lane engine. */
-struct photic_cfg;
 int ph_build_const(const struct photic_cfg *cfg, int n_dates, int has_seed, PhConst *out);
 int ph_const_size(void);
 

@@ -772,6 +772,13 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
             double r4[4];
 #pragma unroll
             for (int bb = 0; bb < 4; bb++) r4[bb] = (double)a.rrs[((long)s * 4 + bb) * npix + rp];
+            // a perturbed run adds n_sigma R_sigma to every region's values after the no-data
+            // test (model/samodel.c:3003-3007)
+            if (a.pert && !ph_approx_equal(a.pert[q], 0.0f, 1.0e-6f)) {
+                const double ns_ = (double)a.pert[q];
+#pragma unroll
+                for (int bb = 0; bb < 4; bb++) r4[bb] += ns_ * a.rsig[s * 4 + bb];
+            }
             g[a.o_rm + j] = ph_band_sel(r4, b);
             double r440 = ph_rrs_at(c, r4, 0);
             const double r490 = ph_rrs_at(c, r4, 1);
@@ -885,6 +892,7 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
                 else d += H;
             }
             if (a.depth) a.depth[q] = (float)(d / (ow + ((double)R) - 1.0));
+            if (a.depth64) a.depth64[q] = d / (ow + ((double)R) - 1.0);
             if (a.rrs_error) a.rrs_error[q] = (float)side.Rrs_error;
             if (a.model_error) a.model_error[q] = (float)side.model_error;
             if (a.albedo) a.albedo[q] = (float)side.albedo;

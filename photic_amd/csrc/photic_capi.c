@@ -608,6 +608,18 @@ int photic_invert_regions(const photic_cfg *cfg, int n_rows, int n_cols, int n_s
                           const float *hstart, const photic_outputs *out, int *n_regions,
                           void *stream)
 {
+    return ph_invert_regions_pert(cfg, n_rows, n_cols, n_scenes, rrs, nodata, sec_sun, sec_view,
+                                  n_spatial, n, idx, hstart, NULL, NULL, out, n_regions, NULL,
+                                  stream);
+}
+
+int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
+                           const float *rrs, const float *nodata, const double *sec_sun,
+                           const double *sec_view, int n_spatial, long n, const int64_t *idx,
+                           const float *hstart, const float *pert, const double *rsig,
+                           const photic_outputs *out, int *n_regions, double *depth64,
+                           void *stream)
+{
     int rc = ph_have_device();
     if (rc) return rc;
     if (!cfg || !out || !nodata || !sec_sun || !sec_view)
@@ -679,6 +691,9 @@ int photic_invert_regions(const photic_cfg *cfg, int n_rows, int n_cols, int n_s
     a.rrs = rrs;
     a.idx = idx;
     a.hstart = hstart;
+    if (pert && !rsig) return ph_fail(PHOTIC_EINVAL, "a perturbation needs R_sigma");
+    a.pert = pert;
+    for (int k = 0; pert && k < 4 * n_scenes; k++) a.rsig[k] = rsig[k];
     a.depth = out->depth;
     a.rrs_error = out->rrs_error;
     a.model_error = out->model_error;
@@ -693,6 +708,7 @@ int photic_invert_regions(const photic_cfg *cfg, int n_rows, int n_cols, int n_s
     a.n_iter = out->n_iter;
     a.n_starts = out->n_starts;
     a.n_regions = n_regions;
+    a.depth64 = depth64;
 
     int dev = 0, wrate = 0, n_cu = 0, bpc = 0, max_lds = 0;
     rc = ph_hip(hipGetDevice(&dev), "hipGetDevice");

@@ -8,15 +8,19 @@
  * each scene's sun and view angles -- and the outputs scattered back into the caller's grids
  * with the reference's defaults and sign conventions.  With n_spatial > 1 each pixel is
  * inverted jointly with its (2 n_spatial - 1)^2 neighbours (model/samodel.c:2957-3027) by
- * photic_invert_regions, on the smoothed band grids held on the device. */
+ * photic_invert_regions, on the smoothed band grids held on the device.  When the caller
+ * passes a depth_sigma grid, the depth-error pass of model/samodel.c:1376-1482 fills it
+ * (sm_sigma_pass). */
 #include "photic_samodel.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "photic.h"
+#include "ph_launch.h"
 
 static int sm_feq(float a, float b, float eps)
 {
@@ -133,29 +137,32 @@ static int sm_run_group(const photic_cfg *cfg, sm_group *g, int nd, const double
     return rc;
 }
 
-/* n_spatial > 1: the smoothed grids of every (scene, band) go to the device once; each depth-
- * start group's pixel list is inverted with its neighbourhoods by photic_invert_regions. */
-static int sm_run_regions(const photic_cfg *cfg, const float *d_grid, const float *nodata,
-                          int nrows, int ncols, int nd, const double *sec_sun,
-                          const double *sec_view, int n_spatial, sm_group *g, float **depth,
-                          float **model_error, float **bottom_albedo, float **bottom_sand,
-                          float **bottom_seagrass, float **bottom_coral, float **K_min,
-                          float **bottom_type, float **index_optical_depth)
+/* Inverts the pixels idx[0..n-1] with their neighbourhoods on the smoothed grids held on the
+ * device (ph_invert_regions_pert; pert [n] host n_sigma per pixel, or NULL); h receives the
+ * host outputs [9][n] (depth, rrs_error, albedo, k_min, iod, bottom_type, b_pct[3]) and st
+ * [n] the status. */
+static int sm_invert_regions(const photic_cfg *cfg, const float *d_grid, const float *nodata,
+                             int nrows, int ncols, int nd, const double *sec_sun,
+                             const double *sec_view, int n_spatial, long n, const long *idx,
+                             const float *hstart, const float *pert, const double *rsig,
+                             float *h, int *st, double *depth64)
 {
-    if (g->n == 0) return PHOTIC_OK;
-    const long n = g->n;
     int64_t *d_idx = NULL;
-    float *d_h = NULL, *d_out = NULL;
+    float *d_h = NULL, *d_out = NULL, *d_pert = NULL;
+    double *d_d64 = NULL;
     int *d_st = NULL;
     int64_t *ix = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
     int rc = ix ? PHOTIC_OK : PHOTIC_ENOMEM;
-    for (long q = 0; !rc && q < n; q++) ix[q] = (int64_t)g->idx[q];
+    for (long q = 0; !rc && q < n; q++) ix[q] = (int64_t)idx[q];
     if (!rc) rc = photic_device_malloc((void **)&d_idx, sizeof(int64_t) * (size_t)n);
-    if (!rc && g->hstart) rc = photic_device_malloc((void **)&d_h, sizeof(float) * (size_t)n);
+    if (!rc && hstart) rc = photic_device_malloc((void **)&d_h, sizeof(float) * (size_t)n);
+    if (!rc && pert) rc = photic_device_malloc((void **)&d_pert, sizeof(float) * (size_t)n);
     if (!rc) rc = photic_device_malloc((void **)&d_out, sizeof(float) * 9 * (size_t)n);
     if (!rc) rc = photic_device_malloc((void **)&d_st, sizeof(int) * (size_t)n);
+    if (!rc && depth64) rc = photic_device_malloc((void **)&d_d64, sizeof(double) * (size_t)n);
     if (!rc) rc = photic_memcpy_htod(d_idx, ix, sizeof(int64_t) * (size_t)n);
-    if (!rc && d_h) rc = photic_memcpy_htod(d_h, g->hstart, sizeof(float) * (size_t)n);
+    if (!rc && d_h) rc = photic_memcpy_htod(d_h, hstart, sizeof(float) * (size_t)n);
+    if (!rc && d_pert) rc = photic_memcpy_htod(d_pert, pert, sizeof(float) * (size_t)n);
     if (!rc) {
         photic_outputs o;
         memset(&o, 0, sizeof(o));
@@ -167,19 +174,41 @@ static int sm_run_regions(const photic_cfg *cfg, const float *d_grid, const floa
         o.bottom_type = d_out + 5 * n;
         o.b_pct = d_out + 6 * n;
         o.status = d_st;
-        rc = photic_invert_regions(cfg, nrows, ncols, nd, d_grid, nodata, sec_sun, sec_view,
-                                   n_spatial, n, d_idx, d_h, &o, NULL, NULL);
+        rc = ph_invert_regions_pert(cfg, nrows, ncols, nd, d_grid, nodata, sec_sun, sec_view,
+                                    n_spatial, n, d_idx, d_h, d_pert, rsig, &o, NULL, d_d64,
+                                    NULL);
         if (!rc) rc = photic_synchronize();
-    }
-    float *h = NULL;
-    int *st = NULL;
-    if (!rc) {
-        h = (float *)malloc(sizeof(float) * 9 * (size_t)n);
-        st = (int *)malloc(sizeof(int) * (size_t)n);
-        if (!h || !st) rc = PHOTIC_ENOMEM;
     }
     if (!rc) rc = photic_memcpy_dtoh(h, d_out, sizeof(float) * 9 * (size_t)n);
     if (!rc) rc = photic_memcpy_dtoh(st, d_st, sizeof(int) * (size_t)n);
+    if (!rc && depth64) rc = photic_memcpy_dtoh(depth64, d_d64, sizeof(double) * (size_t)n);
+    free(ix);
+    if (d_d64) photic_device_free(d_d64);
+    if (d_idx) photic_device_free(d_idx);
+    if (d_h) photic_device_free(d_h);
+    if (d_pert) photic_device_free(d_pert);
+    if (d_out) photic_device_free(d_out);
+    if (d_st) photic_device_free(d_st);
+    return rc;
+}
+
+/* n_spatial > 1: each depth-start group's pixel list inverted with its neighbourhoods, the
+ * outputs scattered into the caller's grids. */
+static int sm_run_regions(const photic_cfg *cfg, const float *d_grid, const float *nodata,
+                          int nrows, int ncols, int nd, const double *sec_sun,
+                          const double *sec_view, int n_spatial, sm_group *g, float **depth,
+                          float **model_error, float **bottom_albedo, float **bottom_sand,
+                          float **bottom_seagrass, float **bottom_coral, float **K_min,
+                          float **bottom_type, float **index_optical_depth)
+{
+    if (g->n == 0) return PHOTIC_OK;
+    const long n = g->n;
+    float *h = (float *)malloc(sizeof(float) * 9 * (size_t)n);
+    int *st = (int *)malloc(sizeof(int) * (size_t)n);
+    int rc = (h && st) ? PHOTIC_OK : PHOTIC_ENOMEM;
+    if (!rc)
+        rc = sm_invert_regions(cfg, d_grid, nodata, nrows, ncols, nd, sec_sun, sec_view,
+                               n_spatial, n, g->idx, g->hstart, NULL, NULL, h, st, NULL);
     for (long p = 0; !rc && p < n; p++) {
         if (st[p] < 0) continue; /* no region left: the defaults stay */
         long k = g->idx[p];
@@ -196,11 +225,198 @@ static int sm_run_regions(const photic_cfg *cfg, const float *d_grid, const floa
     }
     free(h);
     free(st);
-    free(ix);
-    if (d_idx) photic_device_free(d_idx);
-    if (d_h) photic_device_free(d_h);
-    if (d_out) photic_device_free(d_out);
-    if (d_st) photic_device_free(d_st);
+    return rc;
+}
+
+/* ---- the depth-error estimate (model/samodel.c:1376-1482) ----
+ * Per 0.25 m depth interval up to the deepest retrieved depth (at most 30 m): 128 samples, each
+ * a pixel drawn at random (up to sqrt(n_rows n_cols) tries) whose depth lies in the interval,
+ * its reflectances perturbed by n_sigma R_sigma with n_sigma uniform in (-1, 1), and inverted
+ * again; the interval's sigma is the standard deviation of the sampled depths, and every
+ * modelled pixel gets its interval's sigma.  The reference draws with rand() seeded from the
+ * clock and starts each sample from the previous sample's result; here the draws come from
+ * Philox4x32-10 with a fixed key and every sample is inverted independently, from the same
+ * start rules as the main pass, in one batched call. */
+#define SM_SIGMA_SAMPLES 128
+#define SM_SIGMA_INTERVAL 0.25
+#define SM_SIGMA_KEY 0x0000414D47495344ull /* the key of the sample draws */
+
+static void sm_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                      uint32_t k1, uint32_t out[4])
+{
+    for (int rr = 0; rr < 10; rr++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* Two uniforms in [0, 1) from counter (c0, c1, c2, 0) (53 bits each). */
+static void sm_uniforms(uint32_t c0, uint32_t c1, uint32_t c2, double *u0, double *u1)
+{
+    uint32_t o[4];
+    sm_philox(c0, c1, c2, 0, (uint32_t)SM_SIGMA_KEY, (uint32_t)(SM_SIGMA_KEY >> 32), o);
+    *u0 = (double)((((uint64_t)o[0] << 32) | o[1]) >> 11) * 0x1p-53;
+    *u1 = (double)((((uint64_t)o[2] << 32) | o[3]) >> 11) * 0x1p-53;
+}
+
+static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene_data,
+                         const int *scene_indexes, int nd, int nrows, int ncols, int n_spatial,
+                         int empirical_depth_present, struct photic_geogrid empirical_depths,
+                         const double *sec_sun, const double *sec_view, const float *d_grid,
+                         const float *nodata, float **depth, float **depth_sigma)
+{
+    /* the deepest retrieved depth (the grid holds -H; unmodelled pixels 0), array_max2 */
+    float maxf = -1.0e30f;
+    for (int i = 0; i < nrows; i++)
+        for (int j = 0; j < ncols; j++) {
+            float v = -depth[i][j];
+            if (sm_feq(v, 0.0f, 1.0e-4f)) continue;
+            if (v > maxf) maxf = v;
+        }
+    if (!(maxf > 0.0f)) return PHOTIC_OK;
+    double maxd = maxf > 1.0e9f ? 1.0e9 : maxf; /* only an int's range matters below */
+    maxd = SM_SIGMA_INTERVAL * ((int)maxd / SM_SIGMA_INTERVAL);
+    if (maxd > 30.0) maxd = 30.0;
+    const int n_int = (int)((int)maxd / SM_SIGMA_INTERVAL);
+    if (n_int <= 0) return PHOTIC_OK;
+    const int n_trials = (int)sqrt((double)nrows * ncols);
+    const long n_s = (long)n_int * SM_SIGMA_SAMPLES;
+
+    long *pix = (long *)malloc(sizeof(long) * (size_t)n_s);
+    float *ns = (float *)malloc(sizeof(float) * (size_t)n_s);
+    int *grp = (int *)malloc(sizeof(int) * (size_t)n_s);
+    float *hs = (float *)malloc(sizeof(float) * (size_t)n_s);
+    double *trial = (double *)calloc((size_t)n_s, sizeof(double));
+    double *table = (double *)calloc((size_t)n_int, sizeof(double));
+    long *gidx = (long *)malloc(sizeof(long) * (size_t)n_s);
+    long *gpos = (long *)malloc(sizeof(long) * (size_t)n_s);
+    float *gns = (float *)malloc(sizeof(float) * (size_t)n_s);
+    float *ghs = (float *)malloc(sizeof(float) * (size_t)n_s);
+    float *h = (float *)malloc(sizeof(float) * 9 * (size_t)n_s);
+    int *st = (int *)malloc(sizeof(int) * (size_t)n_s);
+    double *d64 = (double *)malloc(sizeof(double) * (size_t)n_s);
+    int rc = (pix && ns && grp && hs && trial && table && gidx && gpos && gns && ghs && h && st &&
+              d64) ? PHOTIC_OK : PHOTIC_ENOMEM;
+    double rsig[8 * 4];
+    for (int d = 0; d < nd; d++)
+        for (int b = 0; b < 4; b++) rsig[4 * d + b] = scene_data[scene_indexes[d]].R_sigma[b];
+
+    /* the samples: a pixel in the interval, its n_sigma, its depth-start group */
+    for (int k = 0; !rc && k < n_int; k++) {
+        const double d = SM_SIGMA_INTERVAL * k;
+        for (int s = 0; s < SM_SIGMA_SAMPLES; s++) {
+            const long q = (long)k * SM_SIGMA_SAMPLES + s;
+            const uint32_t ctr = (uint32_t)q;
+            pix[q] = -1;
+            for (int t = 0; t < n_trials; t++) {
+                double u0, u1;
+                sm_uniforms(ctr, (uint32_t)t, 300, &u0, &u1);
+                const int i = (int)(u0 * nrows), j = (int)(u1 * ncols);
+                const float v = -depth[i][j];
+                if (v > d && v < d + SM_SIGMA_INTERVAL) {
+                    pix[q] = (long)i * ncols + j;
+                    break;
+                }
+            }
+            if (pix[q] < 0) continue;
+            double us, um;
+            sm_uniforms(ctr, 0, 301, &us, &um);
+            ns[q] = us < 0.5 ? (float)um : -(float)um;
+            grp[q] = 0;
+            if (empirical_depth_present) {
+                const long p = pix[q];
+                const float e = empirical_depths.array[p / ncols][p % ncols];
+                if (sm_feq(e, empirical_depths.nodata_value, 1.0e-6f)) {
+                    pix[q] = -1; /* the reference skips such a sample */
+                    continue;
+                }
+                hs[q] = e > -1.0f ? 1.0f : fabsf(e);
+                grp[q] = hs[q] > 8.0 ? 2 : 1;
+            }
+        }
+    }
+    /* one batched inversion per depth-start group */
+    for (int k = 0; !rc && k < 3; k++) {
+        long m = 0;
+        for (long q = 0; q < n_s; q++)
+            if (pix[q] >= 0 && grp[q] == k) {
+                gidx[m] = pix[q];
+                gpos[m] = q;
+                gns[m] = ns[q];
+                ghs[m] = hs[q];
+                m++;
+            }
+        if (m == 0) continue;
+        photic_cfg c = *cfg;
+        if (k == 2) {
+            c.n_bottoms = 1;
+            c.bottom_idx[0] = 0; /* sand */
+        }
+        rc = sm_invert_regions(&c, d_grid, nodata, nrows, ncols, nd, sec_sun, sec_view,
+                               n_spatial < 1 ? 1 : n_spatial, m, gidx, k ? ghs : NULL, gns, rsig,
+                               h, st, d64);
+        for (long p = 0; !rc && p < m; p++) trial[gpos[p]] = st[p] >= 0 ? d64[p] : 0.0;
+    }
+    /* per interval: the mean (vec_mean2_double, returned as a float) and the standard
+     * deviation (vec_stddev_double) of the sampled depths, zeros skipped */
+    for (int k = 0; !rc && k < n_int; k++) {
+        const double *v = trial + (long)k * SM_SIGMA_SAMPLES;
+        double total = 0.0, sumdev = 0.0;
+        int count = 0;
+        for (int s = 0; s < SM_SIGMA_SAMPLES; s++)
+            if (!sm_feq((float)v[s], 0.0f, 1.0e-4f)) {
+                total += v[s];
+                count++;
+            }
+        const float mean = count == 0 ? 0.0f : (float)(total / (double)count);
+        count = 0;
+        for (int s = 0; s < SM_SIGMA_SAMPLES; s++)
+            if (!sm_feq((float)v[s], 0.0f, 1.0e-4f)) {
+                const double e = v[s] - (double)mean;
+                sumdev += e * e;
+                count++;
+            }
+        table[k] = count == 0 ? 0.0 : sqrt(sumdev / (double)count);
+    }
+    /* every modelled pixel gets its interval's sigma: d < H <= d + 0.25 */
+    for (int i = 0; !rc && i < nrows; i++)
+        for (int j = 0; j < ncols; j++) {
+            const float v = -depth[i][j];
+            depth_sigma[i][j] = 0.0f;
+            if (!(v > 0.0f)) continue;
+            int k = 0;
+            for (double d = 0.0; d < maxd; d += SM_SIGMA_INTERVAL) {
+                if (v > d && v <= d + SM_SIGMA_INTERVAL) {
+                    depth_sigma[i][j] = (float)table[k];
+                    break;
+                }
+                k++;
+            }
+        }
+    free(pix);
+    free(ns);
+    free(grp);
+    free(hs);
+    free(trial);
+    free(table);
+    free(gidx);
+    free(gpos);
+    free(gns);
+    free(ghs);
+    free(h);
+    free(st);
+    free(d64);
     return rc;
 }
 
@@ -325,10 +541,10 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
             memmove(gr[k].rrs + (long)q * gr[k].n, gr[k].rrs + (long)q * npix,
                     sizeof(float) * (size_t)gr[k].n);
 
-    /* n_spatial > 1: every (scene, band) grid, smoothed, and its no-data value, for the
-     * neighbourhood gather on the device */
+    /* every (scene, band) grid, smoothed, and its no-data value, on the device for the
+     * neighbourhood gather (n_spatial > 1) and the depth-error pass */
     float *grid = NULL, *d_grid = NULL, nod[8 * 4];
-    if (!rc && n_spatial > 1) {
+    if (!rc && (n_spatial > 1 || depth_sigma)) {
         grid = (float *)malloc(sizeof(float) * 4 * (size_t)nd * (size_t)(npix ? npix : 1));
         if (!grid) rc = PHOTIC_ENOMEM;
         for (int d = 0; !rc && d < nd; d++)
@@ -362,6 +578,10 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
                               bottom_albedo, bottom_sand, bottom_seagrass, bottom_coral, K_min,
                               bottom_type, index_optical_depth);
     }
+    if (!rc && depth_sigma)
+        rc = sm_sigma_pass(&cfg, scene_data, scene_indexes, nd, nrows, ncols, n_spatial,
+                           empirical_depth_present, empirical_depths, sec_sun, sec_view, d_grid,
+                           nod, depth, depth_sigma);
     free(grid);
     if (d_grid) photic_device_free(d_grid);
     for (int k = 0; k < 3; k++) {

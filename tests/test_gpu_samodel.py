@@ -120,7 +120,7 @@ def test_samodel_rejects_unsupported():
 
 
 def _call_scenes(bands, theta_w, theta_v, n_bottoms, nodata=-9999.0, emp=None,
-                 emp_nodata=-99999.0, n_spatial=1, smoothing=1):
+                 emp_nodata=-99999.0, n_spatial=1, smoothing=1, rsigma=0.0):
     """samodel() over several scenes: bands [nd][4][nrows][ncols], one grid per (scene, band)."""
     nd, nb, nrows, ncols = bands.shape
     grids = [RowGrid(bands[d, b]) for d in range(nd) for b in range(nb)]
@@ -136,6 +136,8 @@ def _call_scenes(bands, theta_w, theta_v, n_bottoms, nodata=-9999.0, emp=None,
             sc.band_indexes[b] = d * nb + b
             sc.wavelengths[b] = wl
         sc.theta_w, sc.theta_v = theta_w[d], theta_v[d]
+        for b in range(nb):
+            sc.R_sigma[b] = rsigma
     idx = (C.c_int * nd)(*range(nd))
     empg = Geogrid()
     keep = None
@@ -275,3 +277,50 @@ def test_samodel_dropin_regions(nbot, smoothing, with_emp, nd):
     for a, b in ((out["err"], o_err), (out["kmin"], o_kmin), (out["iod"], o_iod),
                  (out["alb"], o_alb), (out["btype"], o_btype), (out["sand"], o_sand)):
         assert np.array_equal(a.reshape(-1)[ok][same], b[ok][same])
+
+
+@pytest.mark.parametrize("nbot,ns,nd,nrows,ncols,with_emp", [(2, 1, 1, 12, 16, False),
+                                                             (3, 2, 1, 6, 7, False),
+                                                             (1, 1, 2, 10, 10, True)])
+def test_samodel_depth_sigma(nbot, ns, nd, nrows, ncols, with_emp):
+    """The depth-error pass (model/samodel.c:1376-1482) of the drop-in against the oracle's
+    restatement, given the same retrieved depths: the same Philox-drawn samples, each
+    perturbed by n_sigma R_sigma and inverted again, per-interval standard deviations of the
+    sampled depths in double, written to the float grid.  Every interval within 2 %, at most
+    two intervals not bit-identical: a sample whose Nelder-Mead path splits from the
+    oracle's (the oracle takes exp / log / pow from the host C library, which misrounds about
+    one argument in 10^4; the kernels round correctly) moves its interval's sigma slightly.
+    Seen once: 1 of 126 samples of one interval, 5 mm apart, the same pixel's other 5
+    samples bit-identical."""
+    n = nrows * ncols
+    if nd == 1:
+        rrs = O.gen(120 + nbot, n)[0][None]
+    else:
+        rrs = O.gen_stack(121 + nbot, n, n_dates=nd, p_cloud=0.0, k_min=nd)[0]
+    bands = np.ascontiguousarray(rrs.reshape(nd, 4, nrows, ncols), np.float32).copy()
+    bands[0, :, 1, 2] = -9999.0
+    tw, tv = (33.0, 44.0)[:nd], (0.0, 4.0)[:nd]
+    emp = None
+    if with_emp:
+        emp = np.random.default_rng(8).uniform(-14.0, -0.5, (nrows, ncols)).astype(np.float32)
+        emp[0, :4] = -99999.0
+    rsig = 3e-4
+    rc, out = _call_scenes(bands, tw, tv, nbot, emp=emp, n_spatial=ns, rsigma=rsig)
+    assert rc == 0
+    depth = -out["depth"].reshape(-1)
+    ss = np.array([O.sec_deg(t) for t in tw])
+    sv = np.array([O.sec_deg(t) for t in tv])
+    table, sigma = O.depth_sigma(depth, bands.reshape(nd, 4, n), nrows, ncols, -9999.0,
+                                 np.full((nd, 4), rsig), ss, sv, n_spatial=ns, n_bottoms=nbot,
+                                 emp=None if emp is None else emp.reshape(-1))
+    got = out["sigma"].reshape(-1)
+    print("depth sigma: %d intervals, %d non-zero, max %.3f m" %
+          (len(table), int((table > 0).sum()), float(table.max()) if len(table) else 0.0))
+    assert len(table) > 0 and (table > 0).sum() > 0
+    assert np.allclose(got, sigma, rtol=0.02, atol=1e-6)
+    k = np.ceil(depth / 0.25).astype(np.int64) - 1  # the pixel's interval (d, d + 0.25]
+    m = (depth > 0) & (k < len(table))
+    split = np.unique(k[m & (got != sigma)])
+    print("  intervals not bit-identical: %s" % split.tolist())
+    assert len(split) <= 2
+    assert np.all(got[depth <= 0] == 0.0) and np.all(got[depth > 0] >= 0.0)
