@@ -13,11 +13,16 @@
  * where an empirical depth deeper than 8 m is given, :1824-1858 -- the multi-start of
  * :2119-2427 with the reference's settings, the outputs of :1992-2079, the grids of
  * :1120-1128 with depth negated as at :1486-1490), each pixel inverted independently on the
- * GPU.  Not carried over: the look-up table and hot start (order-dependent, sequential), the
- * depth-sigma Monte-Carlo pass (depth_sigma is written as 0) and the netCDF files the
- * reference writes as a side effect.  Supported: one region (n_spatial <= 1), one scene,
- * 4 bands; anything else prints a message and leaves the outputs at the reference's
- * defaults.  Use photic_samodel() for the status code.
+ * GPU -- with n_spatial > 1 jointly with its (2 n_spatial - 1)^2 neighbours and their
+ * continuity penalties (:2432-2759), with several scenes as one problem (:1808-1815).  When
+ * depth_sigma is not NULL the depth-error pass of :1376-1482 fills it, with Philox draws in
+ * place of the reference's clock-seeded rand() and every sample inverted independently (the
+ * reference chains each sample's start to the previous one).  Not carried over: the look-up
+ * table and hot start (order-dependent, sequential) and the netCDF files the reference writes
+ * as a side effect.  Supported: 1-8 scenes of one grid with the same 4 bands, 1-3 bottoms,
+ * at most 128 parameters per pixel ((2 n_spatial - 1)^2 (1 + 2 n_bottoms) + 3 nscenes);
+ * anything else prints a message and leaves the outputs at the reference's defaults.  Use
+ * photic_samodel() for the status code.
  */
 #ifndef PHOTIC_SAMODEL_H
 #define PHOTIC_SAMODEL_H
