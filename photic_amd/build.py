@@ -28,7 +28,7 @@ CC_FLAGS = [
 ]
 HIP_SOURCES = ["ph_kernels.hip", "ph_mlp.hip", "ph_stack.hip", "ph_stack_native.hip",
                "ph_region.hip"]
-C_SOURCES = ["photic_capi.c", "photic_samodel.c"]
+C_SOURCES = ["photic_capi.c", "photic_samodel.c", "photic_io.c"]
 
 
 def _run(cmd):

@@ -195,6 +195,9 @@ int ph_invert_regions_pert(const struct photic_cfg *cfg, int n_rows, int n_cols,
 
 /* Host side (photic_capi.c): the launch constants for a photic_cfg; exported for the test
  * suite's host build of the lane engine. */
+/* Sets photic_last_error()'s message and returns code (for the host modules beside
+ * photic_capi.c). */
+int ph_error(int code, const char *msg);
 int ph_build_const(const struct photic_cfg *cfg, int n_dates, int has_seed, PhConst *out);
 int ph_const_size(void);
 

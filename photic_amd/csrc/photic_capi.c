@@ -27,6 +27,12 @@ static _Thread_local char ph_err[512];
 
 const char *photic_last_error(void) { return ph_err; }
 
+int ph_error(int code, const char *msg)
+{
+    snprintf(ph_err, sizeof(ph_err), "%s", msg);
+    return code;
+}
+
 static int ph_fail(int code, const char *fmt, ...)
 {
     va_list ap;

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     names = set()
-    for h in ("photic.h", "photic_samodel.h"):
+    for h in sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"^[A-Za-z_][\w \t\*]*?\b(\w+)\s*\(", text, flags=re.M):
@@ -24,7 +24,8 @@ def _declared():
 def test_every_declared_function_is_exported():
     names = _declared()
     assert {"samodel", "photic_samodel", "photic_invert", "photic_invert_h", "photic_forward",
-            "photic_objective", "photic_mlp_train", "photic_band_coefficients"} <= names
+            "photic_objective", "photic_mlp_train", "photic_band_coefficients",
+            "photic_invert_regions", "photic_read_asc", "photic_write_asc"} <= names
     lib = C.CDLL(P.LIB_PATH)
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
