@@ -156,13 +156,14 @@ PH_INLINE double ph_div_const_nonneg(double x, double d, double inv)
 // forward model, one date and band
 // ---------------------------------------------------------------------------------------
 
-// FP64: model/samodel.c:2886-2944 in the reference's operation order.
-PH_INLINE double ph_band64(double P, double G, double X, double rho, double H,
-                                            double a0, double a1, double aw, double bbw,
-                                            double eg, double pw, double ss, double sv,
-                                            double* K_out, double* rB_out, double* r_out)
+// FP64: model/samodel.c:2886-2944 in the reference's operation order, with lnP = log|P|
+// given (the same value in every band of a date, so a caller evaluating several bands forms it
+// once).
+PH_INLINE double ph_band64_l(double P, double lnP, double G, double X, double rho, double H,
+                             double a0, double a1, double aw, double bbw, double eg, double pw,
+                             double ss, double sv, double* K_out, double* rB_out, double* r_out)
 {
-    double a_phi = (a0 + a1 * PH_LOG(fabs(P))) * fabs(P);
+    double a_phi = (a0 + a1 * lnP) * fabs(P);
     double a_g = fabs(G) * eg;
     double a = aw + a_phi + a_g;
     double b_p = X * pw;
@@ -184,6 +185,14 @@ PH_INLINE double ph_band64(double P, double G, double X, double rho, double H,
     *rB_out = rB;
     *r_out = r;
     return 0.5 * r / (1.0 - 1.5 * r) + 0.0;
+}
+
+PH_INLINE double ph_band64(double P, double G, double X, double rho, double H, double a0,
+                           double a1, double aw, double bbw, double eg, double pw, double ss,
+                           double sv, double* K_out, double* rB_out, double* r_out)
+{
+    return ph_band64_l(P, PH_LOG(fabs(P)), G, X, rho, H, a0, a1, aw, bbw, eg, pw, ss, sv, K_out,
+                       rB_out, r_out);
 }
 
 // Fast path: the same model with the arithmetic in FP64 (full rate on gfx950) and the
@@ -310,6 +319,7 @@ PH_INLINE double ph_objective(const double* x, const PhConst& c,
         double G = 0.01 * fabs(x[L::off + 3 * s + 1]);
         double X = 0.01 * fabs(x[L::off + 3 * s + 2]);
         if constexpr (sizeof(FT) == 8) {
+            const double lnP = PH_LOG(fabs(P)); // a_phi's log, shared by the date's bands
 #pragma unroll
             for (int b = 0; b < NB; b++) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -324,7 +334,7 @@ PH_INLINE double ph_objective(const double* x, const PhConst& c,
                     rho += qb[k] * cb.bot[t][b];
                 }
                 double K, rB, r;
-                double m = ph_band64(P, G, X, rho, H, cb.a0[b], cb.a1[b], cb.aw[b], cb.bbw[b],
+                double m = ph_band64_l(P, lnP, G, X, rho, H, cb.a0[b], cb.a1[b], cb.aw[b], cb.bbw[b],
                                      cb.eg[b], px.pw[s][b], px.ss[s], px.sv[s], &K, &rB, &r);
                 double d = m - px.rrs[s][b];
                 err += d * d;
