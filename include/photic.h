@@ -147,7 +147,8 @@ int photic_invert_stack_h(const photic_cfg *cfg, long n_pix, int n_dates, const 
  * when the caller smooths), nodata [4 n_scenes] (host) the no-data value of each band grid,
  * sec_sun and sec_view [n_scenes] (host) each scene's subsurface secants.  idx [n] (device) or
  * NULL for every pixel (n = n_rows * n_cols), each then screened by the reference's rule (any
- * band no-data or negative: not modelled, model/samodel.c:933-947).  hstart [n] (device): one
+ * band no-data or negative: not modelled, model/samodel.c:933-947); a listed index outside the
+ * scene is not modelled (status -1).  hstart [n] (device): one
  * depth start per listed pixel (an empirical depth), or NULL for the configured list.
  * Outputs per listed pixel as photic_invert: depth is the regions' depths weighted 1 and the
  * centre region sqrt(N_r), bottom percentages and type are the centre region's, IOD is over

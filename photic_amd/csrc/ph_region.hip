@@ -705,6 +705,12 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
         const long q = (long)__shfl(qq, 0);
         if (q >= a.n) break;
         const long p = a.idx ? a.idx[q] : q;
+        if (p < 0 || p >= npix) { // a listed index outside the scene: not modelled
+            if (li == 0) ph_reg_write_nodata(a, q, 0);
+            if (a.x_out)
+                for (int i = li; i < a.n_max; i += 64) a.x_out[(long)i * a.n + q] = 0.0;
+            continue;
+        }
         const int pi = (int)(p / a.n_cols), pj = (int)(p % a.n_cols);
 
         // without a pixel list, the pixel's own screen: any band of any scene no-data or

@@ -99,6 +99,11 @@ def test_regions_pixel_list_and_unsupported():
     for k in ("x", "depth", "rrs_error", "status", "n_evals", "n_regions"):
         a = full[k][..., idx]
         assert np.array_equal(a, part[k]), k
+    # indexes outside the scene are not modelled
+    bad = np.array([idx[0], -1, n_rows * n_cols, idx[1]], np.int64)
+    o = P.invert_regions(cfg, rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns, idx=bad)
+    assert list(o["status"][1:3]) == [-1, -1] and list(o["n_regions"][1:3]) == [0, 0]
+    assert o["depth"][0] == full["depth"][idx[0]] and o["depth"][3] == full["depth"][idx[1]]
     # 5 x 5 regions of 3 bottoms: 178 parameters
     with pytest.raises(P.PhoticError):
         P.invert_regions(P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=3), rrs, n_rows, n_cols,
