@@ -12,14 +12,17 @@
 //                   LDS ([vertex][coordinate], stride N);
 //   lane l          evaluates pairs l, l + 64, ... (pair j = (region, scene, band) in the
 //                   reference's loop order) of every objective evaluation;
+//   lane k % 64     holds vertex value k in register slot k / 64; the scans for the best and
+//                   worst vertex are wave reductions;
 //   every lane      runs the modified AS 47 Nelder-Mead state machine (model/asa047.c:10-502)
 //                   and the multi-start (model/samodel.c:2119-2427) on identical values.
 //
 // The trial point goes through LDS once per evaluation; the pairs' squared residuals, the last
-// region's attenuations and the per-region bottom fractions come back through LDS, and every
-// lane re-forms the misfit from them in the reference's order, so the result is the oracle's
-// bit for bit (FP64, correctly rounded transcendentals, no contraction).  Pixels come from one
-// global counter, one per wavefront.
+// region's attenuations and the per-region depth and bottom terms come back through LDS, and
+// every lane re-forms the misfit from them in the reference's order, so the result is the
+// oracle's bit for bit (FP64, correctly rounded transcendentals, no contraction).  Band
+// coefficients and scene geometry sit in small LDS tables.  Pixels come from one global
+// counter, one per wavefront.
 #include <hip/hip_runtime.h>
 
 #include "ph_grpengine.h"
@@ -122,9 +125,6 @@ struct PhReg {
 struct PhRegSide {
     double k_min, iod_sum, nobs, Rrs_error, model_error, albedo;
 };
-
-// A band coefficient for a lane-varying band b.
-#define PH_BSEL(arr, b) ph_band_sel(arr, b)
 
 // The region misfit of the point held in the lanes' xe (model/samodel.c:2432-2759).
 template <int NBOT, int MC>
