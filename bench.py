@@ -102,6 +102,10 @@ def main():
     ap.add_argument("--equal-misfit-pixels", type=int, default=1 << 22)
     ap.add_argument("--train-samples", type=int, default=1 << 24)
     ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--seed-abort", type=float, default=5.0,
+                    help="seeded runs: give a seed start up while its objective is above this "
+                         "(the early exit stops below 2.5) at its first convergence check "
+                         "(0: never)")
     ap.add_argument("--workload", choices=("scene", "stacks", "stream", "regions"),
                     default="scene")
     ap.add_argument("--region-side", type=int, default=512)
@@ -141,7 +145,7 @@ def main():
     prec = P.FP64 if args.precision == "fp64" else P.FP32
     n = args.side * args.side
     cfg = P.cfg_default(WL, reqmin=1e-10, konvge=100, kcount=2000, precision=prec,
-                        watchdog_s=900.0)
+                        watchdog_s=900.0, seed_abort_misfit=args.seed_abort)
 
     stream = C.c_void_p()
     P.check(P.lib.photic_stream_create(C.byref(stream)))
@@ -211,6 +215,7 @@ def main():
             "starts_per_pixel": float(res["n_starts"][v].mean()),
             "converged_frac": float((res["status"][v] == 0).mean()),
             "median_rrs_error_pct": float(np.median(res["rrs_error"][v])),
+            "p90_rrs_error_pct": float(np.percentile(res["rrs_error"][v], 90)),
             "depth_rmse_m": float(np.sqrt(np.mean(dH * dH))),
             "depth_median_abs_error_m": float(np.median(np.abs(dH))),
             "depth_rmse_fitted_m": float(np.sqrt(np.mean(dH[fit] ** 2))),
@@ -319,6 +324,7 @@ def main():
                             "MLP-seeded Nelder-Mead with plain fallback" if head == "seeded"
                             else "plain GPU Nelder-Mead", args.side, args.side),
             "pixels": n, "seed": 4, "noise_sigma": 2e-4,
+            "seed_abort_misfit": args.seed_abort if head == "seeded" else None,
             "parallelism": "1 GPU, persistent work-queue kernel, autotuned launch shape",
         },
         head: head_sum,

@@ -71,6 +71,11 @@ typedef struct photic_cfg {
     double watchdog_s;                       /* a kernel running longer stops (<= 0: 600 s) */
     int check_watchdog;                      /* 1: photic_invert waits for the kernel and
                                                 returns PHOTIC_EDEVICE if the watchdog fired */
+    /* Seeded single-date inversions only: a seed start whose best misfit is still above this
+     * at its first convergence check is given up (as not converged) and the depth starts
+     * follow.  0 (the default) runs every seed start to its end.  The seed start is this
+     * build's addition; the reference's depth starts are never cut short. */
+    double seed_abort_misfit;
 } photic_cfg;
 
 /* The reference's settings (model/samodel.c:2142-2147, 2216-2218): reqmin 1e-2, konvge 100,

@@ -50,6 +50,7 @@ class Cfg(C.Structure):
         ("chunk", C.c_int),
         ("watchdog_s", C.c_double),
         ("check_watchdog", C.c_int),
+        ("seed_abort_misfit", C.c_double),
     ]
 
 

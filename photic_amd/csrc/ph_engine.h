@@ -728,7 +728,7 @@ PH_INLINE void ph_nm_consume(PhNM<N>& S, const PhConst& c, double f)
         act = A_RUN_DONE;
     }
 
-    bool iter_begin = false, nm_finish = false;
+    bool iter_begin = false, nm_finish = false, give_up = false;
     if (act == A_AFTER_BUILD) {
         S.ylo = S.y[0];
         S.ilo = 0;
@@ -745,6 +745,13 @@ PH_INLINE void ph_nm_consume(PhNM<N>& S, const PhConst& c, double f)
         S.jcount--;
         if (S.jcount > 0) {
             iter_begin = true;
+        } else if (S.kstart < 0 && S.nstart == 0 && S.numres == 0 && c.seed_abort > 0.0 &&
+                   S.ylo > c.seed_abort) {
+            // a seed start still above the abort level at its first convergence check is
+            // given up, as a run that did not converge; the depth starts then run as usual
+            // (the seed start has no counterpart in the reference)
+            give_up = true;
+            nm_finish = true;
         } else if (S.icount <= c.kcount) {
             S.jcount = c.konvge;
             double z = 0.0;
@@ -789,7 +796,7 @@ PH_INLINE void ph_nm_consume(PhNM<N>& S, const PhConst& c, double f)
         ph_for<N>([&](auto i) PH_BODY { S.xb[i] = S.m.p(lo, i); });
         S.ynewlo = ph_sel(S.y, lo);
         S.yr = ph_trunc10(S.ynewlo);
-        if (c.kcount < S.icount) {
+        if (c.kcount < S.icount || give_up) {
             S.ifault = 2;
             act = A_RUN_DONE;
         } else {

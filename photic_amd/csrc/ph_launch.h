@@ -37,6 +37,7 @@ typedef struct PhConst {
     double ip_w0[5], ip_w1[5];
     double seed_rel_step[8]; /* seeded start: simplex step as a fraction of the seed value, */
     double seed_min_step[8]; /* and its floor, per class H, bottom, P, G, X              */
+    double seed_abort;       /* give a seed start up above this misfit at its first check */
 } PhConst;
 
 /* One single-date inversion launch.  Arrays are device pointers, planar per band / per

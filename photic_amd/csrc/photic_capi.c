@@ -227,6 +227,7 @@ static int ph_make_const(const photic_cfg *cfg, int n_dates, int has_seed, PhCon
         c->seed_rel_step[k] = cfg->seed_rel_step[k];
         c->seed_min_step[k] = cfg->seed_min_step[k];
     }
+    c->seed_abort = cfg->seed_abort_misfit > 0.0 ? cfg->seed_abort_misfit : 0.0;
     return PHOTIC_OK;
 }
 

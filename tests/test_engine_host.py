@@ -58,6 +58,33 @@ def test_engine_matches_oracle_seeded():
     assert np.array_equal(h["n_starts"], o["n_starts"])
 
 
+def test_engine_seed_abort_hands_over_to_the_depth_starts():
+    """cfg.seed_abort_misfit: a seed start still above the level at its first convergence check
+    is given up.  The depth starts then run as the reference's: where the plain oracle ends
+    below the level, an aborted pixel's result is the oracle's plain result bit for bit, with
+    the seed start counted as one more start; every other pixel runs as without the abort."""
+    rrs, ss, truth, nodata, _ = O.gen(5, 256)
+    seed = np.stack([truth[0] * 0.9, truth[4], 100 * truth[1], 100 * truth[2],
+                     100 * truth[3]]).astype(np.float32)
+    wrong = np.arange(seed.shape[1]) % 2 == 1  # every other pixel: its neighbour's seed
+    seed[:, wrong] = np.roll(seed, 1, axis=1)[:, wrong]
+    h = O.hostsim_invert(P.const_bytes(_cfg(seed_abort_misfit=5.0), has_seed=True), 0, rrs, ss,
+                         seed=seed)
+    full = O.hostsim_invert(P.const_bytes(_cfg(), has_seed=True), 0, rrs, ss, seed=seed)
+    o = O.invert(rrs, ss, reqmin=1e-10, konvge=100, kcount=2000)
+    valid = o["status"] >= 0
+    same = np.all(h["x"] == full["x"], axis=0) & (h["n_evals"] == full["n_evals"])
+    aborted = valid & ~same
+    sel = aborted & (o["model_error"] < 5.0)
+    print("aborted %d of %d (wrong seed %d), compared with the plain oracle %d"
+          % (aborted.sum(), valid.sum(), (aborted & wrong).sum(), sel.sum()))
+    assert sel.sum() >= 5
+    for k in ("x", "status", "n_iter", "rrs_error", "model_error"):
+        assert np.array_equal(h[k][..., sel], o[k][..., sel]), k
+    assert np.array_equal(h["n_starts"][sel], o["n_starts"][sel] + 1)
+    assert np.all(h["n_evals"][sel] > o["n_evals"][sel] - o["n_starts"][sel])
+
+
 def test_engine_small_kcount_and_restarts():
     # tiny kcount: runs end on the evaluation limit (ifault 2) and the restart branch
     rrs, ss, _, _, _ = O.gen(4, 96)

@@ -177,6 +177,38 @@ def test_invert_fp64_bit_identical_to_host_engine(mode, nbot, seeded):
         assert np.array_equal(g[k], h[k]), k
 
 
+def test_seed_abort_bit_identical_and_never_worse_than_plain():
+    """cfg.seed_abort_misfit: a seed start above the level at its first convergence check is
+    given up and the depth starts run as in the plain method.  The GPU kernel matches the host
+    build of the engine bit for bit, some pixels (the ones with a wrong seed) do abort, and no
+    pixel ends worse than the plain depth starts unless it is under the early exit (2.5), where
+    both methods stop."""
+    rrs, ss, truth, nodata, _ = O.gen(23, 2048)
+    seed = np.stack([truth[0] * 0.9, truth[4], 100 * truth[1], 100 * truth[2],
+                     100 * truth[3]]).astype(np.float32)
+    wrong = np.arange(seed.shape[1]) % 2 == 1  # every other pixel: its neighbour's seed
+    seed[:, wrong] = np.roll(seed, 1, axis=1)[:, wrong]
+    cfg = P.cfg_default(reqmin=1e-10, kcount=2000, check_watchdog=1, seed_abort_misfit=5.0)
+    g = P.invert(cfg, rrs, ss, seed=seed)
+    h = O.hostsim_invert(P.const_bytes(cfg, has_seed=True), 0, rrs, ss, seed=seed, math="cr")
+    assert np.array_equal(g["status"], h["status"])
+    assert np.array_equal(g["x"], h["x"])
+    for k in ("n_iter", "n_evals", "n_starts", "rrs_error", "model_error", "depth"):
+        assert np.array_equal(g[k], h[k]), k
+    full = P.invert(P.cfg_default(reqmin=1e-10, kcount=2000, check_watchdog=1), rrs, ss,
+                    seed=seed)
+    plain = P.invert(P.cfg_default(reqmin=1e-10, kcount=2000, check_watchdog=1), rrs, ss)
+    v = g["status"] >= 0
+    aborted = v & (g["n_evals"] != full["n_evals"])  # otherwise the run is the same
+    print("aborted %d of %d (wrong seed %d), evaluations %.1f vs %.1f without the abort"
+          % (aborted.sum(), v.sum(), (aborted & wrong).sum(), g["n_evals"][v].mean(),
+             full["n_evals"][v].mean()))
+    assert aborted.sum() > 0
+    assert np.array_equal(g["status"] >= 0, plain["status"] >= 0)
+    ok = (g["model_error"] <= plain["model_error"]) | (g["model_error"] < 2.5)
+    assert ok[v].all()
+
+
 def test_invert_fp32_statistics():
     """FP32 forward: the trajectories leave the FP64 ones once simplex values differ by less
     than the FP32 rounding of the misfit, so only distributions are compared here; the
