@@ -30,10 +30,14 @@
 #define PH_EXP exp
 #define PH_LOG log
 #define PH_POW pow
+#define PH_EXP_NB(x, slow) exp(x)
+#define PH_LOG_NB(x, slow) log(x)
 #else
 #define PH_EXP ph_exp_cr
 #define PH_LOG ph_log_cr
 #define PH_POW ph_pow_cr
+#define PH_EXP_NB ph_exp_cr_nb // the branch-free forms: *slow set where they do not apply
+#define PH_LOG_NB ph_log_cr_nb
 #endif
 
 // Every engine function is inlined into the kernel: the lane state must stay in registers,
@@ -156,12 +160,14 @@ PH_INLINE double ph_div_const_nonneg(double x, double d, double inv)
 // forward model, one date and band
 // ---------------------------------------------------------------------------------------
 
-// FP64: model/samodel.c:2886-2944 in the reference's operation order, with lnP = log|P|
-// given (the same value in every band of a date, so a caller evaluating several bands forms it
-// once).
-PH_INLINE double ph_band64_l(double P, double lnP, double G, double X, double rho, double H,
-                             double a0, double a1, double aw, double bbw, double eg, double pw,
-                             double ss, double sv, double* K_out, double* rB_out, double* r_out)
+// FP64: model/samodel.c:2886-2944 in the reference's operation order, split at its two
+// exponentials: the head gives K, r_dp and the two exponents, the tail the rest from
+// exp(exponent).  lnP = log|P| is given (the same value in every band of a date, so a caller
+// evaluating several bands forms it once).
+PH_INLINE void ph_band64_head(double P, double lnP, double G, double X, double H, double a0,
+                              double a1, double aw, double bbw, double eg, double pw, double ss,
+                              double sv, double* K_out, double* dp_out, double* argC,
+                              double* argB)
 {
     double a_phi = (a0 + a1 * lnP) * fabs(P);
     double a_g = fabs(G) * eg;
@@ -176,15 +182,33 @@ PH_INLINE double ph_band64_l(double P, double lnP, double G, double X, double rh
     double DuC = 1.03 * sqrt(1.0 + 2.4 * u);
     double DuB = 1.04 * sqrt(1.0 + 5.4 * u);
     double M = ss + DuC * sv;
-    double rC = rrs_dp * (1.0 - PH_EXP(-M * K * H));
+    *argC = -M * K * H;
     M = ss + DuB * sv;
-    const double pi = 3.141592653589793;
-    double rB = ph_div_const_nonneg(rho, pi, 1.0 / pi) * PH_EXP(-M * K * H);
-    double r = rC + rB;
+    *argB = -M * K * H;
     *K_out = K;
+    *dp_out = rrs_dp;
+}
+
+// eC = exp(argC), eB = exp(argB) of the head
+PH_INLINE double ph_band64_tail(double rho, double rrs_dp, double eC, double eB,
+                                double* rB_out, double* r_out)
+{
+    double rC = rrs_dp * (1.0 - eC);
+    const double pi = 3.141592653589793;
+    double rB = ph_div_const_nonneg(rho, pi, 1.0 / pi) * eB;
+    double r = rC + rB;
     *rB_out = rB;
     *r_out = r;
     return 0.5 * r / (1.0 - 1.5 * r) + 0.0;
+}
+
+PH_INLINE double ph_band64_l(double P, double lnP, double G, double X, double rho, double H,
+                             double a0, double a1, double aw, double bbw, double eg, double pw,
+                             double ss, double sv, double* K_out, double* rB_out, double* r_out)
+{
+    double dp, argC, argB;
+    ph_band64_head(P, lnP, G, X, H, a0, a1, aw, bbw, eg, pw, ss, sv, K_out, &dp, &argC, &argB);
+    return ph_band64_tail(rho, dp, PH_EXP(argC), PH_EXP(argB), rB_out, r_out);
 }
 
 PH_INLINE double ph_band64(double P, double G, double X, double rho, double H, double a0,
@@ -193,6 +217,25 @@ PH_INLINE double ph_band64(double P, double G, double X, double rho, double H, d
 {
     return ph_band64_l(P, PH_LOG(fabs(P)), G, X, rho, H, a0, a1, aw, bbw, eg, pw, ss, sv, K_out,
                        rB_out, r_out);
+}
+
+// ph_band64 with exp and log in their branch-free forms and one branch at the end, taken only
+// when an argument falls outside them: the same value, with the table reads of the log and of
+// both exponentials in one straight line.
+PH_INLINE double ph_band64_nb(double P, double G, double X, double rho, double H, double a0,
+                              double a1, double aw, double bbw, double eg, double pw, double ss,
+                              double sv, double* K_out, double* rB_out, double* r_out)
+{
+    bool slow = false;
+    double lnP = PH_LOG_NB(fabs(P), &slow);
+    double dp, argC, argB;
+    ph_band64_head(P, lnP, G, X, H, a0, a1, aw, bbw, eg, pw, ss, sv, K_out, &dp, &argC, &argB);
+    double eC = PH_EXP_NB(argC, &slow);
+    double eB = PH_EXP_NB(argB, &slow);
+    double m = ph_band64_tail(rho, dp, eC, eB, rB_out, r_out);
+    if (slow)
+        m = ph_band64(P, G, X, rho, H, a0, a1, aw, bbw, eg, pw, ss, sv, K_out, rB_out, r_out);
+    return m;
 }
 
 // Fast path: the same model with the arithmetic in FP64 (full rate on gfx950) and the
@@ -319,7 +362,17 @@ PH_INLINE double ph_objective(const double* x, const PhConst& c,
         double G = 0.01 * fabs(x[L::off + 3 * s + 1]);
         double X = 0.01 * fabs(x[L::off + 3 * s + 2]);
         if constexpr (sizeof(FT) == 8) {
-            const double lnP = PH_LOG(fabs(P)); // a_phi's log, shared by the date's bands
+            // The date's bands in three passes -- every band's head, the exponentials, every
+            // band's tail -- with exp and log in their branch-free forms, so that the table
+            // reads of all the date's exponentials can be in flight together.  Each value is
+            // computed by the same operations as in ph_band64_l; arguments outside the
+            // branch-free range (P zero or not finite; subnormal, overflowing or NaN exps)
+            // take the full functions.
+            bool slow = false;
+            double lnP = PH_LOG_NB(fabs(P), &slow); // a_phi's log, shared by the date's bands
+            if (slow) lnP = PH_LOG(fabs(P));
+            slow = false;
+            double rho[NB], dp[NB], aC[NB], aB[NB], eC[NB], eB[NB];
 #pragma unroll
             for (int b = 0; b < NB; b++) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -327,20 +380,36 @@ PH_INLINE double ph_objective(const double* x, const PhConst& c,
 #else
                 const PhConst& cb = c;
 #endif
-                double rho = 0.0;
+                rho[b] = 0.0;
 #pragma unroll
                 for (int k = 0; k < nbot; k++) {
                     int t = NBOT == 0 ? k : cb.bottom_idx[k];
-                    rho += qb[k] * cb.bot[t][b];
+                    rho[b] += qb[k] * cb.bot[t][b];
                 }
-                double K, rB, r;
-                double m = ph_band64_l(P, lnP, G, X, rho, H, cb.a0[b], cb.a1[b], cb.aw[b], cb.bbw[b],
-                                     cb.eg[b], px.pw[s][b], px.ss[s], px.sv[s], &K, &rB, &r);
+                ph_band64_head(P, lnP, G, X, H, cb.a0[b], cb.a1[b], cb.aw[b], cb.bbw[b],
+                               cb.eg[b], px.pw[s][b], px.ss[s], px.sv[s], &Kd[s][b], &dp[b],
+                               &aC[b], &aB[b]);
+            }
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                eC[b] = PH_EXP_NB(aC[b], &slow);
+                eB[b] = PH_EXP_NB(aB[b], &slow);
+            }
+            if (slow) {
+#pragma unroll
+                for (int b = 0; b < NB; b++) {
+                    eC[b] = PH_EXP(aC[b]);
+                    eB[b] = PH_EXP(aB[b]);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                double rB, r;
+                double m = ph_band64_tail(rho[b], dp[b], eC[b], eB[b], &rB, &r);
                 double d = m - px.rrs[s][b];
                 err += d * d;
-                Kd[s][b] = K;
                 if (SIDE) {
-                    side->K[s][b] = K;
+                    side->K[s][b] = Kd[s][b];
                     side->rrs_bot[s][b] = rB;
                     side->rrs_mod[s][b] = r;
                 }

@@ -153,8 +153,8 @@ PH_DEV double ph_grp_objective(const PhGrp<ND, NBOT>& S, const PhConst& c, doubl
 #pragma unroll
         for (int k = 0; k < NBT; k++) rho += qb[k] * S.bt[k];
         double K, rB, r;
-        const double m = ph_band64(P, Gc, X, rho, H, S.a0, S.a1, S.aw, S.bbw, S.eg, S.pw, S.ss,
-                                   S.sv, &K, &rB, &r);
+        const double m = ph_band64_nb(P, Gc, X, rho, H, S.a0, S.a1, S.aw, S.bbw, S.eg, S.pw,
+                                      S.ss, S.sv, &K, &rB, &r);
         const double d = m - S.rrs;
         S.g[L::D0 + li] = d * d;
         S.g[L::K0 + li] = K;

@@ -164,8 +164,8 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
         const double Xc = 0.01 * fabs(X[off + 2 + 3 * s]);
         const double ss = g[a.o_geo + s], sv = g[a.o_geo + PH_MAXD + s];
         double K, rB, rr;
-        const double m = ph_band64(P, Gc, Xc, rho, H, bc[0], bc[1], bc[2], bc[3], bc[4],
-                                   g[a.o_pw + j], ss, sv, &K, &rB, &rr);
+        const double m = ph_band64_nb(P, Gc, Xc, rho, H, bc[0], bc[1], bc[2], bc[3], bc[4],
+                                      g[a.o_pw + j], ss, sv, &K, &rB, &rr);
         const double d = m - g[a.o_rm + j];
         g[a.o_d + j] = d * d;
         if (r == R - 1) g[a.o_k + 4 * s + b] = K; // the reference keeps the last region's K
