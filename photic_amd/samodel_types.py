@@ -1,5 +1,6 @@
 """ctypes mirrors of the drop-in's structures (include/photic_samodel.h, which carry the
-layouts of the reference's geogrid and scene) and a helper that builds row-pointer grids."""
+layouts of the reference's geogrid and scene, model/common.h:69-218) and a helper that builds
+row-pointer grids, for Python callers of samodel() / photic_samodel()."""
 import ctypes as C
 
 import numpy as np

@@ -8,7 +8,7 @@ import pytest
 
 import _oracle as O
 import photic_amd as P
-from samodel_types import Geogrid, RowGrid, Scene
+from photic_amd.samodel_types import Geogrid, RowGrid, Scene
 
 pytestmark = pytest.mark.gpu
 

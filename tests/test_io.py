@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import photic_amd as P
-from samodel_types import RowGrid
+from photic_amd.samodel_types import RowGrid
 
 L = C.CDLL(P.LIB_PATH)
 FP = C.POINTER(C.POINTER(C.c_float))
