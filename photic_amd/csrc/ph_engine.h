@@ -80,6 +80,7 @@ struct PhPixel {
     double pw[ND][NB];        // (440/lambda)^Y, Y from the measured 440/490 ratio
     double ss[ND], sv[ND];    // sec(sun), sec(view), subsurface
     double tot;               // sum of measured Rrs in (date, band) order
+    double p0[ND];            // every depth start's P: 100 * 0.072 (Rrs490 / Rrs550)^-1.7
 };
 
 // ---------------------------------------------------------------------------------------
@@ -514,6 +515,10 @@ PH_INLINE bool ph_pixel_init(PhPixel<FT, ND, NB>& px, const PhConst& c,
         if (Y > 2.5) Y = 2.5;
 #pragma unroll
         for (int b = 0; b < NB; b++) px.pw[s][b] = PH_POW(440.0 / c.wl[b], Y);
+        // the start rule's P depends on the pixel alone (model/samodel.c:2342): formed
+        // once here rather than at each of the pixel's depth starts
+        px.p0[s] = 100.0 * 0.072 * PH_POW(r490 / (double)1 / (ph_rrs_at(c, r, 2) / (double)1),
+                                          -1.7);
         px.ss[s] = ss[s];
         px.sv[s] = sv[s];
     }
@@ -613,10 +618,8 @@ PH_INLINE void ph_start_step(PhNM<PhLayout<NBOT>::template n<ND>()>& S, const Ph
     }
     ph_for<ND>([&](auto s) PH_BODY {
         constexpr int o = L::off + 3 * s;
-        double m490 = ph_rrs_at(c, px.rrs[s], 1) / (double)1;
-        double m550 = ph_rrs_at(c, px.rrs[s], 2) / (double)1;
         double m640 = ph_rrs_at(c, px.rrs[s], 3) / (double)1;
-        S.xs[o] = 100.0 * 0.072 * PH_POW(m490 / m550, -1.7);
+        S.xs[o] = px.p0[s];
         S.xs[o + 1] = 1.5 * S.xs[o];
         S.xs[o + 2] = 100.0 * 30.0 * c.aw640 * m640;
         S.m.set_step(o, 2.0 * S.xs[o]);
