@@ -73,8 +73,9 @@ typedef struct photic_cfg {
                                                 returns PHOTIC_EDEVICE if the watchdog fired */
     /* Seeded single-date inversions only: a seed start whose best misfit is still above this
      * at its first convergence check is given up (as not converged) and the depth starts
-     * follow.  0 (the default) runs every seed start to its end.  The seed start is this
-     * build's addition; the reference's depth starts are never cut short. */
+     * follow.  0 (the default) runs every seed start to its end; a level below the early exit
+     * acts as the early exit.  The seed start is this library's addition; the reference's
+     * depth starts are never cut short. */
     double seed_abort_misfit;
 } photic_cfg;
 

@@ -227,7 +227,9 @@ static int ph_make_const(const photic_cfg *cfg, int n_dates, int has_seed, PhCon
         c->seed_rel_step[k] = cfg->seed_rel_step[k];
         c->seed_min_step[k] = cfg->seed_min_step[k];
     }
+    /* never below the early exit: a seed start given up must not end the pixel as accepted */
     c->seed_abort = cfg->seed_abort_misfit > 0.0 ? cfg->seed_abort_misfit : 0.0;
+    if (c->seed_abort > 0.0 && c->seed_abort < c->early_exit) c->seed_abort = c->early_exit;
     return PHOTIC_OK;
 }
 

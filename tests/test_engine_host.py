@@ -83,6 +83,14 @@ def test_engine_seed_abort_hands_over_to_the_depth_starts():
         assert np.array_equal(h[k][..., sel], o[k][..., sel]), k
     assert np.array_equal(h["n_starts"][sel], o["n_starts"][sel] + 1)
     assert np.all(h["n_evals"][sel] > o["n_evals"][sel] - o["n_starts"][sel])
+    # a level below the early exit (2.5) acts as the early exit: a given-up seed start can
+    # never be the accepted result
+    lo = O.hostsim_invert(P.const_bytes(_cfg(seed_abort_misfit=1.0), has_seed=True), 0, rrs, ss,
+                          seed=seed)
+    at = O.hostsim_invert(P.const_bytes(_cfg(seed_abort_misfit=2.5), has_seed=True), 0, rrs, ss,
+                          seed=seed)
+    for k in ("x", "status", "n_evals", "n_starts"):
+        assert np.array_equal(lo[k], at[k]), k
 
 
 def test_engine_small_kcount_and_restarts():
