@@ -24,6 +24,7 @@ HIPCC_FLAGS = [
 ]
 CC_FLAGS = [
     "-std=c11", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-D__HIP_PLATFORM_AMD__",
+    "-fopenmp",  # the drop-in's host-side gather/scatter loops (libgomp)
     "-I" + os.path.join(ROCM, "include"), "-I" + CSRC, "-I" + os.path.join(ROOT, "include"),
 ]
 HIP_SOURCES = ["ph_kernels.hip", "ph_mlp.hip", "ph_stack.hip", "ph_stack_native.hip",
@@ -56,7 +57,7 @@ def build(verbose=True):
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=" + ARCH, "-shared", "-fPIC",
-          "-o", tmp] + objs + ["-lm"])
+          "-o", tmp] + objs + ["-lgomp", "-lm"])
     os.replace(tmp, LIB)
     return LIB
 

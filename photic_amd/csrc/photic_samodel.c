@@ -11,6 +11,7 @@
  * photic_invert_regions, on the smoothed band grids held on the device.  When the caller
  * passes a depth_sigma grid, the depth-error pass of model/samodel.c:1376-1482 fills it
  * (sm_sigma_pass). */
+#define _POSIX_C_SOURCE 199309L /* clock_gettime */
 #include "photic_samodel.h"
 
 #include <math.h>
@@ -18,9 +19,19 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "photic.h"
 #include "ph_launch.h"
+
+/* PHOTIC_TIMING set in the environment: the drop-in prints the wall time of its phases to
+ * stderr (host gather, device grids, inversion with the scatter, depth-error pass). */
+static double sm_ms(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return 1.0e3 * (double)t.tv_sec + 1.0e-6 * (double)t.tv_nsec;
+}
 
 static int sm_feq(float a, float b, float eps)
 {
@@ -112,6 +123,7 @@ static int sm_run_group(const photic_cfg *cfg, sm_group *g, int nd, const double
     }
     if (!rc) rc = photic_memcpy_dtoh(h, d_out, sizeof(float) * 9 * n);
     if (!rc) {
+#pragma omp parallel for schedule(static)
         for (long p = 0; p < n; p++) {
             long k = g->idx[p];
             int i = (int)(k / ncols), j = (int)(k % ncols);
@@ -285,6 +297,7 @@ static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene
             if (v > maxf) maxf = v;
         }
     if (!(maxf > 0.0f)) return PHOTIC_OK;
+    const double t_start = sm_ms();
     double maxd = maxf > 1.0e9f ? 1.0e9 : maxf; /* only an int's range matters below */
     maxd = SM_SIGMA_INTERVAL * ((int)maxd / SM_SIGMA_INTERVAL);
     if (maxd > 30.0) maxd = 30.0;
@@ -312,11 +325,13 @@ static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene
     for (int d = 0; d < nd; d++)
         for (int b = 0; b < 4; b++) rsig[4 * d + b] = scene_data[scene_indexes[d]].R_sigma[b];
 
-    /* the samples: a pixel in the interval, its n_sigma, its depth-start group */
-    for (int k = 0; !rc && k < n_int; k++) {
-        const double d = SM_SIGMA_INTERVAL * k;
-        for (int s = 0; s < SM_SIGMA_SAMPLES; s++) {
-            const long q = (long)k * SM_SIGMA_SAMPLES + s;
+    /* the samples: a pixel in the interval, its n_sigma, its depth-start group (each sample
+     * from its own Philox counters, so the samples are independent of one another) */
+    if (!rc) {
+#pragma omp parallel for schedule(dynamic, 8)
+        for (long q = 0; q < n_s; q++) {
+            const int k = (int)(q / SM_SIGMA_SAMPLES);
+            const double d = SM_SIGMA_INTERVAL * k;
             const uint32_t ctr = (uint32_t)q;
             pix[q] = -1;
             for (int t = 0; t < n_trials; t++) {
@@ -346,6 +361,7 @@ static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene
             }
         }
     }
+    const double t_sampled = sm_ms();
     /* one batched inversion per depth-start group */
     for (int k = 0; !rc && k < 3; k++) {
         long m = 0;
@@ -368,6 +384,10 @@ static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene
                                h, st, d64);
         for (long p = 0; !rc && p < m; p++) trial[gpos[p]] = st[p] >= 0 ? d64[p] : 0.0;
     }
+    if (getenv("PHOTIC_TIMING"))
+        fprintf(stderr, "photic_samodel: depth-error pass: %d intervals, %ld samples, sampling "
+                        "%.1f ms, inversion %.1f ms\n",
+                n_int, n_s, t_sampled - t_start, sm_ms() - t_sampled);
     /* per interval: the mean (vec_mean2_double, returned as a float) and the standard
      * deviation (vec_stddev_double) of the sampled depths, zeros skipped */
     for (int k = 0; !rc && k < n_int; k++) {
@@ -389,20 +409,16 @@ static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene
             }
         table[k] = count == 0 ? 0.0 : sqrt(sumdev / (double)count);
     }
-    /* every modelled pixel gets its interval's sigma: d < H <= d + 0.25 */
-    for (int i = 0; !rc && i < nrows; i++)
-        for (int j = 0; j < ncols; j++) {
+    /* every modelled pixel gets its interval's sigma: the k with 0.25 k < H <= 0.25 (k + 1),
+     * k < n_int, i.e. k = ceil(4 H) - 1 (4 H and the interval ends are exact in double) */
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nrows; i++)
+        for (int j = 0; !rc && j < ncols; j++) {
             const float v = -depth[i][j];
             depth_sigma[i][j] = 0.0f;
             if (!(v > 0.0f)) continue;
-            int k = 0;
-            for (double d = 0.0; d < maxd; d += SM_SIGMA_INTERVAL) {
-                if (v > d && v <= d + SM_SIGMA_INTERVAL) {
-                    depth_sigma[i][j] = (float)table[k];
-                    break;
-                }
-                k++;
-            }
+            const double k = ceil(4.0 * (double)v) - 1.0;
+            if (k < (double)n_int) depth_sigma[i][j] = (float)table[(int)k];
         }
     free(pix);
     free(ns);
@@ -493,6 +509,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     long npix = (long)nrows * ncols;
     sm_group gr[3];
     memset(gr, 0, sizeof(gr));
+    const double t0 = sm_ms();
     int rc = PHOTIC_OK;
     for (int k = 0; k < 3 && !rc; k++) {
         gr[k].idx = (long *)malloc(sizeof(long) * (size_t)(npix ? npix : 1));
@@ -544,6 +561,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     /* every (scene, band) grid, smoothed, and its no-data value, on the device for the
      * neighbourhood gather (n_spatial > 1) and the depth-error pass */
     float *grid = NULL, *d_grid = NULL, nod[8 * 4];
+    const double t1 = sm_ms();
     if (!rc && (n_spatial > 1 || depth_sigma)) {
         grid = (float *)malloc(sizeof(float) * 4 * (size_t)nd * (size_t)(npix ? npix : 1));
         if (!grid) rc = PHOTIC_ENOMEM;
@@ -562,6 +580,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
         if (!rc) rc = photic_device_malloc((void **)&d_grid, sizeof(float) * 4 * nd * npix);
         if (!rc) rc = photic_memcpy_htod(d_grid, grid, sizeof(float) * 4 * nd * npix);
     }
+    const double t2 = sm_ms();
     for (int k = 0; k < 3 && !rc; k++) {
         photic_cfg c = cfg;
         if (k == 2) {
@@ -578,10 +597,15 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
                               bottom_albedo, bottom_sand, bottom_seagrass, bottom_coral, K_min,
                               bottom_type, index_optical_depth);
     }
+    const double t3 = sm_ms();
     if (!rc && depth_sigma)
         rc = sm_sigma_pass(&cfg, scene_data, scene_indexes, nd, nrows, ncols, n_spatial,
                            empirical_depth_present, empirical_depths, sec_sun, sec_view, d_grid,
                            nod, depth, depth_sigma);
+    if (getenv("PHOTIC_TIMING"))
+        fprintf(stderr, "photic_samodel: gather %.1f ms, device grids %.1f ms, inversion and "
+                        "scatter %.1f ms, depth-error pass %.1f ms\n",
+                t1 - t0, t2 - t1, t3 - t2, sm_ms() - t3);
     free(grid);
     if (d_grid) photic_device_free(d_grid);
     for (int k = 0; k < 3; k++) {
