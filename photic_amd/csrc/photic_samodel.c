@@ -517,41 +517,80 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
         gr[k].hstart = k ? (float *)malloc(sizeof(float) * (size_t)(npix ? npix : 1)) : NULL;
         if (!gr[k].idx || !gr[k].rrs || (k && !gr[k].hstart)) rc = PHOTIC_ENOMEM;
     }
-    /* pass 1: pixel screening over every scene and gather, per group (rows [scene][band],
-     * compacted later per group) */
-    for (int i = 0; i < nrows && !rc; i++)
-        for (int j = 0; j < ncols; j++) {
-            int bad = 0;
-            float v[8 * 4];
-            for (int d = 0; d < nd; d++) {
-                const struct photic_scene *t = &scene_data[scene_indexes[d]];
-                for (int b = 0; b < 4; b++) {
-                    const struct photic_geogrid *g = &gridded_data[t->band_indexes[b]];
-                    float refl = g->array[i][j];
-                    if (sm_feq(refl, g->nodata_value, 1.0e-6f) || refl < 0.0f) bad = 1;
-                    float s = r > 1 ? sm_smoothed(g->array, i, j, nrows, ncols, r, g->nodata_value)
-                                    : refl;
-                    /* one region: a smoothed no-data value drops it (with several, the
-                     * region gather on the device decides) */
-                    if (n_spatial <= 1 && sm_feq(s, g->nodata_value, 1.0e-6f)) bad = 1;
-                    v[4 * d + b] = s;
+    /* pixel screening over every scene and gather, per group, in row-major order: each row
+     * screens its pixels into a row buffer (OpenMP over rows), the per-row counts give every
+     * row its place in each group, and the rows are then copied there (rows [scene][band] of
+     * each group at stride npix, compacted to n below) */
+    signed char *code = NULL;
+    float *hrow = NULL, *vrow = NULL;
+    long *cnt = NULL;
+    if (!rc) {
+        code = (signed char *)malloc((size_t)(npix ? npix : 1));
+        hrow = (float *)malloc(sizeof(float) * (size_t)(npix ? npix : 1));
+        vrow = (float *)malloc(sizeof(float) * 4 * nd * (size_t)(npix ? npix : 1));
+        cnt = (long *)calloc(3 * (size_t)nrows + 3, sizeof(long));
+        if (!code || !hrow || !vrow || !cnt) rc = PHOTIC_ENOMEM;
+    }
+    if (!rc) {
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int i = 0; i < nrows; i++)
+            for (int j = 0; j < ncols; j++) {
+                const long p = (long)i * ncols + j;
+                int bad = 0;
+                for (int d = 0; d < nd; d++) {
+                    const struct photic_scene *t = &scene_data[scene_indexes[d]];
+                    for (int b = 0; b < 4; b++) {
+                        const struct photic_geogrid *g = &gridded_data[t->band_indexes[b]];
+                        float refl = g->array[i][j];
+                        if (sm_feq(refl, g->nodata_value, 1.0e-6f) || refl < 0.0f) bad = 1;
+                        float sv = r > 1 ? sm_smoothed(g->array, i, j, nrows, ncols, r,
+                                                       g->nodata_value)
+                                         : refl;
+                        /* one region: a smoothed no-data value drops it (with several, the
+                         * region gather on the device decides) */
+                        if (n_spatial <= 1 && sm_feq(sv, g->nodata_value, 1.0e-6f)) bad = 1;
+                        vrow[(long)(4 * d + b) * npix + p] = sv;
+                    }
                 }
+                code[p] = -1;
+                if (bad) continue;
+                int k = 0;
+                float h = 0.0f;
+                if (empirical_depth_present &&
+                    !sm_feq(empirical_depths.array[i][j], empirical_depths.nodata_value,
+                            1.0e-6f)) {
+                    float e = empirical_depths.array[i][j];
+                    h = e > -1.0f ? 1.0f : fabsf(e);
+                    k = h > 8.0 ? 2 : 1;
+                }
+                code[p] = (signed char)k;
+                hrow[p] = h;
+                cnt[3 * (long)(i + 1) + k]++;
             }
-            if (bad) continue;
-            int k = 0;
-            float h = 0.0f;
-            if (empirical_depth_present &&
-                !sm_feq(empirical_depths.array[i][j], empirical_depths.nodata_value, 1.0e-6f)) {
-                float e = empirical_depths.array[i][j];
-                h = e > -1.0f ? 1.0f : fabsf(e);
-                k = h > 8.0 ? 2 : 1;
+        /* cnt[3 (i + 1) + k]: row i's pixels in group k; prefix sums give row i's start */
+        for (int i = 0; i < nrows; i++)
+            for (int k = 0; k < 3; k++) cnt[3 * (long)(i + 1) + k] += cnt[3 * (long)i + k];
+        for (int k = 0; k < 3; k++) gr[k].n = cnt[3 * (long)nrows + k];
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nrows; i++) {
+            long at[3] = {cnt[3 * (long)i], cnt[3 * (long)i + 1], cnt[3 * (long)i + 2]};
+            for (int j = 0; j < ncols; j++) {
+                const long p = (long)i * ncols + j;
+                const int k = code[p];
+                if (k < 0) continue;
+                sm_group *g = &gr[k];
+                const long m = at[k]++;
+                g->idx[m] = p;
+                for (int q = 0; q < 4 * nd; q++)
+                    g->rrs[(long)q * npix + m] = vrow[(long)q * npix + p];
+                if (k) g->hstart[m] = hrow[p];
             }
-            sm_group *g = &gr[k];
-            g->idx[g->n] = (long)i * ncols + j;
-            for (int q = 0; q < 4 * nd; q++) g->rrs[(long)q * npix + g->n] = v[q];
-            if (k) g->hstart[g->n] = h;
-            g->n++;
         }
+    }
+    free(code);
+    free(hrow);
+    free(vrow);
+    free(cnt);
     /* compact the planar (scene, band) rows to n per group */
     for (int k = 0; k < 3 && !rc; k++)
         for (int q = 1; q < 4 * nd; q++)
