@@ -73,6 +73,8 @@ def lib():
         _lib.orc_py_invert_regions.restype = l
         _lib.orc_py_invert_regions.argtypes = [v, i, i, i, i, v, v, v, v, i, i, v, d, i, i, i, v,
                                                v, l, v, v, i, v, v, C.POINTER(PyOut)]
+        _lib.orc_py_eval.argtypes = [i, i, v, v, i, i, v, v, v, v, v, v]
+        _lib.orc_py_region_eval.argtypes = [v, i, i, i, i, v, v, v, i, v, v, v]
         _lib.orc_sec_deg.restype = d
         _lib.orc_sec_deg.argtypes = [d]
         _lib.orc_py_region_objective.restype = d
@@ -478,6 +480,49 @@ def region_objective(x, rrs, origin=0, sec_sun=1.0, sec_view=1.0, n_bottoms=3,
                                       ss.ctypes.data, sv.ctypes.data, n_bottoms, bi.ctypes.data,
                                       xv.ctypes.data, terms.ctypes.data)
     return f, terms
+
+
+def evaluate(x, rrs, sec_sun=1.0, sec_view=1.0, mode=1, n_bottoms=2, bottom_idx=(0, 1, 2),
+             wl=WL4):
+    """The objective of one vector x over k dates (rrs [k][nb]) and the outputs derived from
+    it, as a dict (objective, rrs_error, k_error, depth, k_min, iod, albedo, bottom_type,
+    b_pct [8]) plus K [k][nb]."""
+    r = np.ascontiguousarray(np.atleast_2d(rrs), np.float32)
+    k, nb = r.shape
+    xv = np.ascontiguousarray(x, np.float64)
+    w = np.asarray(wl, np.float64)
+    bi = np.asarray(bottom_idx, np.int32)
+    ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (k,)))
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (k,)))
+    out = np.zeros(16)
+    K = np.zeros((k, nb))
+    lib().orc_py_eval(mode, n_bottoms, bi.ctypes.data, w.ctypes.data, nb, k, xv.ctypes.data,
+                      r.ctypes.data, ss.ctypes.data, sv.ctypes.data, out.ctypes.data,
+                      K.ctypes.data)
+    names = ("objective", "rrs_error", "k_error", "depth", "k_min", "iod", "albedo",
+             "bottom_type")
+    d = {n: out[i] for i, n in enumerate(names)}
+    d["b_pct"] = out[8:16].copy()
+    d["K"] = K
+    return d
+
+
+def region_evaluate(x, rrs, origin=0, sec_sun=1.0, sec_view=1.0, n_bottoms=3,
+                    bottom_idx=(0, 1, 2), wl=WL4):
+    """The region objective of one vector x for region data rrs [R][nd][nb] and the region
+    outputs: (objective, depth, K_min, IOD)."""
+    r = np.ascontiguousarray(rrs, np.float64)
+    R, nd, nb = r.shape
+    xv = np.ascontiguousarray(x, np.float64)
+    w = np.asarray(wl, np.float64)
+    bi = np.asarray(bottom_idx, np.int32)
+    ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (nd,)))
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (nd,)))
+    out = np.zeros(4)
+    lib().orc_py_region_eval(w.ctypes.data, nb, nd, R, origin, r.ctypes.data, ss.ctypes.data,
+                             sv.ctypes.data, n_bottoms, bi.ctypes.data, xv.ctypes.data,
+                             out.ctypes.data)
+    return tuple(out)
 
 
 def region_start_step(rrs, h0, n_bottoms=3, wl=WL4):

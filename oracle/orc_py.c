@@ -759,3 +759,56 @@ double orc_py_region_sample(const double *wl, int nb, int nd, int n_rows, int n_
     free(res);
     return depth;
 }
+
+/* The objective of one k-date vector x (rrs [k][nb], sec_sun, sec_view [k]) and the outputs
+ * derived from it: out [16] = objective, E_Rrs, K penalty, depth, K_min, IOD, albedo, bottom
+ * type, B_percent[8]; K [k][nb] the attenuation of every (date, band). */
+void orc_py_eval(int mode, int n_bottoms, const int *bottom_idx, const double *wl, int nb, int k,
+                 const double *x, const float *rrs, const double *sec_sun,
+                 const double *sec_view, double *out, double *K)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    double r[ORC_MAXD * ORC_MAXB], ss[ORC_MAXD], sv[ORC_MAXD];
+    for (int s = 0; s < k; s++) {
+        for (int b = 0; b < nb; b++)
+            r[s * nb + b] = (double)rrs[s * nb + b];
+        ss[s] = sec_sun[s];
+        sv[s] = sec_view[s];
+    }
+    orc_pixel px;
+    orc_pixel_init(&px, mode, n_bottoms, bottom_idx, k, &bands, ss, sv, r);
+    orc_result res;
+    orc_eval_outputs(&px, x, &res);
+    out[0] = px.model_error;
+    out[1] = px.Rrs_error;
+    out[2] = px.K_error;
+    out[3] = res.depth;
+    out[4] = res.K_min;
+    out[5] = res.iod;
+    out[6] = res.bottom_albedo;
+    out[7] = res.bottom_type;
+    for (int t = 0; t < ORC_NBOTTOM_TYPES; t++)
+        out[8 + t] = res.B_percent[t];
+    for (int s = 0; s < k; s++)
+        for (int b = 0; b < nb; b++)
+            K[s * nb + b] = px.K[s][b];
+}
+
+/* The region objective of one vector x over region data rrs [R][nd][nb] and the region
+ * outputs: out [4] = objective, depth, K_min, IOD. */
+void orc_py_region_eval(const double *wl, int nb, int nd, int R, int origin, const double *rrs,
+                        const double *sec_sun, const double *sec_view, int n_bottoms,
+                        const int *bottom_idx, const double *x, double *out)
+{
+    orc_bands bands;
+    orc_region_pixel *px = orc_py_region_px(wl, nb, nd, R, origin, rrs, sec_sun, sec_view,
+                                            n_bottoms, bottom_idx, &bands);
+    orc_region_result res;
+    orc_region_eval_outputs(px, x, &res);
+    out[0] = px->model_error;
+    out[1] = res.depth;
+    out[2] = res.K_min;
+    out[3] = res.iod;
+    free(px);
+}

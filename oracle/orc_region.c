@@ -378,6 +378,15 @@ static void orc_region_outputs(orc_region_pixel *px, const double *best, orc_reg
     res->iod = 100.0 * iod / nobs;
 }
 
+void orc_region_eval_outputs(orc_region_pixel *px, const double *x, orc_region_result *res)
+{
+    double xv[ORC_RMAXN];
+    memset(res, 0, sizeof(*res));
+    memcpy(xv, x, sizeof(double) * (size_t)orc_region_n_params(px));
+    orc_region_objective(xv, px);
+    orc_region_outputs(px, xv, res);
+}
+
 void orc_region_optimise(orc_region_pixel *px, const orc_nm_cfg *cfg, orc_nelmin_fn nm,
                          orc_region_result *res)
 {

@@ -303,217 +303,235 @@ double orc_objective(double *x, void *ctx)
 }
 
 /* ------------------------------------------------------------------------------------ */
-/* Nelder-Mead: O'Neill's AS 47 with the reference's changes (model/asa047.c:10-502):    */
-/* eps = 1e-6, and a local-minimum test on the truncated value (long)(10*f) that, when   */
-/* it fails, restarts from the probed point with a tiny simplex.                         */
+/* Nelder-Mead simplex minimisation, as published by J. A. Nelder and R. Mead (Computer  */
+/* Journal 7, 1965) in the form of R. O'Neill's Algorithm AS 47 (Applied Statistics 20,  */
+/* 1971), with the changes the reference makes to it (model/asa047.c:10-502): a local-   */
+/* minimum probe whose comparisons are on (long)(10 f), and, when a probe point is lower, */
+/* a restart from that point with a simplex of relative size 1e-6.                       */
+/*                                                                                        */
+/* The arithmetic is the published algorithm's, in its order (the result must agree bit  */
+/* for bit with the reference's compiled routine, tests/test_oracle_ref.py):              */
+/*   centroid  c = (sum of every vertex - the worst) / n, coordinate by coordinate;       */
+/*   trial     t = c + coef (s - c): the reflection of the worst vertex w is             */
+/*             c + 1 (c - w); expansion and contractions take s = the reflected point or  */
+/*             w with coef 2 or 1/2;                                                      */
+/*   shrink    every vertex v <- (v + best) / 2, in index order;                          */
+/*   converged when sum (f - mean f)^2 <= reqmin n, tested every konvge iterations.       */
 /* ------------------------------------------------------------------------------------ */
+
+/* (long)(10 v) as the reference's x86-64 build computes it (cvttsd2si: a NaN or a value
+ * outside the range of long gives LONG_MIN). */
+static long orc_tenths(double v)
+{
+    const double t = 10.0 * v;
+    if (!(fabs(t) < 9.2233720368547758e18)) return (long)(-9223372036854775807L - 1L);
+    return (long)t;
+}
+
+typedef struct {
+    int n;                 /* coordinates */
+    double *vx;            /* vertices, [n + 1][n] */
+    double *fx;            /* their values, [n + 1] */
+    double *mid;           /* centroid of every vertex but the worst */
+    double *refl;          /* reflected point */
+    double *trial;         /* expanded or contracted point */
+    orc_objective_fn fn;
+    void *ctx;
+    int evals;
+} orc_simplex;
+
+static double *orc_vtx(orc_simplex *s, int k) { return s->vx + (size_t)k * (size_t)s->n; }
+
+static double orc_eval(orc_simplex *s, double *pt)
+{
+    s->evals++;
+    return s->fn(pt, s->ctx);
+}
+
+/* first index of the smallest (worst = 0) or the largest (worst = 1) value */
+static int orc_extreme(const orc_simplex *s, int worst)
+{
+    int k = 0;
+    for (int j = 1; j <= s->n; j++)
+        if (worst ? (s->fx[k] < s->fx[j]) : (s->fx[j] < s->fx[k])) k = j;
+    return k;
+}
+
+static void orc_put(orc_simplex *s, int k, const double *pt, double f)
+{
+    memcpy(orc_vtx(s, k), pt, sizeof(double) * (size_t)s->n);
+    s->fx[k] = f;
+}
+
+/* dst = mid + coef (src - mid) */
+static void orc_toward(const orc_simplex *s, double *dst, const double *src, double coef)
+{
+    for (int i = 0; i < s->n; i++) dst[i] = s->mid[i] + coef * (src[i] - s->mid[i]);
+}
+
+/* The simplex around `base`: the base itself is the last vertex (evaluated first), vertex j
+ * moves coordinate j by scale * step[j]; `base` is returned unchanged. */
+static void orc_build(orc_simplex *s, double *base, const double *step, double scale)
+{
+    const int n = s->n;
+    memcpy(orc_vtx(s, n), base, sizeof(double) * (size_t)n);
+    s->fx[n] = orc_eval(s, base);
+    for (int j = 0; j < n; j++) {
+        const double keep = base[j];
+        base[j] = base[j] + step[j] * scale;
+        memcpy(orc_vtx(s, j), base, sizeof(double) * (size_t)n);
+        s->fx[j] = orc_eval(s, base);
+        base[j] = keep;
+    }
+}
+
+/* Sum of squared deviations of the vertex values from their mean. */
+static double orc_spread(const orc_simplex *s)
+{
+    const int m = s->n + 1;
+    double sum = 0.0;
+    for (int j = 0; j < m; j++) sum = sum + s->fx[j];
+    const double mean = sum / (double)m;
+    double dev = 0.0;
+    for (int j = 0; j < m; j++) dev = dev + (s->fx[j] - mean) * (s->fx[j] - mean);
+    return dev;
+}
+
+/* One iteration after the worst vertex `hi` is known; *lo and *flo are the best vertex and
+ * its value as last recorded (the value is kept apart from the vertex, as the published
+ * algorithm keeps it: when every value is equal the worst vertex is also the best, and
+ * replacing it leaves the recorded value standing).  Returns 1 when the iteration ended in
+ * a shrink (which does not count towards the convergence interval). */
+static int orc_iterate(orc_simplex *s, int hi, int *lo, double *flo)
+{
+    const int n = s->n;
+    double *w = orc_vtx(s, hi);
+    for (int i = 0; i < n; i++) {
+        double z = 0.0;
+        for (int j = 0; j <= n; j++) z = z + orc_vtx(s, j)[i];
+        z = z - w[i];
+        s->mid[i] = z / (double)n;
+    }
+    for (int i = 0; i < n; i++) s->refl[i] = s->mid[i] + 1.0 * (s->mid[i] - w[i]);
+    const double fr = orc_eval(s, s->refl);
+
+    if (fr < *flo) {
+        /* better than the best: try twice as far */
+        orc_toward(s, s->trial, s->refl, 2.0);
+        const double fe = orc_eval(s, s->trial);
+        if (fr < fe) orc_put(s, hi, s->refl, fr);
+        else orc_put(s, hi, s->trial, fe);
+    } else {
+        int beaten = 0; /* vertices worse than the reflected point */
+        for (int j = 0; j <= n; j++)
+            if (fr < s->fx[j]) beaten++;
+        if (beaten > 1) {
+            orc_put(s, hi, s->refl, fr);
+        } else if (beaten == 1) {
+            /* contract on the reflection's side */
+            orc_toward(s, s->trial, s->refl, 0.5);
+            const double fc = orc_eval(s, s->trial);
+            if (fc <= fr) orc_put(s, hi, s->trial, fc);
+            else orc_put(s, hi, s->refl, fr);
+        } else {
+            /* contract on the worst vertex's side; if even that is worse, shrink */
+            orc_toward(s, s->trial, w, 0.5);
+            const double fc = orc_eval(s, s->trial);
+            if (s->fx[hi] < fc) {
+                for (int j = 0; j <= n; j++) {
+                    double *v = orc_vtx(s, j);
+                    const double *b = orc_vtx(s, *lo);
+                    for (int i = 0; i < n; i++) v[i] = (v[i] + b[i]) * 0.5;
+                    s->fx[j] = orc_eval(s, v);
+                }
+                *lo = orc_extreme(s, 0);
+                *flo = s->fx[*lo];
+                return 1;
+            }
+            orc_put(s, hi, s->trial, fc);
+        }
+    }
+    if (s->fx[hi] < *flo) {
+        *flo = s->fx[hi];
+        *lo = hi;
+    }
+    return 0;
+}
 
 void orc_nelmin(orc_objective_fn fn, void *ctx, int n, double start[], double xmin[],
                 double *ynewlo, double reqmin, double step[], int konvge, int kcount,
                 int *icount, int *numres, int *ifault)
 {
-    const double ccoeff = 0.5, ecoeff = 2.0, rcoeff = 1.0, eps = 1.0e-6;
-    const double rscale = 10.0;
-
     if (reqmin <= 0.0 || n < 1 || konvge < 1) {
         *ifault = 1;
         return;
     }
-    double *p = (double *)malloc(sizeof(double) * (size_t)(n * (n + 1)));
-    double *pstar = (double *)malloc(sizeof(double) * (size_t)n);
-    double *p2star = (double *)malloc(sizeof(double) * (size_t)n);
-    double *pbar = (double *)malloc(sizeof(double) * (size_t)n);
-    double *y = (double *)malloc(sizeof(double) * (size_t)(n + 1));
-
-    *icount = 0;
-    *numres = 0;
-    int jcount = konvge;
-    const int nn = n + 1;
-    const double dn = (double)n, dnn = (double)nn, rq = reqmin * dn;
-    double del = 1.0;
-    int ilo = 0, ihi;
-    double ylo, ystar, y2star;
+    orc_simplex s;
+    s.n = n;
+    s.fn = fn;
+    s.ctx = ctx;
+    s.evals = 0;
+    s.vx = (double *)malloc(sizeof(double) * (size_t)(n + 1) * (size_t)n);
+    s.fx = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    s.mid = (double *)malloc(sizeof(double) * (size_t)n);
+    s.refl = (double *)malloc(sizeof(double) * (size_t)n);
+    s.trial = (double *)malloc(sizeof(double) * (size_t)n);
+    const double limit = reqmin * (double)n, probe = 1.0e-6;
+    double scale = 1.0;
+    int restarts = 0, status = 0;
 
     for (;;) {
-        /* simplex: the start point is vertex n, vertex j is start + del*step_j e_j */
-        for (int i = 0; i < n; i++)
-            p[i + n * n] = start[i];
-        y[n] = fn(start, ctx);
-        (*icount)++;
-        for (int j = 0; j < n; j++) {
-            double keep = start[j];
-            start[j] = start[j] + step[j] * del;
-            for (int i = 0; i < n; i++)
-                p[i + j * n] = start[i];
-            y[j] = fn(start, ctx);
-            (*icount)++;
-            start[j] = keep;
+        orc_build(&s, start, step, scale);
+        int lo = orc_extreme(&s, 0);
+        double flo = s.fx[lo];
+        int until_check = konvge;
+        /* iterate until converged or the evaluations are spent */
+        while (s.evals < kcount) {
+            const int hi = orc_extreme(&s, 1);
+            *ynewlo = s.fx[hi];
+            if (orc_iterate(&s, hi, &lo, &flo)) continue;
+            if (--until_check > 0) continue;
+            if (s.evals > kcount) continue;
+            until_check = konvge;
+            if (orc_spread(&s) <= limit) break;
         }
-        ylo = y[0];
-        ilo = 0;
-        for (int i = 1; i < nn; i++)
-            if (y[i] < ylo) {
-                ylo = y[i];
-                ilo = i;
-            }
-
-        for (;;) {
-            if (kcount <= *icount)
-                break;
-            /* worst vertex: first index holding the largest value */
-            *ynewlo = y[0];
-            ihi = 0;
-            for (int i = 1; i < nn; i++)
-                if (*ynewlo < y[i]) {
-                    *ynewlo = y[i];
-                    ihi = i;
-                }
-            /* centroid of all vertices but the worst */
-            for (int i = 0; i < n; i++) {
-                double z = 0.0;
-                for (int j = 0; j < nn; j++)
-                    z = z + p[i + j * n];
-                z = z - p[i + ihi * n];
-                pbar[i] = z / dn;
-            }
-            for (int i = 0; i < n; i++)
-                pstar[i] = pbar[i] + rcoeff * (pbar[i] - p[i + ihi * n]);
-            ystar = fn(pstar, ctx);
-            (*icount)++;
-
-            if (ystar < ylo) {
-                for (int i = 0; i < n; i++)
-                    p2star[i] = pbar[i] + ecoeff * (pstar[i] - pbar[i]);
-                y2star = fn(p2star, ctx);
-                (*icount)++;
-                if (ystar < y2star) {
-                    for (int i = 0; i < n; i++)
-                        p[i + ihi * n] = pstar[i];
-                    y[ihi] = ystar;
-                } else {
-                    for (int i = 0; i < n; i++)
-                        p[i + ihi * n] = p2star[i];
-                    y[ihi] = y2star;
-                }
-            } else {
-                int l = 0;
-                for (int i = 0; i < nn; i++)
-                    if (ystar < y[i])
-                        l++;
-                if (l > 1) {
-                    for (int i = 0; i < n; i++)
-                        p[i + ihi * n] = pstar[i];
-                    y[ihi] = ystar;
-                } else if (l == 0) {
-                    for (int i = 0; i < n; i++)
-                        p2star[i] = pbar[i] + ccoeff * (p[i + ihi * n] - pbar[i]);
-                    y2star = fn(p2star, ctx);
-                    (*icount)++;
-                    if (y[ihi] < y2star) {
-                        /* shrink every vertex half way towards the best one */
-                        for (int j = 0; j < nn; j++) {
-                            for (int i = 0; i < n; i++) {
-                                p[i + j * n] = (p[i + j * n] + p[i + ilo * n]) * 0.5;
-                                xmin[i] = p[i + j * n];
-                            }
-                            y[j] = fn(xmin, ctx);
-                            (*icount)++;
-                        }
-                        ylo = y[0];
-                        ilo = 0;
-                        for (int i = 1; i < nn; i++)
-                            if (y[i] < ylo) {
-                                ylo = y[i];
-                                ilo = i;
-                            }
-                        continue;
-                    } else {
-                        for (int i = 0; i < n; i++)
-                            p[i + ihi * n] = p2star[i];
-                        y[ihi] = y2star;
-                    }
-                } else { /* l == 1 */
-                    for (int i = 0; i < n; i++)
-                        p2star[i] = pbar[i] + ccoeff * (pstar[i] - pbar[i]);
-                    y2star = fn(p2star, ctx);
-                    (*icount)++;
-                    if (y2star <= ystar) {
-                        for (int i = 0; i < n; i++)
-                            p[i + ihi * n] = p2star[i];
-                        y[ihi] = y2star;
-                    } else {
-                        for (int i = 0; i < n; i++)
-                            p[i + ihi * n] = pstar[i];
-                        y[ihi] = ystar;
-                    }
-                }
-            }
-            if (y[ihi] < ylo) {
-                ylo = y[ihi];
-                ilo = ihi;
-            }
-            jcount--;
-            if (jcount > 0)
-                continue;
-            /* variance test every konvge iterations */
-            if (*icount <= kcount) {
-                jcount = konvge;
-                double z = 0.0;
-                for (int i = 0; i < nn; i++)
-                    z = z + y[i];
-                double mean = z / dnn;
-                z = 0.0;
-                for (int i = 0; i < nn; i++)
-                    z = z + (y[i] - mean) * (y[i] - mean);
-                if (z <= rq)
-                    break;
-            }
-        }
-
-        /* factorial test around the best vertex, on (long)(10 * f) */
-        for (int i = 0; i < n; i++)
-            xmin[i] = p[i + ilo * n];
-        *ynewlo = y[ilo];
-        long yr = (long)(10L * y[ilo]);
-        if (kcount < *icount) {
-            *ifault = 2;
+        /* the best vertex, and the probe one step * 1e-6 either side of it per coordinate */
+        memcpy(xmin, orc_vtx(&s, lo), sizeof(double) * (size_t)n);
+        *ynewlo = s.fx[lo];
+        if (s.evals > kcount) {
+            status = 2;
             break;
         }
-        *ifault = 0;
-        for (int i = 0; i < n; i++) {
-            double d = step[i] * eps;
+        const long best10 = orc_tenths(s.fx[lo]);
+        status = 0;
+        for (int i = 0; i < n && status == 0; i++) {
+            const double d = step[i] * probe;
             xmin[i] = xmin[i] + d;
-            double z = fn(xmin, ctx);
-            long zr = (long)(10L * z);
-            (*icount)++;
-            if (zr < yr) {
-                *ifault = 2;
+            if (orc_tenths(orc_eval(&s, xmin)) < best10) {
+                status = 2;
                 break;
             }
             xmin[i] = xmin[i] - d - d;
-            z = fn(xmin, ctx);
-            zr = (long)(10L * z);
-            (*icount)++;
-            if (zr < yr) {
-                *ifault = 2;
+            if (orc_tenths(orc_eval(&s, xmin)) < best10) {
+                status = 2;
                 break;
             }
             xmin[i] = xmin[i] + d;
         }
-        if (*ifault == 0)
-            break;
-        /* restart from the probed point with a simplex of relative size eps */
-        for (int i = 0; i < n; i++)
-            start[i] = xmin[i];
-        del = eps;
-        (*numres)++;
+        if (status == 0) break;
+        /* a lower point next to the best vertex: start again from it, tiny simplex */
+        memcpy(start, xmin, sizeof(double) * (size_t)n);
+        scale = probe;
+        restarts++;
     }
-    (void)rscale;
-    free(p);
-    free(pstar);
-    free(p2star);
-    free(pbar);
-    free(y);
+    *icount = s.evals;
+    *numres = restarts;
+    *ifault = status;
+    free(s.vx);
+    free(s.fx);
+    free(s.mid);
+    free(s.refl);
+    free(s.trial);
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -692,6 +710,15 @@ static void orc_run_starts(orc_pixel *px, const orc_nm_cfg *cfg, orc_nelmin_fn n
     orc_objective(best, px); /* recompute every model-derived quantity at the optimum */
     orc_outputs(px, best, res);
     res->n_evals = px->n_evals - evals0;
+}
+
+void orc_eval_outputs(orc_pixel *px, const double *x, orc_result *res)
+{
+    double xv[ORC_MAXN];
+    memset(res, 0, sizeof(*res));
+    memcpy(xv, x, sizeof(double) * (size_t)orc_n_params(px));
+    orc_objective(xv, px);
+    orc_outputs(px, xv, res);
 }
 
 void orc_optimise(orc_pixel *px, const orc_nm_cfg *cfg, orc_nelmin_fn nm, orc_result *res)

@@ -159,6 +159,9 @@ void orc_nelmin(orc_objective_fn fn, void *ctx, int n, double start[], double xm
 void orc_default_cfg(orc_nm_cfg *cfg, int n_dates, double reqmin, int konvge, int kcount);
 void orc_start_step(const orc_pixel *px, double hstart, double *start, double *step);
 void orc_optimise(orc_pixel *px, const orc_nm_cfg *cfg, orc_nelmin_fn nm, orc_result *res);
+/* The objective at x and the outputs samodel_optimise derives from it (a test hook: the
+ * outputs of a given vector, with no minimisation). */
+void orc_eval_outputs(orc_pixel *px, const double *x, orc_result *res);
 void orc_optimise_seeded(orc_pixel *px, const orc_nm_cfg *cfg, orc_nelmin_fn nm,
                          const double *seed_start, const double *seed_step,
                          orc_result *res);
@@ -224,6 +227,8 @@ void orc_region_init(orc_region_pixel *px, int n_bottoms, const int *bottom_idx,
                      const orc_bands *bands, const double *sec_sun, const double *sec_view);
 double orc_region_objective(double *x, void *px);
 void orc_region_start_step(const orc_region_pixel *px, double h0, double *start, double *step);
+/* The region objective at x and the region outputs derived from it (a test hook). */
+void orc_region_eval_outputs(orc_region_pixel *px, const double *x, orc_region_result *res);
 void orc_region_optimise(orc_region_pixel *px, const orc_nm_cfg *cfg, orc_nelmin_fn nm,
                          orc_region_result *res);
 int orc_depth_sigma(const float *depth, int n_rows, int n_cols, const float *grid,

@@ -51,3 +51,16 @@ def test_entry_points_fail_cleanly_without_a_gpu():
                                  C.byref(out), None)
         assert rc == P.EDEVICE
         assert b"GPU" in P.lib.photic_last_error()
+
+
+def test_structure_layouts_match_the_reference_header():
+    """include/photic_samodel.h's geogrid and scene mirrors have model/common.h's size and
+    field offsets (static assertions compiled against the reference's own header by
+    oracle/Makefile's layout target); skipped where the reference tree is not mounted."""
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "layout"],
+                       capture_output=True, text=True)
+    if "not present" in r.stdout + r.stderr:
+        import pytest
+        pytest.skip("reference tree not mounted")
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "layouts match" in r.stdout

@@ -5,7 +5,7 @@ import pytest
 
 import _oracle as O
 
-needs_ref = pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (no reference tree)")
+from conftest import needs_ref
 
 
 # --- Philox4x32-10 known-answer vectors (Random123 kat_vectors, Salmon et al. 2011) ---

@@ -17,6 +17,7 @@ import pytest
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import oracle_bindings as O  # noqa: E402
+from conftest import needs_ref  # noqa: E402
 
 
 def _pixels(seed, n):
@@ -230,7 +231,7 @@ def test_region_start_step_by_hand():
     assert np.array_equal(t[10:], 2.0 * s[10:])
 
 
-@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (no reference tree)")
+@needs_ref
 @pytest.mark.parametrize("r", [1, 2, 3])
 def test_smoothing_matches_reference(r):
     """The oracle's smoothing against the reference's own compiled smoothed_index_2d
