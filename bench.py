@@ -27,6 +27,9 @@ Two further workloads, each with its own JSON line (not the headline):
                      one sun zenith), each pixel inverted with its 3 x 3 neighbourhood: the
                      reference's default n_spatial = 2 with 3 bottoms and its own Nelder-Mead
                      settings (reqmin 1e-2, konvge 100, kcount 5000), inputs resident in HBM.
+  --workload ordered the headline scene through the ordered inversion (the reference's
+                     spectral-index order, look-up table and hot start recast as runs with
+                     leaders), beside the MLP-seeded and the plain inversion of the same scene.
   --workload dropin  4096 x 4096 pixels of the same kind of coherent scene through the drop-in
                      samodel() with host row-pointer grids in and out, as the reference's caller
                      hands them over (one scene, n_spatial 1, 2 bottoms, the reference's
@@ -38,6 +41,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -77,12 +81,36 @@ def timed_invert(P, cfg, n, d_rrs, d_ss, d_seed, outs, stream):
     return float(ms.value)
 
 
+def host_cpus():
+    """The host's CPU facts and the thread count the CPU baseline uses: one thread per core
+    this process may run on (its affinity mask), capped by OMP_NUM_THREADS where the
+    environment sets it (the CPU share that a one-GPU job is given)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = aff or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    info = {"nproc": os.cpu_count(), "affinity_cpus": aff, "omp_num_threads": omp or None,
+            "threads_used": threads, "model": None, "threads_per_core": None}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            if k.strip() == "Model name":
+                info["model"] = v.strip()
+            elif k.strip() == "Thread(s) per core":
+                info["threads_per_core"] = int(v.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return threads, info
+
+
 def cpu_baseline(rrs_h, ss_h, n_sample, seconds_hint):
     """The FP64 oracle (OpenMP over pixels, the reference's dynamic schedule) on the first
     n_sample pixels of the same scene; returns px/s and the threads used."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_bindings as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, _ = host_cpus()
     used = O.set_threads(threads)
     t = time.perf_counter()
     O.invert(rrs_h[:, :n_sample], ss_h[:n_sample], reqmin=1e-10, konvge=100, kcount=2000)
@@ -112,9 +140,11 @@ def main():
                     help="seeded runs: give a seed start up while its objective is above this "
                          "(the early exit stops below 2.5) at its first convergence check "
                          "(0: never)")
-    ap.add_argument("--workload", choices=("scene", "stacks", "stream", "regions", "dropin"),
+    ap.add_argument("--workload", choices=("scene", "stacks", "stream", "regions", "dropin",
+                                           "ordered"),
                     default="scene")
     ap.add_argument("--dropin-side", type=int, default=4096)
+    ap.add_argument("--ordered-cpu-sample", type=int, default=65536)
     ap.add_argument("--dropin-bottoms", type=int, default=2)
     ap.add_argument("--region-side", type=int, default=512)
     ap.add_argument("--region-spatial", type=int, default=2)
@@ -144,6 +174,8 @@ def main():
         return bench_regions(args, P)
     if args.workload == "dropin":
         return bench_dropin(args, P)
+    if args.workload == "ordered":
+        return bench_ordered(args, P)
     if args.copy_calibration:
         nbytes = args.copy_calibration
         src, dst = P.DeviceBuffer(nbytes), P.DeviceBuffer(nbytes)
@@ -169,7 +201,7 @@ def main():
     P.check(P.lib.photic_synchronize())
     truth_H = d_truth.to_array(np.float32, (6, n))[0]
     del d_truth
-    fields = ("depth", "rrs_error", "status", "n_evals", "n_starts")
+    fields = ("depth", "rrs_error", "model_error", "status", "n_evals", "n_starts")
 
     # ---- the warm-start network, trained once on forward-model samples (not timed) ----
     net = None
@@ -201,7 +233,8 @@ def main():
         return r, sv
 
     def run(kind, cfg_run, n_run):
-        """One step of `kind` over the first n_run pixels; returns (wall ms, kernel ms,
+        """One step of `kind` over the first n_run pixels -- network inference when seeded, the
+        inversion, and the outputs copied back to the host; returns (wall ms, kernel ms,
         outputs)."""
         r, sv = scene_slice(n_run)
         outs = P.DeviceOutputs(cfg_run, n_run, fields=fields)
@@ -211,8 +244,8 @@ def main():
             net.seed_device(cfg_run, n_run, r.value, sv.value, d_seed.value, stream=stream)
         k_ms = timed_invert(P, cfg_run, n_run, r.value, sv.value,
                             d_seed.value if kind == "seeded" else None, outs, stream)
-        P.check(P.lib.photic_synchronize())
-        return 1e3 * (time.perf_counter() - t0), k_ms, outs.fetch()
+        res = outs.fetch()
+        return 1e3 * (time.perf_counter() - t0), k_ms, res
 
     def summary(ms, res, n_run):
         v = res["status"] >= 0
@@ -272,6 +305,21 @@ def main():
     if net is not None:
         w_ms, _, pres = run("plain", cfg, n)
         comparisons["plain"] = summary(w_ms, pres, n)
+        # The judge rule: how many pixels end with a seeded misfit above the plain method's.
+        # A seeded pixel falls back to the reference's depth starts whenever its seed start
+        # stays at or above the early exit, so it can end worse than the plain method only
+        # where both end below the early exit (the plain method accepts any start below it).
+        both = (res["status"] >= 0) & (pres["status"] >= 0)
+        worse = both & (res["model_error"] > pres["model_error"])
+        judge = {"pixels": int(both.sum()),
+                 "seeded_worse_than_plain_frac": float(worse.sum() / max(both.sum(), 1)),
+                 "worse_with_seeded_at_or_above_early_exit": int(
+                     (worse & (res["model_error"] >= 2.5)).sum()),
+                 "median_misfit_of_worse_seeded": float(np.median(res["rrs_error"][worse]))
+                 if worse.any() else None,
+                 "median_misfit_of_worse_plain": float(np.median(pres["rrs_error"][worse]))
+                 if worse.any() else None}
+        comparisons["judge"] = judge
         if not args.no_equal_misfit:
             # The plain method at equal misfit: the reference's depth starts with the early
             # exit lowered until the plain median misfit is at or below the seeded one, chosen
@@ -296,6 +344,10 @@ def main():
                                    watchdog_s=900.0, early_exit=ee_eq)
             w_ms, _, eres = run("plain", cfg_eq, n)
             full = summary(w_ms, eres, n)
+            both_eq = (res["status"] >= 0) & (eres["status"] >= 0)
+            judge["seeded_worse_than_plain_equal_misfit_frac"] = float(
+                (both_eq & (res["model_error"] > eres["model_error"])).sum() /
+                max(both_eq.sum(), 1))
             v_all = res["status"] >= 0
             comparisons["plain_equal_misfit"] = dict(
                 full, early_exit=ee_eq, pixels=n,
@@ -373,6 +425,7 @@ def main():
             "sample": "first %d pixels of the same scene, FP64 oracle, OpenMP dynamic "
                       "schedule, %.1f s" % (ns, dt),
         }
+        line["cpu_baseline"]["host"] = host_cpus()[1]
         line["gpu_over_cpu"] = px_s / cpx
         if "plain" in line:
             line["plain_gpu_over_cpu"] = line["plain"]["pixels_per_s"] / cpx
@@ -395,7 +448,7 @@ def _stack_cpu(rrs_h, ss_h, valid_h, n_sample, seed=None):
     """The FP64 oracle's stack inversion (OpenMP over stacks) on the first n_sample stacks."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_bindings as O
-    threads = min(16, os.cpu_count() or 1)
+    threads, _ = host_cpus()
     used = O.set_threads(threads)
     t = time.perf_counter()
     O.invert_stack(np.ascontiguousarray(rrs_h[:, :, :n_sample]),
@@ -504,6 +557,7 @@ def bench_stacks(args, P):
                                 "kind": "port",
                                 "sample": "first %d stacks of the same scene, FP64 oracle, "
                                           "OpenMP dynamic schedule, %.1f s" % (ns, dt)}
+        line["cpu_baseline"]["host"] = host_cpus()[1]
         line["gpu_over_cpu"] = line["value"] / cpx
     P.lib.photic_stream_destroy(stream)
     print(json.dumps(line))
@@ -617,7 +671,7 @@ def bench_regions(args, P):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_bindings as O
-        threads = min(16, os.cpu_count() or 1)
+        threads, _ = host_cpus()
         used = O.set_threads(threads)
         ns_cpu = min(args.region_cpu_sample, n)
         rrs_h = d_rrs.to_array(np.float32, (1, 4, n))
@@ -630,6 +684,7 @@ def bench_regions(args, P):
                                 "sample": "first %d pixels of the same scene (with their "
                                           "neighbourhoods), FP64 oracle, OpenMP dynamic "
                                           "schedule, %.1f s" % (ns_cpu, dt)}
+        line["cpu_baseline"]["host"] = host_cpus()[1]
         line["gpu_over_cpu"] = line["value"] / cpx
     P.lib.photic_stream_destroy(stream)
     print(json.dumps(line))
@@ -826,9 +881,113 @@ def bench_stream(args, P):
                                 "kind": "port",
                                 "sample": "first %d stacks of the same scene, FP64 oracle "
                                           "(plain), OpenMP dynamic schedule, %.1f s" % (ns, dt)}
+        line["cpu_baseline"]["host"] = host_cpus()[1]
         line["gpu_over_cpu"] = line["value"] / cpx
     net.close()
     print(json.dumps(line))
+
+def bench_ordered(args, P):
+    """The ordered inversion (spectral-index order, look-up table, hot start) on the headline
+    scene, beside the MLP-seeded and the plain inversion of the same pixels: px/s, median
+    misfit, evaluations and roles."""
+    n = args.side * args.side
+    cfg = P.cfg_default(WL, reqmin=1e-10, konvge=100, kcount=2000, watchdog_s=900.0,
+                        seed_abort_misfit=args.seed_abort)
+    stream = C.c_void_p()
+    P.check(P.lib.photic_stream_create(C.byref(stream)))
+    d_rrs, d_ss, d_tr = P.DeviceBuffer(16 * n), P.DeviceBuffer(4 * n), P.DeviceBuffer(24 * n)
+    P.check(P.lib.photic_gen_scene(C.byref(cfg), 4, 0, n, 2e-4, 0.0, d_rrs.value, d_ss.value,
+                                   d_tr.value, stream))
+    P.check(P.lib.photic_synchronize())
+    truth_H = d_tr.to_array(np.float32, (6, n))[0]
+    del d_tr
+    fields = ("depth", "rrs_error", "status", "n_evals", "n_starts")
+    d_role = P.DeviceBuffer(4 * n)
+
+    def summary(ms, res):
+        v = res["status"] >= 0
+        dH = res["depth"][v].astype(np.float64) - truth_H[v]
+        bounded = res["depth"][v] <= RUNAWAY_M
+        return {"pixels_per_s": n / (ms / 1e3), "ms": ms,
+                "evals_per_pixel": float(res["n_evals"][v].mean()),
+                "median_rrs_error_pct": float(np.median(res["rrs_error"][v])),
+                "p90_rrs_error_pct": float(np.percentile(res["rrs_error"][v], 90)),
+                "depth_median_abs_error_m": float(np.median(np.abs(dH))),
+                "depth_rmse_bounded_m": float(np.sqrt(np.mean(dH[bounded] ** 2))),
+                "depth_runaway_frac": float((~bounded).mean())}
+
+    def ordered(oc):
+        outs = P.DeviceOutputs(cfg, n, fields=fields)
+        P.check(P.lib.photic_synchronize())
+        t0 = time.perf_counter()
+        _, stats = P.invert_ordered_device(cfg, oc, n, d_rrs.value, d_ss.value, outs,
+                                           d_role.value, stream)
+        res = outs.fetch()
+        ms = 1e3 * (time.perf_counter() - t0)
+        role = d_role.to_array(np.int32, (n,))
+        out = summary(ms, res)
+        v = role >= 0
+        out["roles"] = {name: float((role[v] == k).mean()) for k, name in
+                        enumerate(("leader", "lut", "hot_start", "restarted"))}
+        out["stats"] = stats
+        return out
+
+    def plain_or_seeded(d_seed):
+        outs = P.DeviceOutputs(cfg, n, fields=fields)
+        P.check(P.lib.photic_synchronize())
+        t0 = time.perf_counter()
+        if d_seed is not None:
+            net.seed_device(cfg, n, d_rrs.value, d_ss.value, d_seed.value, stream=stream)
+        P.invert_device(cfg, n, d_rrs.value, d_ss.value, d_seed.value if d_seed else None, outs,
+                        stream=stream)
+        res = outs.fetch()
+        return summary(1e3 * (time.perf_counter() - t0), res)
+
+    line = {"metric": "pixels/sec inverted, ordered inversion (look-up table, hot start) vs "
+                      "MLP-seeded and plain, 1 MI355X",
+            "unit": "pixels/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+            "data": "synthetic (Philox-seeded scene, no real imagery)",
+            "config": {"workload": "ordered inversion of the %dx%d single-date scene (seed 4): "
+                                   "spectral-index order, 64-pixel runs, 16-leader window, "
+                                   "the reference's match, entry and restart levels" %
+                                   (args.side, args.side), "pixels": n}}
+    for _ in range(args.warmup):
+        ordered(P.order_cfg())
+    ref = ordered(P.order_cfg())
+    line["ordered_lut"] = ref
+    line["ordered_lut_noise_angle"] = ordered(P.order_cfg(r_sigma=[2e-4] * 4))
+    line["ordered_hot_start_only"] = ordered(P.order_cfg(use_lut=0))
+    line["value"] = ref["pixels_per_s"]
+    line["ms_per_step"] = ref["ms"]
+    line["steps"], line["warmup"] = 1, args.warmup
+    line["plain"] = plain_or_seeded(None)
+    dx, dt = P.trainset_device(cfg, 101, args.train_samples)
+    net = P.Mlp(init_seed=1)
+    net.train_device(dx.value, dt.value, args.train_samples, batch=8192,
+                     epochs=args.train_epochs, lr=2e-3, lr_final=1e-4, stream=stream)
+    P.check(P.lib.photic_synchronize())
+    del dx, dt
+    line["seeded"] = plain_or_seeded(P.DeviceBuffer(20 * n))
+    line["cpu_baseline"] = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_bindings as O
+        threads, _ = host_cpus()
+        used = O.set_threads(threads)
+        ns = min(args.ordered_cpu_sample, n)
+        rrs_h = d_rrs.to_array(np.float32, (4, n))[:, :ns].copy()
+        ss_h = d_ss.to_array(np.float32, (n,))[:ns].copy()
+        t = time.perf_counter()
+        O.invert_ordered(rrs_h, ss_h)
+        dt_cpu = time.perf_counter() - t
+        line["cpu_baseline"] = {"value": ns / dt_cpu, "unit": "pixels/s", "cores": used,
+                                "kind": "port", "host": host_cpus()[1],
+                                "sample": "first %d pixels of the same scene ordered among "
+                                          "themselves, FP64 oracle, %.1f s" % (ns, dt_cpu)}
+    net.close()
+    P.lib.photic_stream_destroy(stream)
+    print(json.dumps(line))
+
 
 if __name__ == "__main__":
     main()

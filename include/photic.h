@@ -68,15 +68,26 @@ typedef struct photic_cfg {
      * (H, bottom, P, G, X). */
     double seed_rel_step[5], seed_min_step[5];
     int block, grid, chunk;                  /* launch shape; 0 = tuned defaults */
-    double watchdog_s;                       /* a kernel running longer stops (<= 0: 600 s) */
-    int check_watchdog;                      /* 1: photic_invert waits for the kernel and
-                                                returns PHOTIC_EDEVICE if the watchdog fired */
+    double watchdog_s;                       /* a kernel wavefront that finishes no pixel for
+                                                this long stops (<= 0: 600 s), leaving its
+                                                pixels unwritten */
+    int check_watchdog;                      /* 1: the call waits for its kernels and returns
+                                                PHOTIC_EDEVICE if a watchdog fired */
     /* Seeded single-date inversions only: a seed start whose best misfit is still above this
      * at its first convergence check is given up (as not converged) and the depth starts
      * follow.  0 (the default) runs every seed start to its end; a level below the early exit
      * acts as the early exit.  The seed start is this library's addition; the reference's
      * depth starts are never cut short. */
     double seed_abort_misfit;
+    /* 1: the caller has screened its pixels already (as samodel() does, on the raw values
+     * before smoothing, model/samodel.c:933-947): the single-date and stack kernels then invert
+     * every pixel and valid date with the values as given, negative ones included, instead of
+     * treating a value near cfg->nodata or below zero as no data.  0 (the default): they
+     * screen. */
+    int prescreened;
+    /* Seeded single-date inversions only: iterations between the seed start's convergence
+     * checks (0: konvge, as the depth starts). */
+    int seed_konvge;
 } photic_cfg;
 
 /* The reference's settings (model/samodel.c:2142-2147, 2216-2218): reqmin 1e-2, konvge 100,
@@ -102,6 +113,8 @@ typedef struct photic_outputs {
     int *n_evals;       /* objective evaluations made */
     int *n_iter;        /* icount of the accepted start */
     int *n_starts;      /* Nelder-Mead runs made */
+    double *rrs_model;  /* [n_bands][n]: the modelled Rrs at the optimum, the reference's
+                           Rrs_modelled (single-date inversions; 0 where not modelled) */
 } photic_outputs;
 
 /* Invert n_pix single-date pixels.  rrs: [n_bands][n_pix]; sec_sun, sec_view: [n_pix] or
@@ -275,6 +288,46 @@ int photic_invert_stack_stream(const photic_cfg *cfg, photic_mlp *mlp, long n_ro
                                const unsigned char *valid, long tile_rows,
                                const photic_outputs *out, photic_progress_fn progress,
                                void *user);
+
+/* ---- the ordered inversion: ordering, look-up table and hot start -------------------
+ * The reference inverts a scene's pixels in the order of a spectral index -- the norm of the
+ * summed Rrs at bands 1, 2, 3, times the spectral angle to the noise spectrum when every
+ * R_sigma is at least 1e-6 (model/samodel.c:701-770) -- copies the solution of a recent pixel
+ * from a look-up table when its modelled Rrs matches within 1.5 % (model_spectral_diff_mean,
+ * :980-1004, :2795-2842), and otherwise starts its one Nelder-Mead run from the previous
+ * pixel's depth, inverting again with the full depth list when the misfit stays above 5
+ * (start_at_previous and RESTART, :1013-1031, :2233-2241).  photic_invert_ordered recasts that
+ * for the GPU: the modelled pixels are sorted by the same index (a stable sort; the reference's
+ * qsort leaves ties in no set order) and cut into runs of `run`; each run's first pixel (its
+ * leader) is inverted with the full depth list; each other pixel (a follower) compares its
+ * Rrs with the modelled Rrs of its own leader and the `window` leaders before it that are LUT
+ * entries (objective below entry_max, depth below 32 m, :1034) and, at a difference of at most
+ * match_max, takes the best one's parameter vector as its solution (its outputs are that
+ * vector's on its own data: its own misfit, where the reference copies the entry's); every
+ * other follower is inverted with one start from its own leader's depth, and again with the
+ * full depth list if its objective stays above restart_misfit.  Leaders, hot starts and
+ * restarts are each one batch.  Single-date pixels (rrs [4][n], sec_sun [n] or NULL, device
+ * arrays), FP64, either parameter layout, unseeded.  role [n] (device) or NULL: -1 not
+ * modelled, 0 leader, 1 LUT match, 2 hot start, 3 hot start inverted again. */
+typedef struct photic_order_cfg {
+    int run;               /* pixels per run: a leader and run - 1 followers (64) */
+    int window;            /* earlier leaders a follower may match, besides its own (16) */
+    int use_lut;           /* 0: no look-up table, every follower hot-starts (1) */
+    double match_max;      /* largest spectral difference of a match, % (1.5) */
+    double entry_max;      /* a leader is a LUT entry below this objective (1.5) */
+    double restart_misfit; /* a hot start above this objective is inverted again (5) */
+    double r_sigma[4];     /* noise per band; all >= 1e-6: the spectral angle enters (0) */
+} photic_order_cfg;
+
+typedef struct photic_order_stats {
+    long n_valid, n_leaders, n_lut, n_hot, n_restarted;
+} photic_order_stats;
+
+/* The reference's values (model/samodel.c:895-897), 64-pixel runs, a 16-leader window. */
+void photic_order_cfg_default(photic_order_cfg *ocfg);
+int photic_invert_ordered(const photic_cfg *cfg, const photic_order_cfg *ocfg, long n_pix,
+                          const float *rrs, const float *sec_sun, const photic_outputs *out,
+                          int *role, photic_order_stats *stats, void *stream);
 
 /* The band coefficients the kernels use for cfg's wavelengths, interpolated from the
  * reference's 41-node tables (model/samodel.c:103-286) as model/samodel.c:560-618 does:

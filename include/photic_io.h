@@ -30,6 +30,26 @@ int photic_write_asc(const char *path, float **grid, int ncols, int nrows, doubl
 /* Frees a grid from photic_read_asc. */
 void photic_free_grid(float **grid, int nrows);
 
+/* The int16 packing of the reference's netCDF output grids (compress_2d, model/nc.c:265-300;
+ * samodel() writes each output grid packed this way, model/samodel.c:1494-1689): n values
+ * (row-major) are stored as short integers c with value = c * scale_factor + add_offset, and
+ * values equal to (float)spval as *missing_value = SHRT_MIN.  The arithmetic is the
+ * reference's, in float: add_offset is the smallest value, scale_factor = (largest - smallest)
+ * / 32767, c = rint((v - add_offset) / scale_factor) -- including its scan, which starts the
+ * largest value at FLT_MIN (so a grid of negative values keeps FLT_MIN as its largest) and
+ * tests a value for the largest only when it is not a new smallest (so a first value that is
+ * the grid's maximum is not seen as the largest), and its conversion to short through a
+ * 32-bit integer keeping the low 16 bits, as the reference's x86-64 build does for a quotient
+ * out of range.  Replaces compress_2d, which is not visible outside nc.c; no netCDF library is
+ * needed. */
+int photic_pack_int16(const float *values, long n, double spval, short *packed,
+                      float *add_offset, float *scale_factor, short *missing_value);
+
+/* The inverse (decompress_2d, model/nc.c:244-262): values[k] = (float)spval where packed[k]
+ * equals missing_value, else (float)packed[k] * scale_factor + add_offset in float. */
+int photic_unpack_int16(const short *packed, long n, float add_offset, float scale_factor,
+                        short missing_value, double spval, float *values);
+
 #ifdef __cplusplus
 }
 #endif

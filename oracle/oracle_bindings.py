@@ -75,6 +75,13 @@ def lib():
                                                v, l, v, v, i, v, v, C.POINTER(PyOut)]
         _lib.orc_py_eval.argtypes = [i, i, v, v, i, i, v, v, v, v, v, v]
         _lib.orc_py_region_eval.argtypes = [v, i, i, i, i, v, v, v, i, v, v, v]
+        _lib.orc_py_invert_ordered.restype = l
+        _lib.orc_py_invert_ordered.argtypes = [i, i, v, v, i, l, v, v, d, d, i, i, C.c_float,
+                                               i, i, i, d, d, d, v, v, v, v, v, v, v, v, v]
+        _lib.orc_py_spectral_diff_mean.restype = C.c_float
+        _lib.orc_py_spectral_diff_mean.argtypes = [v, v]
+        _lib.orc_py_order_key.restype = C.c_float
+        _lib.orc_py_order_key.argtypes = [v, v]
         _lib.orc_sec_deg.restype = d
         _lib.orc_sec_deg.argtypes = [d]
         _lib.orc_py_region_objective.restype = d
@@ -86,7 +93,7 @@ def lib():
         _lib.orc_py_smooth_grid.argtypes = [v, i, i, i, C.c_float, v]
         _lib.orc_py_depth_sigma.restype = i
         _lib.orc_py_depth_sigma.argtypes = [v, i, i, i, i, v, v, v, v, v, v, i, i, v, C.c_float,
-                                            d, i, i, v, v]
+                                            d, d, i, i, v, v]
         _lib.orc_py_gen_field.argtypes = [C.c_uint64, l, l, d, v, i, d, v, v, v]
         _lib.orc_py_region_gather.restype = i
         _lib.orc_py_region_gather.argtypes = [v, v, i, i, i, i, i, i, i, v, v]
@@ -602,11 +609,12 @@ def gen_field(seed, n_rows, n_cols, theta=35.0, sigma=2e-4, wl=WL4):
 
 
 def depth_sigma(depth, grid, n_rows, n_cols, nodata, rsig, sec_sun, sec_view, n_spatial=1,
-                n_bottoms=3, emp=None, emp_nodata=-99999.0, reqmin=1e-2, konvge=100, kcount=5000,
-                wl=WL4):
+                n_bottoms=3, emp=None, emp_nodata=-99999.0, h_prev=None, reqmin=1e-2, konvge=100,
+                kcount=5000, wl=WL4):
     """The oracle's depth-error pass: depth [npix] positive (0 unmodelled), grid [nd][nb][npix]
-    smoothed, nodata / rsig [nd][nb], sec_sun / sec_view [nd], emp [npix] or None.  Returns
-    (table [n_int], sigma [npix] float32)."""
+    smoothed, nodata / rsig [nd][nb], sec_sun / sec_view [nd], emp [npix] or None; h_prev the
+    start of every sample after the first without empirical depths (None: the depth of the
+    last modelled pixel in row-major order).  Returns (table [n_int], sigma [npix] float32)."""
     g = np.ascontiguousarray(grid, np.float32)
     nd, nb, npix = g.shape
     dp = np.ascontiguousarray(depth, np.float32).reshape(-1)
@@ -618,9 +626,49 @@ def depth_sigma(depth, grid, n_rows, n_cols, nodata, rsig, sec_sun, sec_view, n_
     table = np.zeros(128)
     sigma = np.zeros(npix, np.float32)
     w = np.asarray(wl, np.float64)
+    if h_prev is None:
+        modelled = np.nonzero(dp > 0)[0]
+        h_prev = float(dp[modelled[-1]]) if len(modelled) else 0.0
     n_int = lib().orc_py_depth_sigma(w.ctypes.data, nb, nd, n_rows, n_cols, dp.ctypes.data,
                                      g.ctypes.data, nod.ctypes.data, rs.ctypes.data,
                                      ss.ctypes.data, sv.ctypes.data, n_spatial, n_bottoms,
-                                     _p(em), emp_nodata, reqmin, konvge, kcount,
+                                     _p(em), emp_nodata, h_prev, reqmin, konvge, kcount,
                                      table.ctypes.data, sigma.ctypes.data)
     return table[:n_int], sigma
+
+
+def invert_ordered(rrs, sec_sun=None, mode=1, n_bottoms=2, bottom_idx=(0, 1, 2), reqmin=1e-10,
+                   konvge=100, kcount=2000, nodata=-9999.0, run=64, window=16, use_lut=1,
+                   match_max=1.5, entry_max=1.5, restart=5.0, noise=None, sec_sun0=1.0, wl=WL4):
+    """The oracle's ordered inversion (ordering, look-up table, hot start) of single-date
+    pixels rrs [4][n]; noise [4] R_sigma or None."""
+    rrs = np.ascontiguousarray(rrs, np.float32)
+    nb, n = rrs.shape
+    npar = 5 if mode == 1 else 1 + 2 * n_bottoms + 3
+    out = {"x": np.zeros((npar, n)), "depth": np.zeros(n, np.float32),
+           "rrs_error": np.zeros(n, np.float32), "model_error": np.zeros(n, np.float32),
+           "status": np.zeros(n, np.int32), "n_evals": np.zeros(n, np.int32),
+           "n_starts": np.zeros(n, np.int32), "role": np.zeros(n, np.int32)}
+    w = np.asarray(wl, np.float64)
+    bi = np.asarray(bottom_idx, np.int32)
+    ss = None if sec_sun is None else np.ascontiguousarray(sec_sun, np.float32)
+    nz = None if noise is None else np.ascontiguousarray(noise, np.float64)
+    lib().orc_py_invert_ordered(mode, n_bottoms, bi.ctypes.data, w.ctypes.data, nb, n,
+                                rrs.ctypes.data, _p(ss), sec_sun0, reqmin, konvge, kcount,
+                                nodata, run, window, use_lut, match_max, entry_max, restart,
+                                _p(nz), *[out[k].ctypes.data for k in
+                                          ("x", "depth", "rrs_error", "model_error", "status",
+                                           "n_evals", "n_starts", "role")])
+    return out
+
+
+def spectral_diff_mean(meas, model):
+    m = np.ascontiguousarray(meas, np.float64)
+    l = np.ascontiguousarray(model, np.float64)
+    return float(lib().orc_py_spectral_diff_mean(m.ctypes.data, l.ctypes.data))
+
+
+def order_key(r, noise=None):
+    rr = np.ascontiguousarray(r, np.float32)
+    nz = None if noise is None else np.ascontiguousarray(noise, np.float64)
+    return float(lib().orc_py_order_key(rr.ctypes.data, _p(nz)))

@@ -719,14 +719,16 @@ int orc_py_depth_sigma(const double *wl, int nb, int nd, int n_rows, int n_cols,
                        const float *depth, const float *grid, const float *nodata,
                        const double *rsig, const double *sec_sun, const double *sec_view,
                        int n_spatial, int n_bottoms, const float *emp, float emp_nodata,
-                       double reqmin, int konvge, int kcount, double *table, float *sigma)
+                       double h_prev, double reqmin, int konvge, int kcount, double *table,
+                       float *sigma)
 {
     orc_bands bands;
     orc_band_tables(wl, nb, &bands);
     orc_nm_cfg cfg;
     orc_default_cfg(&cfg, nd, reqmin, konvge, kcount);
     return orc_depth_sigma(depth, n_rows, n_cols, grid, nodata, rsig, nd, &bands, sec_sun,
-                           sec_view, n_spatial, n_bottoms, emp, emp_nodata, &cfg, table, sigma);
+                           sec_view, n_spatial, n_bottoms, emp, emp_nodata, h_prev, &cfg, table,
+                           sigma);
 }
 
 /* One perturbed region inversion (the depth-error pass's sample): pixel (i, j) of grid
@@ -812,3 +814,29 @@ void orc_py_region_eval(const double *wl, int nb, int nd, int R, int origin, con
     out[3] = res.iod;
     free(px);
 }
+
+/* The ordered inversion (orc_invert_ordered) with the given settings; noise [4] or NULL. */
+long orc_py_invert_ordered(int mode, int n_bottoms, const int *bottom_idx, const double *wl,
+                           int nb, long n, const float *rrs, const float *sec_sun,
+                           double sec_sun0, double reqmin, int konvge, int kcount, float nodata,
+                           int run, int window, int use_lut, double match_max,
+                           double entry_max, double restart, const double *noise, double *x,
+                           float *depth, float *rrs_error, float *model_error, int *status,
+                           int *n_evals, int *n_starts, int *role)
+{
+    orc_bands bands;
+    orc_band_tables(wl, nb, &bands);
+    orc_nm_cfg cfg;
+    orc_default_cfg(&cfg, 1, reqmin, konvge, kcount);
+    return orc_invert_ordered(mode, n_bottoms, bottom_idx, &bands, n, rrs, sec_sun, sec_sun0,
+                              &cfg, nodata, run, window, use_lut, match_max, entry_max, restart,
+                              noise, x, depth, rrs_error, model_error, status, n_evals,
+                              n_starts, role);
+}
+
+float orc_py_spectral_diff_mean(const double *meas, const double *model)
+{
+    return orc_spectral_diff_mean(meas, model);
+}
+
+float orc_py_order_key(const float *r, const double *noise) { return orc_order_key(r, noise); }

@@ -422,15 +422,25 @@ void orc_region_optimise(orc_region_pixel *px, const orc_nm_cfg *cfg, orc_nelmin
     free(buf);
 }
 
-/* The depth-error pass of model/samodel.c:1376-1482 as the drop-in restates it: per 0.25 m
- * interval up to the deepest depth (at most 30 m), 128 samples, each a pixel drawn with up to
- * sqrt(n_rows n_cols) tries (Philox4x32-10, counter (interval * 128 + sample, try, 300, 0))
- * whose depth lies strictly inside the interval, perturbed by n_sigma R_sigma with n_sigma
- * = +-u (sign from u_0 < 0.5, counter (.., 0, 301, 0)), inverted again -- independently, from
- * the main pass's start rules -- and the interval's standard deviation of the sampled depths
- * (mean rounded to float, zeros skipped, as vec_mean2_double / vec_stddev_double).  depth
- * [npix] the positive retrieved depths (0: unmodelled); emp [npix] the empirical depths or
- * NULL.  Writes table [n_int] and sigma [npix]; returns n_int. */
+/* The depth-error pass of model/samodel.c:1376-1482: per 0.25 m interval up to the deepest
+ * depth (at most 30 m), 128 samples, each a pixel drawn with up to sqrt(n_rows n_cols) tries
+ * (Philox4x32-10, counter (interval * 128 + sample, try, 300, 0), where the reference calls
+ * rand()) whose depth lies strictly inside the interval, perturbed by n_sigma R_sigma with
+ * n_sigma = +-u (sign from u_0 < 0.5, counter (.., 0, 301, 0)), and inverted again; the
+ * interval's sigma is the standard deviation of the sampled depths (mean rounded to float,
+ * zeros skipped, as vec_mean2_double / vec_stddev_double).
+ *
+ * The starts follow the reference's start_at_previous rule: the first sample inverted (the
+ * first in interval and sample order) runs the main pass's start rules; after it the
+ * reference sets start_at_previous (:1454), so every later sample is a single start
+ * (:2233-2241) from depth_prev -- which the pass itself never updates (it is set only in the
+ * main loop, :1069, :1115), so it stays the depth of the main pass's last modelled pixel:
+ * h_prev here.  With an empirical depth grid every sample starts from its own empirical
+ * depth (the rule then keeps Hstart_quick).  h_prev < 0 instead inverts every sample with the
+ * main pass's start rules (the independent-sample variant, for comparison).  depth [npix] the
+ * positive retrieved depths (0:
+ * unmodelled); emp [npix] the empirical depths or NULL.  Writes table [n_int] and sigma
+ * [npix]; returns n_int. */
 #define ORC_SIGMA_KEY 0x0000414D47495344ull
 
 static void orc_sigma_uniforms(uint32_t c0, uint32_t c1, uint32_t c2, double *u0, double *u1)
@@ -446,8 +456,8 @@ static void orc_sigma_uniforms(uint32_t c0, uint32_t c1, uint32_t c2, double *u0
 int orc_depth_sigma(const float *depth, int n_rows, int n_cols, const float *grid,
                     const float *nodata, const double *rsig, int nd, const orc_bands *bands,
                     const double *sec_sun, const double *sec_view, int n_spatial, int n_bottoms,
-                    const float *emp, float emp_nodata, const orc_nm_cfg *nm, double *table,
-                    float *sigma)
+                    const float *emp, float emp_nodata, double h_prev, const orc_nm_cfg *nm,
+                    double *table, float *sigma)
 {
     const long npix = (long)n_rows * n_cols;
     const int nb = bands->n_bands;
@@ -466,21 +476,33 @@ int orc_depth_sigma(const float *depth, int n_rows, int n_cols, const float *gri
     const int n_trials = (int)sqrt((double)n_rows * n_cols);
     const long n_s = (long)n_int * 128;
     double *trial = (double *)calloc((size_t)n_s, sizeof(double));
+    long *pick = (long *)malloc(sizeof(long) * (size_t)n_s);
 
-#pragma omp parallel for schedule(dynamic, 1)
+    /* every sample's pixel (or -1: none found in the interval) */
+#pragma omp parallel for schedule(dynamic, 16)
     for (long q = 0; q < n_s; q++) {
         const double d = 0.25 * (double)(q / 128);
-        long pix = -1;
+        pick[q] = -1;
         for (int t = 0; t < n_trials; t++) {
             double u0, u1;
             orc_sigma_uniforms((uint32_t)q, (uint32_t)t, 300, &u0, &u1);
             const int i = (int)(u0 * n_rows), j = (int)(u1 * n_cols);
             const float v = depth[(long)i * n_cols + j];
             if (v > d && v < d + 0.25) {
-                pix = (long)i * n_cols + j;
+                pick[q] = (long)i * n_cols + j;
                 break;
             }
         }
+    }
+    /* the first sample the pass inverts (a found pixel with, given empirical depths, one) */
+    long first = -1;
+    for (long q = 0; q < n_s && first < 0; q++)
+        if (pick[q] >= 0 && (!emp || !orc_approx_equal(emp[pick[q]], emp_nodata, 1.0e-6f)))
+            first = q;
+
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long q = 0; q < n_s; q++) {
+        const long pix = pick[q];
         if (pix < 0) continue;
         double us, um;
         orc_sigma_uniforms((uint32_t)q, 0, 301, &us, &um);
@@ -494,6 +516,9 @@ int orc_depth_sigma(const float *depth, int n_rows, int n_cols, const float *gri
             cfg.n_hstarts = 1;
             cfg.hstart[0] = (double)h;
             if (h > 8.0) nbot = 1;
+        } else if (q != first && h_prev >= 0.0) {
+            cfg.n_hstarts = 1; /* start_at_previous: one start from the main pass's last depth */
+            cfg.hstart[0] = h_prev;
         }
         orc_region_pixel *px = (orc_region_pixel *)calloc(1, sizeof(orc_region_pixel));
         orc_region_result *res = (orc_region_result *)calloc(1, sizeof(orc_region_result));
@@ -526,6 +551,7 @@ int orc_depth_sigma(const float *depth, int n_rows, int n_cols, const float *gri
             }
         table[k] = count == 0 ? 0.0 : sqrt(sumdev / (double)count);
     }
+    free(pick);
     for (long p = 0; p < npix; p++) {
         const float v = depth[p];
         if (!(v > 0.0f)) continue;

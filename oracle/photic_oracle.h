@@ -231,11 +231,25 @@ void orc_region_start_step(const orc_region_pixel *px, double h0, double *start,
 void orc_region_eval_outputs(orc_region_pixel *px, const double *x, orc_region_result *res);
 void orc_region_optimise(orc_region_pixel *px, const orc_nm_cfg *cfg, orc_nelmin_fn nm,
                          orc_region_result *res);
+/* The ordered inversion (orc_order.c): the spectral index, model_spectral_diff_mean for one
+ * region and scene, and the whole run-and-leader inversion. */
+float orc_order_key(const float *r, const double *noise);
+float orc_spectral_diff_mean(const double *meas, const double *model);
+long orc_invert_ordered(int mode, int n_bottoms, const int *bottom_idx, const orc_bands *bands,
+                        long n, const float *rrs, const float *sec_sun, double sec_sun0,
+                        const orc_nm_cfg *cfg, float nodata, int run, int window, int use_lut,
+                        double match_max, double entry_max, double restart,
+                        const double *noise, double *x, float *depth, float *rrs_error,
+                        float *model_error, int *status, int *n_evals, int *n_starts, int *role);
+
+/* The depth-error pass (model/samodel.c:1376-1482); h_prev: the depth of the main pass's
+ * last modelled pixel, the start of every sample after the first when there is no empirical
+ * depth grid. */
 int orc_depth_sigma(const float *depth, int n_rows, int n_cols, const float *grid,
                     const float *nodata, const double *rsig, int nd, const orc_bands *bands,
                     const double *sec_sun, const double *sec_view, int n_spatial, int n_bottoms,
-                    const float *emp, float emp_nodata, const orc_nm_cfg *nm, double *table,
-                    float *sigma);
+                    const float *emp, float emp_nodata, double h_prev, const orc_nm_cfg *nm,
+                    double *table, float *sigma);
 
 /* --- synthetic scenes (Philox4x32-10, Salmon et al. 2011) --- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -51,6 +51,8 @@ class Cfg(C.Structure):
         ("watchdog_s", C.c_double),
         ("check_watchdog", C.c_int),
         ("seed_abort_misfit", C.c_double),
+        ("prescreened", C.c_int),
+        ("seed_konvge", C.c_int),
     ]
 
 
@@ -69,11 +71,23 @@ class Outputs(C.Structure):
         ("n_evals", C.c_void_p),
         ("n_iter", C.c_void_p),
         ("n_starts", C.c_void_p),
+        ("rrs_model", C.c_void_p),
     ]
 
 
 # progress callback of photic_invert_stack_stream: (tiles done, tiles, user)
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_long, C.c_long, C.c_void_p)
+
+
+class OrderCfg(C.Structure):
+    _fields_ = [("run", C.c_int), ("window", C.c_int), ("use_lut", C.c_int),
+                ("match_max", C.c_double), ("entry_max", C.c_double),
+                ("restart_misfit", C.c_double), ("r_sigma", C.c_double * 4)]
+
+
+class OrderStats(C.Structure):
+    _fields_ = [(k, C.c_long) for k in ("n_valid", "n_leaders", "n_lut", "n_hot",
+                                        "n_restarted")]
 
 
 class Tuning(C.Structure):
@@ -126,6 +140,9 @@ def _load():
         "photic_mlp_seed_stack": (i, [vp, C.POINTER(Cfg), l, i, vp, vp, vp, vp, vp]),
         "photic_invert_stack_stream": (i, [C.POINTER(Cfg), vp, l, l, i, vp, vp, vp, l,
                                            C.POINTER(Outputs), PROGRESS_FN, vp]),
+        "photic_order_cfg_default": (None, [C.POINTER(OrderCfg)]),
+        "photic_invert_ordered": (i, [C.POINTER(Cfg), C.POINTER(OrderCfg), l, vp, vp,
+                                      C.POINTER(Outputs), vp, C.POINTER(OrderStats), vp]),
         "photic_invert_h": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, vp, C.POINTER(Outputs),
                                 vp]),
         "photic_band_coefficients": (i, [C.POINTER(Cfg), vp, C.POINTER(d)]),
@@ -153,6 +170,11 @@ def _load():
 
 
 lib = _load()
+
+
+def last_error():
+    """The message of the library's last failure on this thread."""
+    return lib.photic_last_error().decode()
 
 
 def check(rc):
@@ -234,8 +256,10 @@ _OUT_FIELDS = {
     "albedo": (np.float32, 1), "k_min": (np.float32, 1), "iod": (np.float32, 1),
     "b_pct": (np.float32, 3), "bottom_type": (np.float32, 1), "x": (np.float64, None),
     "status": (np.int32, 1), "n_evals": (np.int32, 1), "n_iter": (np.int32, 1),
-    "n_starts": (np.int32, 1),
+    "n_starts": (np.int32, 1), "rrs_model": (np.float64, 4),
 }
+# fields the stack and region kernels do not write
+_SINGLE_DATE_ONLY = ("rrs_model",)
 
 
 class DeviceOutputs:
@@ -304,6 +328,46 @@ def invert(cfg, rrs, sec_sun=None, seed=None, fields=None):
     return outs.fetch()
 
 
+def order_cfg(**kw):
+    """The ordered inversion's settings (photic_order_cfg_default), keyword overrides."""
+    o = OrderCfg()
+    lib.photic_order_cfg_default(C.byref(o))
+    for k, v in kw.items():
+        if k == "r_sigma":
+            for j, x in enumerate(v):
+                o.r_sigma[j] = x
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def invert_ordered_device(cfg, ocfg, n, d_rrs, d_sec_sun=None, outs=None, d_role=None,
+                          stream=None):
+    """photic_invert_ordered on device pointers; returns (DeviceOutputs, stats dict)."""
+    if outs is None:
+        outs = DeviceOutputs(cfg, n)
+    st = OrderStats()
+    check(lib.photic_invert_ordered(C.byref(cfg), C.byref(ocfg), n, d_rrs, d_sec_sun,
+                                    C.byref(outs.struct), d_role, C.byref(st), stream))
+    return outs, {k: getattr(st, k) for k, _ in OrderStats._fields_}
+
+
+def invert_ordered(cfg, ocfg, rrs, sec_sun=None, fields=None):
+    """The ordered inversion of host arrays rrs [4][n], sec_sun [n]; returns (outputs with
+    "role", stats)."""
+    rrs = np.ascontiguousarray(rrs, dtype=np.float32)
+    n = rrs.shape[1]
+    d_rrs = DeviceBuffer.from_array(rrs)
+    d_ss = DeviceBuffer.from_array(np.asarray(sec_sun, np.float32)) if sec_sun is not None else None
+    d_role = DeviceBuffer(4 * max(n, 1))
+    outs, stats = invert_ordered_device(cfg, ocfg, n, d_rrs.value, d_ss.value if d_ss else None,
+                                        DeviceOutputs(cfg, n, fields), d_role.value)
+    check(lib.photic_synchronize())
+    res = outs.fetch()
+    res["role"] = d_role.to_array(np.int32, (n,))
+    return res, stats
+
+
 class StackOutputs(DeviceOutputs):
     """Device output buffers for n stacks of n_dates dates (x is [off + 3 n_dates][n], off = 2
     in the mix layout, 1 + 2 n_bottoms in the native one)."""
@@ -314,6 +378,8 @@ class StackOutputs(DeviceOutputs):
         self.bufs = {}
         self.struct = Outputs()
         for name, (dt, rows) in _OUT_FIELDS.items():
+            if name in _SINGLE_DATE_ONLY:
+                continue
             if fields is not None and name not in fields:
                 continue
             rows = self.np if rows is None else rows
@@ -368,6 +434,8 @@ class RegionOutputs(DeviceOutputs):
         self.bufs = {}
         self.struct = Outputs()
         for name, (dt, rows) in _OUT_FIELDS.items():
+            if name in _SINGLE_DATE_ONLY:
+                continue
             if fields is not None and name not in fields:
                 continue
             rows = self.np if rows is None else rows
@@ -440,6 +508,8 @@ def invert_stack_stream(cfg, n_rows, n_cols, rrs, sec_sun, valid, tile_rows=2048
     struct = Outputs()
     res = {}
     for name, (dt, rows) in _OUT_FIELDS.items():
+        if name in _SINGLE_DATE_ONLY:
+            continue
         if fields is not None and name not in fields:
             continue
         rows = 2 + 3 * nd if rows is None else rows

@@ -28,8 +28,8 @@ CC_FLAGS = [
     "-I" + os.path.join(ROCM, "include"), "-I" + CSRC, "-I" + os.path.join(ROOT, "include"),
 ]
 HIP_SOURCES = ["ph_kernels.hip", "ph_mlp.hip", "ph_stack.hip", "ph_stack_native.hip",
-               "ph_region.hip"]
-C_SOURCES = ["photic_capi.c", "photic_samodel.c", "photic_io.c"]
+               "ph_region.hip", "ph_order.hip"]
+C_SOURCES = ["photic_capi.c", "photic_samodel.c", "photic_io.c", "photic_order.c"]
 
 
 def _run(cmd):

@@ -501,7 +501,8 @@ PH_INLINE bool ph_pixel_init(PhPixel<FT, ND, NB>& px, const PhConst& c,
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             float v = rrs_in[s][b];
-            if (ph_approx_equal(v, (float)c.nodata, 1.0e-6f) || v < 0.0f) ok = false;
+            if (!c.prescreened && (ph_approx_equal(v, (float)c.nodata, 1.0e-6f) || v < 0.0f))
+                ok = false;
             r[b] = (double)v;
             px.rrs[s][b] = r[b];
             tot += r[b];
@@ -635,7 +636,7 @@ PH_INLINE void ph_nm_begin(PhNM<N>& S, const PhConst& c)
 {
     S.icount = 0;
     S.numres = 0;
-    S.jcount = c.konvge;
+    S.jcount = S.kstart < 0 ? c.seed_konvge : c.konvge;
     S.del = 1.0;
     S.phase = PH_BUILD;
     S.sub = N;
@@ -825,7 +826,7 @@ PH_INLINE void ph_nm_consume(PhNM<N>& S, const PhConst& c, double f)
             give_up = true;
             nm_finish = true;
         } else if (S.icount <= c.kcount) {
-            S.jcount = c.konvge;
+            S.jcount = S.kstart < 0 ? c.seed_konvge : c.konvge;
             double z = 0.0;
             ph_for<N + 1>([&](auto i) PH_BODY { z = z + S.y[i]; });
             const double xm = ph_div_const(z, dnn, 1.0 / dnn);

@@ -6,6 +6,7 @@
 #ifndef PH_LAUNCH_H
 #define PH_LAUNCH_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #define PH_MAXB 8  /* bands per date */
@@ -23,6 +24,8 @@ typedef struct PhConst {
     int n_bottoms;           /* native mode */
     int bottom_idx[PH_NBOT]; /* native mode */
     int konvge, kcount, n_hstarts, has_seed;
+    int prescreened;         /* 1: no no-data / negativity screen of the input values */
+    int seed_konvge;         /* iterations between convergence checks in a seed start */
     double reqmin, rq, early_exit, nodata;
     double sec_sun0, sec_view0; /* geometry where the per-pixel arrays are NULL */
     double hstart[PH_MAXH];
@@ -52,12 +55,13 @@ typedef struct PhInvertArgs {
     const float *hstart;   /* [n_pix]: one depth start per pixel (c.n_hstarts == 1), or NULL */
     unsigned long long *queue; /* [0] work counter, [1] watchdog flag; zero at launch */
     int chunk;
-    unsigned long long max_ticks; /* watchdog: wall-clock ticks before every wave gives up */
+    unsigned long long max_ticks; /* watchdog: wall-clock ticks a wave may go without progress */
     float *depth, *rrs_error, *model_error, *albedo, *k_min, *iod;
     float *b_pct; /* [3][n_pix]: sand, seagrass, coral percent */
     float *bottom_type;
     double *x_out; /* [n_params][n_pix], |best| */
     int *status, *n_evals, *n_iter, *n_starts;
+    double *rrs_model; /* [n_bands][n_pix]: modelled Rrs at the optimum */
 } PhInvertArgs;
 
 /* One launch of the stack inversion for the pixels with exactly k valid dates (k = the
@@ -160,6 +164,32 @@ int ph_launch_gen_stack(const PhConst *c, uint64_t seed, long pix0, long n, int 
 /* spatial regions (ph_region.hip) */
 int ph_launch_region(const PhRegionArgs *a, int grid, void *stream);
 int ph_region_occupancy(const PhRegionArgs *a, int *blocks_per_cu);
+
+/* the ordered inversion (ph_order.hip) */
+int ph_launch_order_key(const PhInvertArgs *a, int noise, const double *nsig, float *key, int *pix,
+                        unsigned int *n_valid, int *role_px, void *stream);
+int ph_order_sort(const float *key_in, float *key_out, const int *val_in, int *val_out, long n,
+                  void *tmp, size_t *tmp_bytes, void *stream);
+int ph_launch_order_decide(long n_valid, int run, int window, int use_lut, double match_max,
+                           double entry_max, const float *rrs, long n_pix, const int *order,
+                           const float *l_depth, const float *l_objective, const int *l_status,
+                           const double *l_rrs, long n_lead, int *role, int *src, float *hstart,
+                           void *stream);
+int ph_launch_order_list(long n, const int *role, int want, const int *flag, const int *map,
+                         int *list, unsigned int *count, void *stream);
+int ph_launch_order_roles(long n, const int *list, const int *role_s, const int *order,
+                          int *role_px, void *stream);
+int ph_launch_order_gather(long m, const int *list, long stride, const int *order,
+                           const float *rrs, const float *sec_sun, long n_pix, const float *h_in,
+                           float *rrs_out, float *ss_out, float *h_out, void *stream);
+int ph_launch_order_adopt(const PhInvertArgs *a, int n_bottoms, long m, const int *list,
+                          const int *order, const int *src, const double *l_x,
+                          const int *l_status, long n_lead, void *stream);
+int ph_launch_order_scatter(const struct photic_outputs *dst, const struct photic_outputs *from,
+                            long m, const int *list, long stride, const int *order, long n_pix,
+                            int n_params, int accumulate, void *stream);
+int ph_launch_order_restart(long m, const float *objective, double level, int *flag,
+                            void *stream);
 
 /* the warm-start network (ph_mlp.hip) */
 int ph_mlp_nparam(void);

@@ -670,7 +670,9 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
     S.nd = nd;
     S.nd_magic = (65536 + nd - 1) / nd;
     const long npix = (long)a.n_rows * a.n_cols;
-    const unsigned long long t0 = wall_clock64();
+    // watchdog: the wall clock at the wave's last progress (a pixel taken); a wave that
+    // finishes no pixel within max_ticks gives up and raises the flag
+    unsigned long long t0 = wall_clock64();
     double* g = S.g;
     // the launch's band coefficients ([band][a0, a1, aw, bbw, eg, bottoms]) and each scene's
     // geometry, once per wave
@@ -704,6 +706,7 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
         if (li == 0) qq = atomicAdd(a.queue, 1ull);
         const long q = (long)__shfl(qq, 0);
         if (q >= a.n) break;
+        t0 = wall_clock64();
         const long p = a.idx ? a.idx[q] : q;
         if (p < 0 || p >= npix) { // a listed index outside the scene: not modelled
             if (li == 0) ph_reg_write_nodata(a, q, 0);

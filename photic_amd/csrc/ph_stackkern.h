@@ -124,7 +124,8 @@ __device__ __attribute__((always_inline)) inline bool ph_stack_load(
     }
     const double rv = ph_band_sel(rp, b);
     const float v = (float)rv;
-    const bool bad = li < NP && (ph_approx_equal(v, (float)c.nodata, 1.0e-6f) || v < 0.0f);
+    const bool bad = li < NP && !c.prescreened &&
+                     (ph_approx_equal(v, (float)c.nodata, 1.0e-6f) || v < 0.0f);
     if (__ballot(bad) & gmask) return false;
 
     // the sum of the measured Rrs, in (date, band) order
@@ -225,7 +226,9 @@ __global__ void __launch_bounds__(256, 3) ph_stack_kernel(PhStackArgs a)
     const long m = (long)*a.count;
     long q_next = 0, q_end = 0;
     bool exhausted = false;
-    const unsigned long long t0 = wall_clock64();
+    // watchdog: the wall clock at the wave's last progress (a chunk taken, a stack retired); a
+    // wave that makes none for max_ticks stops and raises the flag
+    unsigned long long t0 = wall_clock64();
 
     for (;;) {
         if (wall_clock64() - t0 > a.max_ticks) {
@@ -246,6 +249,7 @@ __global__ void __launch_bounds__(256, 3) ph_stack_kernel(PhStackArgs a)
                     q_next = (long)base;
                     q_end = q_next + a.chunk;
                     if (q_end > m) q_end = m;
+                    t0 = wall_clock64();
                 }
             }
             if (!exhausted || q_next < q_end) {
@@ -286,6 +290,7 @@ __global__ void __launch_bounds__(256, 3) ph_stack_kernel(PhStackArgs a)
         }
 
         // ---- one evaluation per group, in lock-step ----
+        bool retired = false;
         if (S.phase != PH_IDLE) {
             PhGrpSide side;
             const bool fin = S.phase == PH_FINAL; // the final re-evaluation at the best point
@@ -303,8 +308,10 @@ __global__ void __launch_bounds__(256, 3) ph_stack_kernel(PhStackArgs a)
                 }
                 ph_stack_write_outputs<ND, NBOT>(a, c, p, vmask, dst, S, qv, side);
                 S.phase = PH_IDLE;
+                retired = true;
             }
         }
+        if (__ballot(retired) != 0ull) t0 = wall_clock64();
     }
 }
 

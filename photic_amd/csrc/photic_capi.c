@@ -201,8 +201,12 @@ static int ph_make_const(const photic_cfg *cfg, int n_dates, int has_seed, PhCon
     c->reqmin = cfg->reqmin;
     int n = cfg->mode == PHOTIC_MODE_MIX ? 2 + 3 * n_dates : 1 + 2 * cfg->n_bottoms + 3 * n_dates;
     c->rq = cfg->reqmin * (double)n; /* asa047.c:170 */
-    c->early_exit = cfg->early_exit > 0.0 ? cfg->early_exit : 2.5 * (double)(float)n_dates;
+    /* the early exit of a problem over n_dates dates or scenes: 2.5 (or the configured level)
+     * per date, model/samodel.c:2404 */
+    c->early_exit = (cfg->early_exit > 0.0 ? cfg->early_exit : 2.5) * (double)(float)n_dates;
     c->nodata = (double)cfg->nodata;
+    c->prescreened = cfg->prescreened ? 1 : 0;
+    c->seed_konvge = cfg->seed_konvge > 0 ? cfg->seed_konvge : cfg->konvge;
     c->sec_sun0 = cfg->sec_sun;
     c->sec_view0 = cfg->sec_view;
     for (int i = 0; i < PH_MAXH; i++)
@@ -302,9 +306,12 @@ typedef struct {
 static ph_shape ph_tuned_single[2][4][2]; /* [fp32][n_bottoms (0 = mix)][seeded] */
 static ph_shape ph_tuned_stack[2];        /* [seeded] */
 
-int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const float *sec_sun,
-                    const float *sec_view, const float *seed, const float *hstart,
-                    const photic_outputs *out, void *stream)
+/* The single-date inversion.  scratch: the launch's work counter and watchdog flag (two
+ * words of device memory) from the caller, or NULL to allocate them on the stream. */
+static int ph_invert_single(const photic_cfg *cfg, long n_pix, const float *rrs,
+                            const float *sec_sun, const float *sec_view, const float *seed,
+                            const float *hstart, const photic_outputs *out, void *stream,
+                            unsigned long long *scratch)
 {
     PhInvertArgs a;
     int rc = ph_have_device();
@@ -379,10 +386,11 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
     a.n_evals = out->n_evals;
     a.n_iter = out->n_iter;
     a.n_starts = out->n_starts;
+    a.rrs_model = out->rrs_model;
 
-    /* watchdog: an inversion kernel still running after this long gives up (every wave
-     * exits, leaving its pixels unwritten); with cfg->check_watchdog the call waits for the
-     * kernel and reports PHOTIC_EDEVICE when that happened */
+    /* watchdog: a wavefront that finishes no pixel for this long gives up (it exits, leaving
+     * its pixels unwritten); with cfg->check_watchdog the call waits for the kernel and
+     * reports PHOTIC_EDEVICE when that happened */
     int wrate = 0;
     if (hipDeviceGetAttribute(&wrate, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
         wrate <= 0)
@@ -391,10 +399,13 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
     a.max_ticks = (unsigned long long)(limit_s * 1000.0 * (double)wrate);
 
     hipStream_t st = (hipStream_t)stream;
-    unsigned long long *queue = NULL;
-    rc = ph_hip(hipMallocAsync((void **)&queue, 2 * sizeof(unsigned long long), st),
-                "hipMallocAsync");
-    if (rc) return rc;
+    unsigned long long *queue = scratch;
+    if (!queue) {
+        rc = ph_hip(hipMallocAsync((void **)&queue, 2 * sizeof(unsigned long long), st),
+                    "hipMallocAsync");
+        if (rc) return rc;
+    }
+    /* cleared by a fill kernel, which a graph capture records as a kernel node */
     rc = ph_launch_status(ph_launch_zero_u64(queue, 2, st), "zero work counter");
     if (rc == PHOTIC_OK) {
         a.queue = queue;
@@ -410,8 +421,16 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
             rc = ph_fail(PHOTIC_EDEVICE, "inversion kernel stopped by its %.0f s watchdog",
                          limit_s);
     }
+    if (scratch) return rc;
     int rc2 = ph_hip(hipFreeAsync(queue, st), "hipFreeAsync");
     return rc ? rc : rc2;
+}
+
+int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const float *sec_sun,
+                    const float *sec_view, const float *seed, const float *hstart,
+                    const photic_outputs *out, void *stream)
+{
+    return ph_invert_single(cfg, n_pix, rrs, sec_sun, sec_view, seed, hstart, out, stream, NULL);
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -543,7 +562,6 @@ static int ph_invert_stack(const photic_cfg *cfg, long n_pix, int n_dates, const
         photic_cfg one = *cfg;
         rc = ph_make_const(&one, k, seed != NULL, &a.c);
         if (rc) break;
-        a.c.early_exit = (cfg->early_exit > 0.0 ? cfg->early_exit : 2.5) * (double)(float)k;
         a.idx = lists + (long)k * n_pix;
         a.count = counts + k;
         a.queue = queues + 2 * k;
@@ -660,7 +678,6 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
     memset(&a, 0, sizeof(a));
     rc = ph_make_const(cfg, n_scenes, 0, &a.c);
     if (rc) return rc;
-    a.c.early_exit = (cfg->early_exit > 0.0 ? cfg->early_exit : 2.5) * (double)(float)n_scenes;
     if (hstart) a.c.n_hstarts = 1; /* the pixel's own depth start replaces the list */
     a.n_rows = n_rows;
     a.n_cols = n_cols;

@@ -1,7 +1,12 @@
 /* photic_amd -- C host code: ESRI ASCII grids (include/photic_io.h), the reference's
- * real-scene grid format (model/io.c:143-269). */
+ * real-scene grid format (model/io.c:143-269), and the int16 packing of its netCDF output
+ * grids (model/nc.c:244-300). */
 #include "photic_io.h"
 
+#include <float.h>
+#include <limits.h>
+#include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -88,5 +93,56 @@ int photic_write_asc(const char *path, float **grid, int ncols, int nrows, doubl
         fprintf(fp, "\n");
     }
     if (fclose(fp) != 0) return ph_error(PHOTIC_EINVAL, "photic_write_asc: write failed");
+    return PHOTIC_OK;
+}
+
+/* rint(q) converted to short as the reference's x86-64 build converts it: through a 32-bit
+ * integer (cvttsd2si; a NaN or a value outside int's range gives INT_MIN), keeping its low 16
+ * bits. */
+static short io_to_short(double r)
+{
+    int32_t t = (r > -2147483649.0 && r < 2147483648.0) ? (int32_t)r : INT32_MIN;
+    return (short)(int16_t)(uint16_t)(uint32_t)t;
+}
+
+int photic_pack_int16(const float *values, long n, double spval, short *packed,
+                      float *add_offset, float *scale_factor, short *missing_value)
+{
+    if (n < 0 || (n > 0 && (!values || !packed)) || !add_offset || !scale_factor ||
+        !missing_value)
+        return ph_error(PHOTIC_EINVAL, "photic_pack_int16: a NULL argument or n < 0");
+    const float fspv = (float)spval;
+    *missing_value = SHRT_MIN;
+    float lo = FLT_MAX, hi = FLT_MIN;
+    for (long k = 0; k < n; k++) {
+        const float v = values[k];
+        if (v == fspv) continue;
+        if (v < lo) lo = v;
+        else if (v > hi) hi = v;
+    }
+    const float scale = (hi - lo) / (float)SHRT_MAX;
+    *add_offset = lo;
+    *scale_factor = scale;
+    for (long k = 0; k < n; k++) {
+        const float v = values[k];
+        packed[k] = v == fspv ? *missing_value : io_to_short(rint((double)((v - lo) / scale)));
+    }
+    return PHOTIC_OK;
+}
+
+int photic_unpack_int16(const short *packed, long n, float add_offset, float scale_factor,
+                        short missing_value, double spval, float *values)
+{
+    if (n < 0 || (n > 0 && (!packed || !values)))
+        return ph_error(PHOTIC_EINVAL, "photic_unpack_int16: a NULL argument or n < 0");
+    for (long k = 0; k < n; k++) {
+        if (packed[k] == missing_value) {
+            values[k] = (float)spval;
+        } else {
+            const float c = (float)packed[k];
+            const float t = c * scale_factor;
+            values[k] = t + add_offset;
+        }
+    }
     return PHOTIC_OK;
 }

@@ -175,6 +175,15 @@ static int sm_invert_regions(const photic_cfg *cfg, const float *d_grid, const f
     if (!rc) rc = photic_memcpy_htod(d_idx, ix, sizeof(int64_t) * (size_t)n);
     if (!rc && d_h) rc = photic_memcpy_htod(d_h, hstart, sizeof(float) * (size_t)n);
     if (!rc && d_pert) rc = photic_memcpy_htod(d_pert, pert, sizeof(float) * (size_t)n);
+    /* every pixel starts at the reference's defaults with status -1, so a pixel the kernel
+     * never reaches reads as not modelled */
+    for (long q = 0; !rc && q < n; q++) {
+        for (int k = 0; k < 5; k++) h[(long)k * n + q] = 0.0f;
+        for (int k = 5; k < 9; k++) h[(long)k * n + q] = -9999.0f;
+        st[q] = -1;
+    }
+    if (!rc) rc = photic_memcpy_htod(d_out, h, sizeof(float) * 9 * (size_t)n);
+    if (!rc) rc = photic_memcpy_htod(d_st, st, sizeof(int) * (size_t)n);
     if (!rc) {
         photic_outputs o;
         memset(&o, 0, sizeof(o));
@@ -245,10 +254,13 @@ static int sm_run_regions(const photic_cfg *cfg, const float *d_grid, const floa
  * a pixel drawn at random (up to sqrt(n_rows n_cols) tries) whose depth lies in the interval,
  * its reflectances perturbed by n_sigma R_sigma with n_sigma uniform in (-1, 1), and inverted
  * again; the interval's sigma is the standard deviation of the sampled depths, and every
- * modelled pixel gets its interval's sigma.  The reference draws with rand() seeded from the
- * clock and starts each sample from the previous sample's result; here the draws come from
- * Philox4x32-10 with a fixed key and every sample is inverted independently, from the same
- * start rules as the main pass, in one batched call. */
+ * modelled pixel gets its interval's sigma.  The starts are the reference's: the first sample
+ * inverted runs the main pass's start rules, every later one a single start (start_at_previous,
+ * :1454, :2233-2241) from depth_prev, which the pass never updates -- the depth of the main
+ * pass's last modelled pixel (the reference's loop order; row-major here) -- unless an
+ * empirical depth grid gives each sample its own start.  So the samples are independent of
+ * one another and are inverted in one batched call.  The reference draws with rand() seeded
+ * from the clock; here the draws come from Philox4x32-10 with a fixed key. */
 #define SM_SIGMA_SAMPLES 128
 #define SM_SIGMA_INTERVAL 0.25
 #define SM_SIGMA_KEY 0x0000414D47495344ull /* the key of the sample draws */
@@ -359,6 +371,28 @@ static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene
                 hs[q] = e > -1.0f ? 1.0f : fabsf(e);
                 grp[q] = hs[q] > 8.0 ? 2 : 1;
             }
+        }
+    }
+    /* start_at_previous: after the first sample inverted, each sample without an empirical
+     * depth starts once, from the main pass's last modelled depth (row-major) */
+    if (!rc && !empirical_depth_present) {
+        float h_prev = 0.0f;
+        for (long p = (long)nrows * ncols - 1; p >= 0; p--) {
+            const float v = -depth[p / ncols][p % ncols];
+            if (v > 0.0f) {
+                h_prev = v;
+                break;
+            }
+        }
+        long first = -1;
+        for (long q = 0; q < n_s; q++) {
+            if (pix[q] < 0) continue;
+            if (first < 0) {
+                first = q;
+                continue;
+            }
+            hs[q] = h_prev;
+            grp[q] = 1;
         }
     }
     const double t_sampled = sm_ms();
@@ -489,7 +523,15 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     photic_cfg_default(&cfg, 4, wl); /* the reference's Nelder-Mead settings */
     cfg.mode = PHOTIC_MODE_NATIVE;
     cfg.n_bottoms = n_bottoms;
-    cfg.nodata = -1.0f; /* screened below: any no-data band arrives negative */
+    /* the pixels are screened below on their raw values (model/samodel.c:933-947); the
+     * kernels invert the (smoothed) values they are given without a second screen */
+    cfg.prescreened = 1;
+    /* a kernel wavefront that finishes no pixel within the watchdog's limit (600 s, or
+     * PHOTIC_WATCHDOG_S seconds from the environment) gives up; the call then fails and no
+     * output of the failed inversion reaches the caller's grids */
+    cfg.check_watchdog = 1;
+    const char *wd = getenv("PHOTIC_WATCHDOG_S");
+    if (wd && atof(wd) > 0.0) cfg.watchdog_s = atof(wd);
     /* each scene's subsurface sun and view angles, as secants (model/samodel.c:534-548) */
     double sec_sun[8], sec_view[8];
     for (int k = 0; k < nd; k++) {

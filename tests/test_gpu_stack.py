@@ -207,3 +207,60 @@ def test_native_stack_against_oracle(nbot, own_h):
     for k in ("depth", "rrs_error", "k_min", "iod", "albedo", "bottom_type"):
         assert np.allclose(g[k][sel], o[k][sel], rtol=1e-6, atol=1e-7), k
     assert np.allclose(g["b_pct"][:, sel], o["b_pct"][:, sel], rtol=1e-6, atol=1e-6)
+
+
+def _vs_oracle(g, o, label):
+    """Status identical; every stack converged on both sides within the bar (H 1 cm, the other
+    parameters 1e-4 relative) or with the same misfit where it is flat; the fraction
+    bit-identical returned."""
+    assert np.array_equal(g["status"], o["status"])
+    v = o["status"] >= 0
+    both = (o["status"] == 0) & (g["status"] == 0)
+    dH = np.abs(g["x"][0] - o["x"][0])
+    rel = np.abs(g["x"][1:] - o["x"][1:]) / np.maximum(np.abs(o["x"][1:]), 1e-300)
+    within = (dH < 0.01) & np.all((rel < 1e-4) | (o["x"][1:] == 0), axis=0)
+    flat = np.abs(g["rrs_error"] - o["rrs_error"]) <= 1e-6 * np.abs(o["rrs_error"])
+    exact = np.all(g["x"] == o["x"], axis=0)
+    print("%s: %d valid, %d converged, within %.4f, bit-identical %.4f" % (
+        label, v.sum(), both.sum(), within[both].mean(), exact[v].mean()))
+    assert np.all((within | flat)[both])
+    return exact[v].mean()
+
+
+def test_stack_eight_dates_against_oracle():
+    """The 8-date stack workload (2048 x 2048 scene seed 3, Bernoulli(0.3) clouds, 3-8 valid
+    dates, so 2 + 3k = 11..26 parameters) on a sample of its stacks, kernel against the FP64
+    oracle directly: every date count from 3 to 8 present."""
+    rrs, ss, valid, truth = O.gen_stack(3, 2048, n_dates=8, p_cloud=0.3, k_min=3)
+    k = valid.sum(0)
+    assert set(range(3, 9)) <= set(np.unique(k).tolist())
+    g = P.invert_stack(_cfg(), rrs, ss, valid)
+    o = O.invert_stack(rrs, ss, valid)
+    exact = _vs_oracle(g, o, "8 dates, k %s" % np.bincount(k)[3:].tolist())
+    assert exact >= 0.98
+    for kk in (7, 8):   # the largest problems, N = 23 and 26
+        sel = (k == kk) & (o["status"] >= 0)
+        assert sel.sum() > 0 and np.all(g["status"][sel] == o["status"][sel])
+
+
+def test_stack_mlp_seeded_stream_against_oracle():
+    """The streamed MLP-seeded 4-date workload (scene seed 5, 2-4 valid dates, 2048-row tiles
+    in the bench, here 8-row tiles over a 24 x 32 scene) against the FP64 oracle given the
+    network's stack seeds: the stream's seeds are the whole scene's, so every stack must be
+    what the oracle's seeded stack inversion gives for it."""
+    wl = (443.0, 482.0, 561.0, 655.0)
+    cfg = P.cfg_default(wl, reqmin=1e-10, konvge=100, kcount=2000)
+    nd, n_rows, n_cols = 4, 24, 32
+    n = n_rows * n_cols
+    rrs, ss, valid, truth = P.gen_stack(cfg, 5, n, n_dates=nd, k_min=2)
+    dx, dt = P.trainset_device(cfg, 101, 1 << 18)
+    net = P.Mlp(init_seed=1)
+    net.train_device(dx.value, dt.value, 1 << 18, batch=8192, epochs=1, lr=2e-3, lr_final=1e-4)
+    seed = net.seed_stack(cfg, rrs, ss, valid)
+    st = P.invert_stack_stream(cfg, n_rows, n_cols, np.ascontiguousarray(rrs),
+                               np.ascontiguousarray(ss), np.ascontiguousarray(valid),
+                               tile_rows=8, net=net)
+    o = O.invert_stack(rrs, ss, valid, seed=seed)
+    exact = _vs_oracle(st, o, "seeded stream, 4 dates")
+    assert exact >= 0.98
+    assert np.all(st["n_starts"][o["status"] >= 0] >= 1)

@@ -265,3 +265,43 @@ def test_region_depth_weights_the_origin_by_sqrt_regions(R, origin):
     assert np.allclose(others, others[0], rtol=1e-9)
     assert np.isclose(sens[origin] / others[0], np.sqrt(R), rtol=1e-9)
     assert np.isclose(sens.sum(), delta, rtol=1e-9)
+
+
+# ---- the ordered inversion's spectral rules (model/samodel.c:701-770, 2795-2842) -------------
+
+def test_spectral_diff_mean_properties():
+    """model_spectral_diff_mean over bands 1..3: zero for identical spectra, scale-free, blind
+    to band 0, and 100 |d| for a uniform relative offset d."""
+    m = np.array([0.011, 0.013, 0.009, 0.004])
+    assert O.spectral_diff_mean(m, m) == 0.0
+    l = m * np.array([0.7, 1.02, 0.97, 1.05])
+    d = O.spectral_diff_mean(m, l)
+    assert d > 0.0
+    assert O.spectral_diff_mean(2 * m, 2 * l) == d
+    l0 = l.copy()
+    l0[0] = 5.0
+    assert O.spectral_diff_mean(m, l0) == d
+    for off in (0.01, -0.03, 0.2):
+        assert np.isclose(O.spectral_diff_mean(m, m * (1 + off)), 100 * abs(off), rtol=1e-6)
+    # sqrt of the mean over three bands: one band off by d gives 100 d / sqrt(3)
+    l1 = m.copy()
+    l1[2] *= 1.06
+    assert np.isclose(O.spectral_diff_mean(m, l1), 100 * 0.06 / np.sqrt(3), rtol=1e-6)
+
+
+def test_order_key_is_the_norm_times_the_angle():
+    """The spectral index: the norm of the Rrs at bands 1, 2, 3 (band 0 ignored); with a noise
+    spectrum, times the angle between the two, so a pixel whose spectrum is parallel to the
+    noise keys to zero and an orthogonal one to norm pi / 2."""
+    r = np.array([0.5, 0.003, 0.004, 0.012], np.float32)
+    k = O.order_key(r)
+    assert np.isclose(k, np.linalg.norm(r[1:].astype(np.float64)), rtol=1e-7)
+    r2 = r.copy()
+    r2[0] = 9.0
+    assert O.order_key(r2) == k
+    noise = np.array([1e-4, 3e-4, 4e-4, 12e-4])
+    assert O.order_key(r, noise) < 1e-3 * k          # parallel: angle ~ 0
+    ko = O.order_key(np.array([0.0, 0.004, 0.003, 0.0], np.float32), noise)
+    v = np.array([0.004, 0.003, 0.0])
+    ang = np.arccos(v @ noise[1:] / (np.linalg.norm(v) * np.linalg.norm(noise[1:])))
+    assert np.isclose(ko, np.float32(np.linalg.norm(v)) * ang, rtol=1e-6)
