@@ -949,13 +949,14 @@ def bench_ordered(args, P):
             "data": "synthetic (Philox-seeded scene, no real imagery)",
             "config": {"workload": "ordered inversion of the %dx%d single-date scene (seed 4): "
                                    "spectral-index order, 64-pixel runs, 16-leader window, "
-                                   "the reference's match, entry and restart levels" %
+                                   "the reference's match and entry levels, no restart" %
                                    (args.side, args.side), "pixels": n}}
     for _ in range(args.warmup):
         ordered(P.order_cfg())
     ref = ordered(P.order_cfg())
     line["ordered_lut"] = ref
     line["ordered_lut_noise_angle"] = ordered(P.order_cfg(r_sigma=[2e-4] * 4))
+    line["ordered_lut_restart_5"] = ordered(P.order_cfg(restart_misfit=5.0))
     line["ordered_hot_start_only"] = ordered(P.order_cfg(use_lut=0))
     line["value"] = ref["pixels_per_s"]
     line["ms_per_step"] = ref["ms"]

@@ -118,8 +118,9 @@ typedef struct photic_outputs {
 } photic_outputs;
 
 /* Invert n_pix single-date pixels.  rrs: [n_bands][n_pix]; sec_sun, sec_view: [n_pix] or
- * NULL; seed: [n_params][n_pix] start points tried before the depth starts, or NULL.
- * Asynchronous on `stream` (NULL = default stream). */
+ * NULL; seed: [n_params][n_pix] start points tried before the depth starts, or NULL (a pixel
+ * whose seed depth is NaN has no seed start).  Asynchronous on `stream` (NULL = default
+ * stream). */
 int photic_invert(const photic_cfg *cfg, long n_pix, const float *rrs, const float *sec_sun,
                   const float *sec_view, const float *seed, const photic_outputs *out,
                   void *stream);
@@ -258,9 +259,17 @@ int photic_mlp_train(photic_mlp *mlp, const float *x, const float *t, long n, lo
 /* Raw network outputs y [n][5] for inputs x [n][5]. */
 int photic_mlp_forward(photic_mlp *mlp, const float *x, long n, float *y, void *stream);
 /* Simplex seeds [5][n] in the MIX layout (H, B, 100 P, 100 G, 100 X) for a single-date scene
- * (rrs [4][n], sec_sun [n] or NULL for cfg->sec_sun). */
+ * (rrs [4][n], sec_sun [n] or NULL for cfg->sec_sun).  A pixel whose predicted depth is below
+ * the network's seed-skip depth (photic_mlp_set_seed_skip) gets a NaN depth: no seed start. */
 int photic_mlp_seed(photic_mlp *mlp, const photic_cfg *cfg, long n, const float *rrs,
                     const float *sec_sun, float *seed, void *stream);
+/* The depth below which photic_mlp_seed gives no seed (0, the default: every pixel is
+ * seeded).  Shallow clear water cannot meet the early exit under the reference's K penalty
+ * (model/samodel.c:2696-2732), so its pixels run every depth start whatever their seed; a
+ * seed start there is work the inversion throws away.  The level is this build's, learned
+ * from labelled forward-model samples (bench.py fits it on a scene other than the one it
+ * times). */
+int photic_mlp_set_seed_skip(photic_mlp *mlp, double h_max);
 /* Stack seeds [2 + 3 D][n] (H, B, then 100 P, 100 G, 100 X at each date's slot) for a D-date
  * scene (rrs [D][4][n], sec_sun [D][n], valid [D][n]): the network runs on every date; H and B
  * are the means of its per-date values over the pixel's valid dates, P, G, X the date's own.
@@ -304,8 +313,8 @@ int photic_invert_stack_stream(const photic_cfg *cfg, photic_mlp *mlp, long n_ro
  * entries (objective below entry_max, depth below 32 m, :1034) and, at a difference of at most
  * match_max, takes the best one's parameter vector as its solution (its outputs are that
  * vector's on its own data: its own misfit, where the reference copies the entry's); every
- * other follower is inverted with one start from its own leader's depth, and again with the
- * full depth list if its objective stays above restart_misfit.  Leaders, hot starts and
+ * other follower is inverted with one start from its own leader's depth, and, with a restart
+ * level set, again with the full depth list if its objective stays above it.  Leaders, hot starts and
  * restarts are each one batch.  Single-date pixels (rrs [4][n], sec_sun [n] or NULL, device
  * arrays), FP64, either parameter layout, unseeded.  role [n] (device) or NULL: -1 not
  * modelled, 0 leader, 1 LUT match, 2 hot start, 3 hot start inverted again. */
@@ -315,7 +324,9 @@ typedef struct photic_order_cfg {
     int use_lut;           /* 0: no look-up table, every follower hot-starts (1) */
     double match_max;      /* largest spectral difference of a match, % (1.5) */
     double entry_max;      /* a leader is a LUT entry below this objective (1.5) */
-    double restart_misfit; /* a hot start above this objective is inverted again (5) */
+    double restart_misfit; /* > 0: a hot start above this objective is inverted again with
+                              the full depth list (the reference's RESTART, 5, :895); 0 (the
+                              default): no restart, as the reference compiles it (:98) */
     double r_sigma[4];     /* noise per band; all >= 1e-6: the spectral angle enters (0) */
 } photic_order_cfg;
 
@@ -323,7 +334,8 @@ typedef struct photic_order_stats {
     long n_valid, n_leaders, n_lut, n_hot, n_restarted;
 } photic_order_stats;
 
-/* The reference's values (model/samodel.c:895-897), 64-pixel runs, a 16-leader window. */
+/* The reference's values (model/samodel.c:895-897; no restart), 64-pixel runs, a 16-leader
+ * window. */
 void photic_order_cfg_default(photic_order_cfg *ocfg);
 int photic_invert_ordered(const photic_cfg *cfg, const photic_order_cfg *ocfg, long n_pix,
                           const float *rrs, const float *sec_sun, const photic_outputs *out,

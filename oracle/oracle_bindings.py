@@ -639,7 +639,7 @@ def depth_sigma(depth, grid, n_rows, n_cols, nodata, rsig, sec_sun, sec_view, n_
 
 def invert_ordered(rrs, sec_sun=None, mode=1, n_bottoms=2, bottom_idx=(0, 1, 2), reqmin=1e-10,
                    konvge=100, kcount=2000, nodata=-9999.0, run=64, window=16, use_lut=1,
-                   match_max=1.5, entry_max=1.5, restart=5.0, noise=None, sec_sun0=1.0, wl=WL4):
+                   match_max=1.5, entry_max=1.5, restart=0.0, noise=None, sec_sun0=1.0, wl=WL4):
     """The oracle's ordered inversion (ordering, look-up table, hot start) of single-date
     pixels rrs [4][n]; noise [4] R_sigma or None."""
     rrs = np.ascontiguousarray(rrs, np.float32)

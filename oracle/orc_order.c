@@ -174,7 +174,7 @@ long orc_invert_ordered(int mode, int n_bottoms, const int *bottom_idx, const or
         orc_optimise(&px, &hc, NULL, &o);
         orc_order_take(&px, &o, &res[p]);
         res[p].role = 2;
-        if ((double)res[p].model_error > restart) {
+        if (restart > 0.0 && (double)res[p].model_error > restart) {
             const int ev = res[p].n_evals, st = res[p].n_starts;
             orc_pixel px2;
             orc_pixel_init(&px2, mode, n_bottoms, bottom_idx, 1, bands, &ss, &sv, r);

@@ -673,7 +673,7 @@ static void orc_run_starts(orc_pixel *px, const orc_nm_cfg *cfg, orc_nelmin_fn n
     memset(best, 0, sizeof(best));
     int accepted = 0;
 
-    if (seed_start) {
+    if (seed_start && seed_start[0] == seed_start[0]) { /* a NaN seed: no seed start */
         double err;
         int icount = 0, numres = 0, ifault = 0;
         memcpy(start, seed_start, sizeof(double) * (size_t)n);

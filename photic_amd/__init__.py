@@ -141,6 +141,7 @@ def _load():
         "photic_invert_stack_stream": (i, [C.POINTER(Cfg), vp, l, l, i, vp, vp, vp, l,
                                            C.POINTER(Outputs), PROGRESS_FN, vp]),
         "photic_order_cfg_default": (None, [C.POINTER(OrderCfg)]),
+        "photic_mlp_set_seed_skip": (i, [vp, d]),
         "photic_invert_ordered": (i, [C.POINTER(Cfg), C.POINTER(OrderCfg), l, vp, vp,
                                       C.POINTER(Outputs), vp, C.POINTER(OrderStats), vp]),
         "photic_invert_h": (i, [C.POINTER(Cfg), l, vp, vp, vp, vp, vp, C.POINTER(Outputs),
@@ -629,6 +630,10 @@ class Mlp:
         if self.h is not None and self.h.value:
             lib.photic_mlp_destroy(self.h)
         self.h = None
+
+    def set_seed_skip(self, h_max):
+        """Seeds whose predicted depth is below h_max are NaN: no seed start there."""
+        check(lib.photic_mlp_set_seed_skip(self.h, h_max))
 
     def __del__(self):
         try:

@@ -297,19 +297,29 @@ __device__ PH_INLINE double ph_band_fast(double P, double lnP, double G, double 
 // objective: model/samodel.c:2432-2759 with one region
 // ---------------------------------------------------------------------------------------
 
+// A kernel's only argument T, read through the kernel-argument segment pointer behind an
+// empty asm the compiler cannot see through: loads through it stay where they are used
+// (scalar loads from the segment) instead of being hoisted out of the persistent loop into
+// scalar registers that then spill into vector lanes.  (The host pass of a HIP build parses
+// kernel bodies without running them; there it is never called.)
+template <typename T>
+__device__ __attribute__((always_inline)) inline const T& ph_kernarg()
+{
 #if defined(__HIP_DEVICE_COMPILE__)
-// The launch constants of a kernel whose first argument starts with its PhConst, read through
-// the kernel-argument segment pointer behind an empty asm the compiler cannot see through:
-// loads through it stay where they are used (scalar loads from the segment) instead of being
-// hoisted out of the persistent loop into scalar registers that then spill into vector lanes.
+    typedef const __attribute__((address_space(4))) T* P4;
+    P4 p = (P4)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const T*)p;
+#else
+    __builtin_unreachable();
+#endif
+}
+
+// The launch constants of a kernel whose first argument starts with its PhConst.
 __device__ __attribute__((always_inline)) inline const PhConst& ph_kernarg_const()
 {
-    typedef const __attribute__((address_space(4))) PhConst* PhConstP4;
-    PhConstP4 p = (PhConstP4)__builtin_amdgcn_kernarg_segment_ptr();
-    asm volatile("" : "+s"(p));
-    return *(const PhConst*)p;
+    return ph_kernarg<PhConst>();
 }
-#endif
 
 template <int NBOT>
 struct PhLayout {

@@ -9,6 +9,7 @@
 // Launchers return hipError_t as int; photic_capi.c turns them into the public API.
 #include <hip/hip_runtime.h>
 
+
 #include "ph_engine.h"
 #include "ph_gen.h"
 #include "ph_launch.h"
@@ -28,13 +29,14 @@ static_assert(offsetof(PhInvertArgs, c) == 0, "PhConst must start the kernel arg
 template <int NBOT>
 constexpr int ph_invert_waves() { return NBOT == 0 ? 2 : 1; }
 
-// SEED: every pixel's first start is its seed point (a.seed), before the depth starts.
+// SEED: every pixel's first start is its seed point (a.seed), before the depth starts; a pixel
+// whose seed depth is NaN has no seed start.
 template <typename FT, int NB, int NBOT, bool SEED>
-__global__ void __launch_bounds__(256, ph_invert_waves<NBOT>()) ph_invert_kernel(PhInvertArgs a)
+__global__ void __launch_bounds__(256, ph_invert_waves<NBOT>()) ph_invert_kernel(PhInvertArgs a_in)
 {
+    (void)a_in; // read through ph_kernarg<PhInvertArgs>() in the loop
     using L = PhLayout<NBOT>;
     constexpr int N = L::template n<1>();
-    const PhConst& c = a.c;
     const int lane = (int)__lane_id();
 
     extern __shared__ double ph_lds[];
@@ -55,6 +57,11 @@ __global__ void __launch_bounds__(256, ph_invert_waves<NBOT>()) ph_invert_kernel
     unsigned long long t0 = wall_clock64();
 
     for (;;) {
+        // The launch arguments, read afresh through the kernel-argument segment pointer on
+        // every pass: the compiler cannot hoist them out of the loop into scalar registers,
+        // which it would then spill into vector lanes (a VALU instruction per reload).
+        const PhInvertArgs& a = ph_kernarg<PhInvertArgs>();
+        const PhConst& c = a.c;
         if (wall_clock64() - t0 > a.max_ticks) {
             if (lane == 0) atomicOr(a.queue + 1, 1ull);
             break;
@@ -118,6 +125,7 @@ __global__ void __launch_bounds__(256, ph_invert_waves<NBOT>()) ph_invert_kernel
                 if (S.kstart < 0) {
 #pragma unroll
                     for (int i = 0; i < N; i++) sd[i] = (double)a.seed[(long)i * a.n_pix + pix];
+                    if (sd[0] != sd[0]) S.kstart = 0; // a NaN seed: no seed start
                 }
                 ph_start_step<FT, 1, NB, NBOT>(S, c, px, S.kstart, sd, hov);
             } else {

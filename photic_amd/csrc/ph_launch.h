@@ -207,7 +207,7 @@ int ph_launch_mlp_trainset(const PhConst *c, uint64_t seed, long pix0, long n, d
                            float *x, float *t, void *stream);
 int ph_launch_mlp_prep(const float *rrs, const float *ss, float ss0, long n, float *x,
                        void *stream);
-int ph_launch_mlp_seed(const float *y, long n, float *seed, void *stream);
+int ph_launch_mlp_seed(const float *y, long n, float skip_h, float *seed, void *stream);
 int ph_launch_mlp_stack_seed(const float *y, long n, int d, const unsigned char *valid,
                              float *seed, void *stream);
 int ph_launch_mlp_stack_mean(long n, int n_dates, const unsigned char *valid, float *seed,

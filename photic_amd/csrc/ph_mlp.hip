@@ -451,13 +451,16 @@ __device__ __forceinline__ void ph_mlp_decode(const float* o, float* s)
     s[4] = 100.0f * X;
 }
 
-// Network outputs -> simplex seeds in the MIX layout [H, B, 100 P, 100 G, 100 X] ([5][n]).
-__global__ void ph_mlp_seed_kernel(const float* __restrict__ y, long n, float* __restrict__ seed)
+// Network outputs -> simplex seeds in the MIX layout [H, B, 100 P, 100 G, 100 X] ([5][n]); a
+// predicted depth below skip_h gives a NaN depth: no seed start for that pixel.
+__global__ void ph_mlp_seed_kernel(const float* __restrict__ y, long n, float skip_h,
+                                   float* __restrict__ seed)
 {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s[5];
     ph_mlp_decode(y + i * 5, s);
+    if (s[0] < skip_h) s[0] = __builtin_nanf("");
     for (int k = 0; k < 5; k++) seed[k * n + i] = s[k];
 }
 
@@ -587,11 +590,12 @@ extern "C" int ph_launch_mlp_prep(const float* rrs, const float* ss, float ss0, 
     return (int)hipGetLastError();
 }
 
-extern "C" int ph_launch_mlp_seed(const float* y, long n, float* seed, void* stream)
+extern "C" int ph_launch_mlp_seed(const float* y, long n, float skip_h, float* seed,
+                                  void* stream)
 {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(ph_mlp_seed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       (hipStream_t)stream, y, n, seed);
+                       (hipStream_t)stream, y, n, skip_h, seed);
     return (int)hipGetLastError();
 }
 

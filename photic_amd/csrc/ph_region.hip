@@ -28,6 +28,10 @@
 #include "ph_grpengine.h"
 #include "ph_launch.h"
 
+// The region kernel reads its arguments through the kernel-argument segment pointer
+// (ph_kernarg): they must be its only argument.
+static_assert(offsetof(PhRegionArgs, c) == 0, "PhConst must start the kernel arguments");
+
 namespace {
 
 PH_DEV int ph_uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -860,15 +864,19 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
         PhRegSide side;
         bool timed_out = false;
         for (;;) {
+            // the launch arguments read afresh on every pass, as in the other inversion
+            // kernels: none of them is held in scalar registers across the loop
+            const PhRegionArgs& ka = ph_kernarg<PhRegionArgs>();
+            const PhConst& kc = ka.c;
             if (S.phase == PH_START) {
-                ph_reg_start_step<MC>(S, c, S.kstart);
-                ph_reg_nm_begin<MC>(S, c);
+                ph_reg_start_step<MC>(S, kc, S.kstart);
+                ph_reg_nm_begin<MC>(S, kc);
             }
             const bool fin = S.phase == PH_FINAL;
-            const double f = ph_reg_objective<NBOT, MC>(S, a, c, fin, &side);
-            ph_reg_consume<MC>(S, c, f);
+            const double f = ph_reg_objective<NBOT, MC>(S, ka, kc, fin, &side);
+            ph_reg_consume<MC>(S, kc, f);
             if (S.phase == PH_DONE) break;
-            if ((S.evals & 255) == 0 && wall_clock64() - t0 > a.max_ticks) {
+            if ((S.evals & 255) == 0 && wall_clock64() - t0 > ka.max_ticks) {
                 timed_out = true;
                 break;
             }

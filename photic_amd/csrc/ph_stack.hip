@@ -19,6 +19,10 @@
 #include "ph_launch.h"
 #include "ph_stackkern.h"
 
+// The stack kernel reads its arguments through the kernel-argument segment pointer
+// (ph_kernarg): they must be its only argument.
+static_assert(offsetof(PhStackArgs, c) == 0, "PhConst must start the kernel arguments");
+
 // Buckets: lists[k * n_pix + i] is the i-th pixel with k valid dates, counts[k] their number.
 __global__ void ph_stack_bucket_kernel(PhStackArgs a, unsigned int* counts, int* lists)
 {

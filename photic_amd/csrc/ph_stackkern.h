@@ -206,12 +206,11 @@ __device__ __attribute__((always_inline)) inline bool ph_stack_load(
 // scratch cost less than the extra wave gains (measured: +5% at 8 dates, +19% at 4; four waves
 // spill so much that they lose 40%).
 template <int ND, int NBOT, bool SEED>
-__global__ void __launch_bounds__(256, 3) ph_stack_kernel(PhStackArgs a)
+__global__ void __launch_bounds__(256, 3) ph_stack_kernel(PhStackArgs a_in)
 {
     using L = PhGrpLayout<ND, NBOT>;
     constexpr int N = L::N, G = L::G, NBT = L::NBT;
     extern __shared__ double ph_stack_lds[];
-    const PhConst& c = a.c;
     const int lane = (int)__lane_id();
     PhGrp<ND, NBOT> S;
     S.li = lane & (G - 1);
@@ -223,7 +222,7 @@ __global__ void __launch_bounds__(256, 3) ph_stack_kernel(PhStackArgs a)
     int dst = 0;
     unsigned int vmask = 0;
 
-    const long m = (long)*a.count;
+    const long m = (long)*a_in.count;
     long q_next = 0, q_end = 0;
     bool exhausted = false;
     // watchdog: the wall clock at the wave's last progress (a chunk taken, a stack retired); a
@@ -231,6 +230,11 @@ __global__ void __launch_bounds__(256, 3) ph_stack_kernel(PhStackArgs a)
     unsigned long long t0 = wall_clock64();
 
     for (;;) {
+        // The launch arguments, read afresh through the kernel-argument segment pointer on
+        // every pass: the compiler cannot hoist them out of the loop into scalar registers,
+        // which it would then spill into vector lanes (a VALU instruction per reload).
+        const PhStackArgs& a = ph_kernarg<PhStackArgs>();
+        const PhConst& c = a.c;
         if (wall_clock64() - t0 > a.max_ticks) {
             if (lane == 0) atomicOr(a.queue + 1, 1ull);
             break;

@@ -1002,7 +1002,15 @@ struct photic_mlp {
     float *pb;            /* packed FP32 biases */
     float *loss;
     int step;
+    float skip_h; /* seeds with a predicted depth below this get no seed start (0: none) */
 };
+
+int photic_mlp_set_seed_skip(photic_mlp *m, double h_max)
+{
+    if (!m || !(h_max >= 0.0)) return ph_fail(PHOTIC_EINVAL, "mlp and h_max >= 0 required");
+    m->skip_h = (float)h_max;
+    return PHOTIC_OK;
+}
 
 int photic_mlp_n_params(void) { return ph_mlp_nparam(); }
 
@@ -1182,7 +1190,7 @@ int photic_mlp_seed(photic_mlp *m, const photic_cfg *cfg, long n, const float *r
     if (!rc) rc = ph_launch_status(ph_launch_mlp_prep(rrs, sec_sun, (float)cfg->sec_sun, n, xb, st),
                                    "network inputs");
     if (!rc) rc = photic_mlp_forward(m, xb, n, yb, st);
-    if (!rc) rc = ph_launch_status(ph_launch_mlp_seed(yb, n, seed, st), "seeds");
+    if (!rc) rc = ph_launch_status(ph_launch_mlp_seed(yb, n, m->skip_h, seed, st), "seeds");
     hipFreeAsync(xb, st);
     if (yb) hipFreeAsync(yb, st);
     return rc;

@@ -7,8 +7,9 @@
  *   3. every follower's role: a LUT match with one of the `window + 1` leaders up to its own,
  *      or a hot start from its own leader's depth;
  *   4. the LUT matches take their leader's parameter vector (one evaluation each);
- *   5. the hot starts inverted with one start each, then those still above the restart level
- *      inverted again with the full depth list (model/samodel.c:1013-1031);
+ *   5. the hot starts inverted with one start each, then, when a restart level is set, those
+ *      still above it inverted again with the full depth list (model/samodel.c:1013-1031,
+ *      compiled in the reference only with RESTART, which it leaves at 0, :98);
  *   6. everything scattered back to pixel order.
  *
  * Each batch is one photic_invert_h launch, so every pixel's result is the single-date
@@ -30,7 +31,7 @@ void photic_order_cfg_default(photic_order_cfg *o)
     o->use_lut = 1;
     o->match_max = 1.5;      /* lut_match_threshold for one scene, model/samodel.c:896 */
     o->entry_max = 1.5;      /* lut_entry_threshold, :897 */
-    o->restart_misfit = 5.0; /* max_error_before_restart, :895 */
+    o->restart_misfit = 0.0; /* no restart: the reference compiles RESTART out (:98) */
 }
 
 static int od_hip(hipError_t e, const char *what)
@@ -122,7 +123,7 @@ int photic_invert_ordered(const photic_cfg *cfg, const photic_order_cfg *ocfg, l
     hipStream_t st = (hipStream_t)stream;
     const double match_max = ocfg->match_max > 0.0 ? ocfg->match_max : 1.5;
     const double entry_max = ocfg->entry_max > 0.0 ? ocfg->entry_max : match_max;
-    const double restart = ocfg->restart_misfit > 0.0 ? ocfg->restart_misfit : 5.0;
+    const double restart = ocfg->restart_misfit; /* <= 0: no restart */
     int noise = 1;
     for (int b = 0; b < 4; b++)
         if (!(ocfg->r_sigma[b] >= 1.0e-6)) noise = 0; /* model/samodel.c:590-598 */
@@ -241,8 +242,10 @@ int photic_invert_ordered(const photic_cfg *cfg, const photic_order_cfg *ocfg, l
             rc = od_launch(ph_launch_order_scatter(out, &H.o, m2, list, 0, order, n_pix,
                                                    n_params, 0, st),
                            "hot-start scatter");
+    }
+    if (!rc && m2 > 0 && restart > 0.0) {
         /* the restart list, as sorted positions */
-        if (!rc) rc = od_hip(hipMalloc((void **)&flag, sizeof(int) * m2), "hipMalloc");
+        rc = od_hip(hipMalloc((void **)&flag, sizeof(int) * m2), "hipMalloc");
         if (!rc) rc = od_hip(hipMalloc((void **)&list2, sizeof(int) * m2), "hipMalloc");
         if (!rc)
             rc = od_launch(ph_launch_order_restart(m2, H.o.model_error, restart, flag, st),

@@ -47,8 +47,10 @@ static int run(const PhConst& c, long n_pix, const float* rrs, const float* sec_
         for (;;) {
             if (S.phase == PH_START) {
                 double sd[N];
-                if (S.kstart < 0)
+                if (S.kstart < 0) {
                     for (int i = 0; i < N; i++) sd[i] = (double)seed[(long)i * n_pix + p];
+                    if (sd[0] != sd[0]) S.kstart = 0; // a NaN seed: no seed start
+                }
                 ph_start_step<double, 1, NB, NBOT>(S, c, px, S.kstart, sd);
                 ph_nm_begin(S, c);
             }

@@ -127,3 +127,23 @@ def test_engine_cr_math_against_oracle(mode, nbot):
     assert np.all((within | same_fit)[both])
     assert within[both].mean() > 0.99
     assert np.all(h["x"][:, valid] == o["x"][:, valid], axis=0).mean() > 0.99
+
+
+def test_engine_nan_seed_means_no_seed_start():
+    """A pixel whose seed depth is NaN has no seed start: it runs the depth starts alone and
+    ends exactly where the plain inversion ends, in the host build of the engine and in the
+    oracle alike; the other pixels keep their seed start."""
+    rrs, ss, truth, nodata, _ = O.gen(26, 96)
+    seed = np.stack([truth[0] * 1.1, truth[4], 100 * truth[1], 100 * truth[2],
+                     100 * truth[3]]).astype(np.float32)
+    skip = np.arange(seed.shape[1]) % 3 == 0
+    seed[0, skip] = np.nan
+    cfg = P.cfg_default(reqmin=1e-10, konvge=100, kcount=2000)
+    h = O.hostsim_invert(P.const_bytes(cfg, has_seed=True), 0, rrs, ss, seed=seed)
+    o = O.invert(rrs, ss, seed=seed)
+    plain = O.invert(rrs, ss)
+    v = o["status"] >= 0
+    assert np.array_equal(h["x"], o["x"]) and np.array_equal(h["n_starts"], o["n_starts"])
+    assert np.array_equal(o["x"][:, skip & v], plain["x"][:, skip & v])
+    assert np.array_equal(o["n_starts"][skip & v], plain["n_starts"][skip & v])
+    assert np.any(o["n_starts"][~skip & v] != plain["n_starts"][~skip & v])
