@@ -132,6 +132,18 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
                     const float *sec_view, const float *seed, const float *hstart,
                     const photic_outputs *out, void *stream);
 
+/* As photic_invert_h, allocating nothing: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under stream or graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The launch's work counter and watchdog flag live in a
+ * fixed 256-byte scratch array that each device holds for the life of the process (2 of its
+ * 32 words are used, whatever n_pix is); every call on a device shares that array, so calls
+ * that overlap on one device race -- serialise them (one stream, or events).  The array is
+ * cleared by a fill kernel on `stream` before each launch.  Results are those of
+ * photic_invert_h. */
+int photic_invert_noalloc(const photic_cfg *cfg, long n_pix, const float *rrs,
+                          const float *sec_sun, const float *sec_view, const float *seed,
+                          const float *hstart, const photic_outputs *out, void *stream);
+
 /* Multi-date stacks: each pixel seen on n_dates (<= 8) dates, some of them invalid (clouds).
  * A pixel's problem is the reference's N_s = k run over its k valid dates, in date order
  * (model/samodel.c:1808-1815 layout, one region): H and the bottom shared, P, G, X per date,

@@ -9,6 +9,7 @@
 // Launchers return hipError_t as int; photic_capi.c turns them into the public API.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 
 #include "ph_engine.h"
 #include "ph_gen.h"
@@ -248,6 +249,40 @@ __global__ void ph_zero_u64_kernel(unsigned long long* p, int n)
 {
     int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------------------
+// the no-allocation entry point's scratch
+// ---------------------------------------------------------------------------------------
+
+// photic_invert_noalloc's device memory beyond its inputs and outputs: the launch's work
+// counter and watchdog flag (words 0 and 1).  Each device holds its own copy for the life of
+// the process; every call on a device shares it, so calls that overlap on one device race.
+// In host code `scratch` names a host-side stand-in, never the device array: kernels receive
+// the address ph_scratch_words looks up.
+static __device__ __align__(256) unsigned long long scratch[PH_SCRATCH_WORDS];
+
+#define PH_SCRATCH_MAX_DEVICES 64
+static unsigned long long* ph_scratch_addr[PH_SCRATCH_MAX_DEVICES];
+static std::mutex ph_scratch_lock;
+
+// The address of the current device's scratch array, looked up once per device with that
+// device current and kept.
+extern "C" int ph_scratch_words(unsigned long long** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= PH_SCRATCH_MAX_DEVICES) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(ph_scratch_lock);
+    if (!ph_scratch_addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        ph_scratch_addr[dev] = (unsigned long long*)p;
+    }
+    *out = ph_scratch_addr[dev];
+    return 0;
 }
 
 // ---------------------------------------------------------------------------------------

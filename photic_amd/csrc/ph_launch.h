@@ -132,6 +132,11 @@ extern "C" {
 
 int ph_launch_invert(const PhInvertArgs *a, int fp32, int n_bands, int n_bottoms, int grid,
                      int block, void *stream);
+/* The no-allocation entry point's scratch: a file-scope device array of PH_SCRATCH_WORDS
+ * 64-bit words (ph_kernels.hip), one copy per device.  *out: its address on the current
+ * device, looked up once per device with that device current. */
+#define PH_SCRATCH_WORDS 32
+int ph_scratch_words(unsigned long long **out);
 int ph_launch_zero_u64(unsigned long long *p, int n, void *stream);
 int ph_launch_copy16(const void *src, void *dst, long n16, void *stream);
 int ph_launch_fwd(const PhConst *c, long n, const double *params, const double *sec_sun,

@@ -433,6 +433,25 @@ int photic_invert_h(const photic_cfg *cfg, long n_pix, const float *rrs, const f
     return ph_invert_single(cfg, n_pix, rrs, sec_sun, sec_view, seed, hstart, out, stream, NULL);
 }
 
+int photic_invert_noalloc(const photic_cfg *cfg, long n_pix, const float *rrs,
+                          const float *sec_sun, const float *sec_view, const float *seed,
+                          const float *hstart, const photic_outputs *out, void *stream)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    /* the launch needs two words of the device's scratch array: its work counter and its
+     * watchdog flag, whatever n_pix is */
+    const int need = 2;
+    if (need > PH_SCRATCH_WORDS)
+        return ph_fail(PHOTIC_EUNSUPPORTED, "the inversion needs %d scratch words, the device "
+                                            "array holds %d", need, PH_SCRATCH_WORDS);
+    unsigned long long *scratch = NULL;
+    rc = ph_launch_status(ph_scratch_words(&scratch), "scratch symbol address");
+    if (rc) return rc;
+    return ph_invert_single(cfg, n_pix, rrs, sec_sun, sec_view, seed, hstart, out, stream,
+                            scratch);
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* multi-date stacks                                                                      */
 /* ------------------------------------------------------------------------------------ */
