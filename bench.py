@@ -81,6 +81,48 @@ def timed_invert(P, cfg, n, d_rrs, d_ss, d_seed, outs, stream):
     return float(ms.value)
 
 
+SKIP_FIT_SCENE = 14  # Philox key of the scene the seed-skip depth is fitted on (the timed one is 4)
+
+
+def fit_seed_skip(P, net, cfg, stream, fixed):
+    """The depth below which the network's seed is not tried, learned from labelled forward-
+    model pixels of a scene other than the timed one: both forms are run on 2^20 pixels of it,
+    which gives each pixel's evaluations with and without its seed start, and the depth chosen
+    is the one whose skipped pixels (network depth below it) save the most evaluations in
+    total.  Shallow clear water cannot meet the early exit under the K penalty, so it runs
+    every depth start whatever its seed; there the seed start is pure cost.  fixed > 0: that
+    depth, no fit.  Sets the network's skip depth and returns what was fitted."""
+    if fixed > 0.0:
+        net.set_seed_skip(fixed)
+        return {"depth_m": fixed, "fitted": False}
+    n_c = 1 << 20
+    d_r, d_s, d_t = P.DeviceBuffer(16 * n_c), P.DeviceBuffer(4 * n_c), P.DeviceBuffer(24 * n_c)
+    P.check(P.lib.photic_gen_scene(C.byref(cfg), SKIP_FIT_SCENE, 0, n_c, 2e-4, 0.0, d_r.value,
+                                   d_s.value, d_t.value, stream))
+    net.set_seed_skip(0.0)
+    d_seed = P.DeviceBuffer(20 * n_c)
+    net.seed_device(cfg, n_c, d_r.value, d_s.value, d_seed.value, stream=stream)
+    ev = {}
+    for kind in ("seeded", "plain"):
+        outs = P.DeviceOutputs(cfg, n_c, fields=("status", "n_evals"))
+        P.invert_device(cfg, n_c, d_r.value, d_s.value,
+                        d_seed.value if kind == "seeded" else None, outs, stream=stream)
+        ev[kind] = outs.fetch()
+    P.check(P.lib.photic_synchronize())
+    h_pred = d_seed.to_array(np.float32, (5, n_c))[0]
+    v = (ev["seeded"]["status"] >= 0) & (ev["plain"]["status"] >= 0)
+    gain = (ev["seeded"]["n_evals"].astype(np.int64) - ev["plain"]["n_evals"])[v]
+    hv = h_pred[v]
+    grid = np.arange(1.0, 8.01, 0.25)
+    saved = [int(gain[hv < h].sum()) for h in grid]
+    best = int(np.argmax(saved))
+    h = float(grid[best]) if saved[best] > 0 else 0.0
+    net.set_seed_skip(h)
+    return {"depth_m": h, "fitted": True, "fit_scene_seed": SKIP_FIT_SCENE, "fit_pixels": n_c,
+            "evals_saved_per_pixel_on_fit_scene": saved[best] / max(int(v.sum()), 1) if h else 0.0,
+            "skipped_frac_on_fit_scene": float((hv < h).mean()) if h else 0.0}
+
+
 def host_cpus():
     """The host's CPU facts and the thread count the CPU baseline uses: one thread per core
     this process may run on (its affinity mask), capped by OMP_NUM_THREADS where the
@@ -136,6 +178,9 @@ def main():
     ap.add_argument("--equal-misfit-pixels", type=int, default=1 << 22)
     ap.add_argument("--train-samples", type=int, default=1 << 24)
     ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--seed-skip", type=float, default=-1.0,
+                    help="seeded runs: no seed start where the network predicts a depth below "
+                         "this; -1 (default): fitted on a separate labelled scene; 0: none")
     ap.add_argument("--seed-abort", type=float, default=5.0,
                     help="seeded runs: give a seed start up while its objective is above this "
                          "(the early exit stops below 2.5) at its first convergence check "
@@ -217,6 +262,8 @@ def main():
         train_info = {"samples": n_tr, "epochs": args.train_epochs, "final_loss": loss,
                       "seconds": time.perf_counter() - t}
         del dx, dt
+        if args.seed_skip != 0.0:
+            train_info["seed_skip"] = fit_seed_skip(P, net, cfg, stream, args.seed_skip)
     d_seed = P.DeviceBuffer(4 * 5 * n) if net is not None else None
 
     def scene_slice(n_sub):
