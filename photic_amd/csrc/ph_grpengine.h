@@ -27,10 +27,11 @@
 
 #define PH_DEV __device__ __attribute__((always_inline)) inline
 
-// Lanes per stack: the smallest of 8, 16, 32 that holds n coordinates and np pairs.
+// Lanes per stack: the smallest of 8, 16, 32 that holds the n + 1 vertex values (one per
+// lane) and np pairs.
 __host__ __device__ constexpr int ph_grp_lanes(int n, int np)
 {
-    return (n <= 8 && np <= 8) ? 8 : ((n <= 16 && np <= 16) ? 16 : 32);
+    return (n + 1 <= 8 && np <= 8) ? 8 : ((n + 1 <= 16 && np <= 16) ? 16 : 32);
 }
 
 // A stack's LDS block, in doubles.  Sub-arrays start at even offsets (16-byte aligned).
@@ -42,14 +43,14 @@ struct PhGrpLayout {
     static constexpr int NP = 4 * ND;
     static constexpr int G = ph_grp_lanes(N, NP);
     static constexpr int P0 = 0;                        // simplex, [N + 1][N] (lane i: column i)
-    static constexpr int Y0 = (N + 1) * N;              // vertex values, [N + 1]
+    static constexpr int Y0 = (N + 1) * N;              // vertex values at a convergence test, [N + 1]
     static constexpr int D0 = Y0 + ((N + 2) & ~1);      // squared residual per pair, [NP]
     static constexpr int K0 = D0 + NP;                  // attenuation K per pair, [NP]
     static constexpr int Q0 = K0 + NP;                  // rB / r per pair, [NP]
     static constexpr int T0 = Q0 + NP;                  // K-penalty term per date, [ND]
     static constexpr int KL = T0 + ND;                  // the last date's Kmin
     static constexpr int n = (KL + 2) & ~1;
-    static_assert(N <= G && NP <= G, "group too small");
+    static_assert(N + 1 <= G && NP <= G, "group too small");
 };
 
 // Orders the group's LDS accesses around a cross-lane exchange (one wavefront: in-order LDS,
@@ -76,6 +77,8 @@ struct PhGrp {
     using L = PhGrpLayout<ND, NBOT>;
     double* g; // the stack's LDS block
     int li;    // lane index in the group
+    int gb;    // the group's first lane in the wavefront
+    double yv; // vertex value li (li <= N)
 
     // coordinate li (li < N)
     double xs, xb, xe, step, best;
@@ -93,8 +96,38 @@ struct PhGrp {
 
     PH_DEV double p(int k) const { return g[L::P0 + k * L::N + li]; }
     PH_DEV void set_p(int k, double v) const { g[L::P0 + k * L::N + li] = v; }
-    PH_DEV double y(int k) const { return g[L::Y0 + k]; }
-    PH_DEV void set_y(int k, double v) const { g[L::Y0 + k] = v; }
+    // vertex value k (k the same in every lane of the group)
+    PH_DEV double y(int k) const { return __shfl(yv, k, L::G); }
+    PH_DEV void set_y(int k, double v) { yv = ph_pick(li == k, v, yv); }
+    // the group's bits of a wavefront ballot, lane li at bit li
+    PH_DEV unsigned long long bits(bool pred) const
+    {
+        constexpr unsigned long long low = L::G == 64 ? ~0ull : ((1ull << L::G) - 1ull);
+        return (__ballot(pred) >> gb) & low;
+    }
+    // The scan "v = y_0, at = 0; for i = 1..N: if (v < y_i) v = y_i, at = i" (asa047.c:
+    // 223-231) as a reduction over the group: no comparison with a NaN holds, so a NaN y_0
+    // stays; otherwise it ends at the first index of the largest value.  WANT_MAX false: the
+    // same with ">" (the lowest vertex, asa047.c:196-204).  *val = y_at.
+    template <bool WANT_MAX>
+    PH_DEV int scan(double* val) const
+    {
+        const double y0 = y(0);
+        if (y0 != y0) {
+            *val = y0;
+            return 0;
+        }
+        double m = y0;
+        if (li <= L::N && (WANT_MAX ? yv > m : yv < m)) m = yv;
+#pragma unroll
+        for (int off = L::G / 2; off >= 1; off >>= 1) {
+            const double o = __shfl_xor(m, off, L::G);
+            if (WANT_MAX ? o > m : o < m) m = o;
+        }
+        const int at = __ffsll((long long)bits(li <= L::N && yv == m)) - 1;
+        *val = y(at);
+        return at;
+    }
 };
 
 // Side quantities of the final evaluation (PhStackSide of the per-lane engine).
@@ -320,8 +353,7 @@ PH_DEV void ph_grp_consume(PhGrp<ND, NBOT>& S, const PhConst& c, double f)
             mv_coef = 2.0;
             mv_phase = PH_EXPAND;
         } else {
-            int l = 0;
-            ph_for<N + 1>([&](auto i) PH_BODY { l += (f < S.y(i)) ? 1 : 0; });
+            const int l = __popcll(S.bits(li <= N && f < S.yv));
             if (l > 1) {
                 wr = true;
                 wj = S.ihi;
@@ -432,16 +464,7 @@ PH_DEV void ph_grp_consume(PhGrp<ND, NBOT>& S, const PhConst& c, double f)
 
     bool iter_begin = false, nm_finish = false;
     if (act == A_AFTER_BUILD) {
-        S.ylo = S.y(0);
-        S.ilo = 0;
-        ph_for<N>([&](auto i0) PH_BODY {
-            constexpr int i = i0 + 1;
-            const double yi = S.y(i);
-            if (yi < S.ylo) {
-                S.ylo = yi;
-                S.ilo = i;
-            }
-        });
+        S.ilo = S.template scan<false>(&S.ylo);
         iter_begin = true;
     }
     // ---- end of an iteration: asa047.c:397-434 ----
@@ -456,12 +479,17 @@ PH_DEV void ph_grp_consume(PhGrp<ND, NBOT>& S, const PhConst& c, double f)
             iter_begin = true;
         } else if (S.icount <= c.kcount) {
             S.jcount = c.konvge;
+            // the sums in vertex order, every lane alike, from the group's values in LDS
+            using L = PhGrpLayout<ND, NBOT>;
+            ph_wave_sync();
+            if (li <= N) S.g[L::Y0 + li] = S.yv;
+            ph_wave_sync();
             double z = 0.0;
-            ph_for<N + 1>([&](auto i) PH_BODY { z = z + S.y(i); });
+            ph_for<N + 1>([&](auto i) PH_BODY { z = z + S.g[L::Y0 + i]; });
             const double xm = ph_div_const(z, dnn, 1.0 / dnn);
             z = 0.0;
             ph_for<N + 1>([&](auto i) PH_BODY {
-                const double yi = S.y(i);
+                const double yi = S.g[L::Y0 + i];
                 z = z + (yi - xm) * (yi - xm);
             });
             if (z <= c.rq) nm_finish = true;
@@ -475,26 +503,13 @@ PH_DEV void ph_grp_consume(PhGrp<ND, NBOT>& S, const PhConst& c, double f)
         if (c.kcount <= S.icount) {
             nm_finish = true;
         } else {
-            S.ynewlo = S.y(0);
-            S.ihi = 0;
-            ph_for<N>([&](auto i0) PH_BODY {
-                constexpr int i = i0 + 1;
-                const double yi = S.y(i);
-                if (S.ynewlo < yi) {
-                    S.ynewlo = yi;
-                    S.ihi = i;
-                }
-            });
+            S.ihi = S.template scan<true>(&S.ynewlo);
             if (co) {
                 // centroid without the worst vertex: z = sum over vertices k = 0..N in k
                 // order, less the worst vertex's phi
-                const int hi = S.ihi;
-                double z = 0.0, phi = 0.0;
-                ph_for<N + 1>([&](auto k) PH_BODY {
-                    const double v = S.p(k);
-                    z = z + v;
-                    phi = ph_pick(k == hi, v, phi);
-                });
+                const double phi = S.p(S.ihi);
+                double z = 0.0;
+                ph_for<N + 1>([&](auto k) PH_BODY { z = z + S.p(k); });
                 const double zi = z - phi;
                 const double b = ph_div_const(zi, dn, 1.0 / dn);
                 const double r = b + 1.0 * (b - phi);

@@ -214,6 +214,8 @@ __global__ void __launch_bounds__(256, 3) ph_stack_kernel(PhStackArgs a_in)
     const int lane = (int)__lane_id();
     PhGrp<ND, NBOT> S;
     S.li = lane & (G - 1);
+    S.gb = lane & ~(G - 1);
+    S.yv = 0.0;
     S.g = ph_stack_lds + (long)(threadIdx.x / G) * L::n;
     S.phase = PH_IDLE;
     const int gbase = lane & ~(G - 1);
