@@ -187,10 +187,11 @@ int photic_invert_stack_h(const photic_cfg *cfg, long n_pix, int n_dates, const 
  * every region; out->x is [R_max (1 + 2 n_bottoms) + 3 n_scenes][n] with R_max = (2 n_spatial
  * - 1)^2, in the reference's layout [H_r], [B_{r,k}], [q_{r,k}], [(P, G, X)_s] and zero past
  * the pixel's own parameter count.  n_regions [n] (device) or NULL: each pixel's region count
- * (0: not modelled, status -1).  PHOTIC_MODE_NATIVE with 1..3 bottoms, 4 bands, FP64, and
- * R_max (1 + 2 n_bottoms) + 3 n_scenes <= 128 parameters (n_spatial 2: every bottom and scene
- * count; n_spatial 3: one bottom, or two with one scene); the simplex must fit the device's
- * LDS per block. */
+ * (0: not modelled, status -1).  PHOTIC_MODE_NATIVE with 1..3 bottoms, 4 bands, FP64, 1..32
+ * scenes (the reference's MAX_SCENES), n_spatial <= 4 (R_max <= 64) and R_max (1 + 2
+ * n_bottoms) + 3 n_scenes <= 384 parameters.  A pixel's simplex and working arrays live in
+ * LDS when they fit there (at most 128 parameters), otherwise in device memory allocated for
+ * the call (one block per wavefront of the launch); the results are the same. */
 int photic_invert_regions(const photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
                           const float *rrs, const float *nodata, const double *sec_sun,
                           const double *sec_view, int n_spatial, long n, const int64_t *idx,

@@ -16,13 +16,15 @@
  * GPU -- with n_spatial > 1 jointly with its (2 n_spatial - 1)^2 neighbours and their
  * continuity penalties (:2432-2759), with several scenes as one problem (:1808-1815).  When
  * depth_sigma is not NULL the depth-error pass of :1376-1482 fills it, with Philox draws in
- * place of the reference's clock-seeded rand() and every sample inverted independently (the
- * reference chains each sample's start to the previous one).  Not carried over: the look-up
- * table and hot start (order-dependent, sequential) and the netCDF files the reference writes
- * as a side effect.  Supported: 1-8 scenes of one grid with the same 4 bands, 1-3 bottoms,
- * at most 128 parameters per pixel ((2 n_spatial - 1)^2 (1 + 2 n_bottoms) + 3 nscenes);
- * anything else prints a message and leaves the outputs at the reference's defaults.  Use
- * photic_samodel() for the status code.
+ * place of the reference's clock-seeded rand(); as in the reference, the first sample inverted
+ * runs the main pass's start rules and every later one a single start from the depth of the
+ * main pass's last modelled pixel (start_at_previous, :1454, :2233-2241).  Not carried over: the look-up table and hot start
+ * (order-dependent and sequential in the reference; photic_invert_ordered is their batched
+ * form) and the netCDF files the reference writes as a side effect (photic_pack_int16 is
+ * their packing).  Supported: 1-32 scenes of one grid with the same 4 bands, 1-3 bottoms,
+ * n_spatial <= 4 and at most 384 parameters per pixel ((2 n_spatial - 1)^2 (1 + 2 n_bottoms)
+ * + 3 nscenes); anything else prints a message and leaves the outputs at the reference's
+ * defaults.  Use photic_samodel() for the status code.
  */
 #ifndef PHOTIC_SAMODEL_H
 #define PHOTIC_SAMODEL_H

@@ -185,8 +185,8 @@ long orc_invert_scene(const orc_scene_cfg *cfg, long n_pix, const float *rrs,
                       int *n_evals, double *x_out, int x_stride);
 
 /* --- spatial regions (n_spatial > 1), orc_region.c --- */
-#define ORC_MAXR 25    /* regions: (2 n_spatial - 1)^2 for n_spatial <= 3 */
-#define ORC_RMAXD 8    /* scenes */
+#define ORC_MAXR 49    /* regions: (2 n_spatial - 1)^2 for n_spatial <= 4 */
+#define ORC_RMAXD 32   /* scenes (the reference's MAX_SCENES) */
 #define ORC_RMAXBOT 3  /* bottoms */
 #define ORC_RMAXN (ORC_MAXR * (1 + 2 * ORC_RMAXBOT) + 3 * ORC_RMAXD)
 

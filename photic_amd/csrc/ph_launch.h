@@ -13,6 +13,9 @@
 #define PH_NBOT 8  /* bottom types in the table */
 #define PH_MAXH 16 /* depth starts */
 #define PH_MAXD 8  /* dates per pixel (stacks) */
+#define PH_MAXS 32 /* scenes of a region problem (the reference's MAX_SCENES) */
+#define PH_REG_MAXR 64  /* regions of a pixel: (2 n_spatial - 1)^2, n_spatial <= 4 */
+#define PH_REG_MAXN 384 /* parameters of a region problem: 64 per lane slot, 6 slots */
 
 #define PH_LAUNCH_UNSUPPORTED 100001
 
@@ -91,14 +94,15 @@ typedef struct PhStackArgs {
 
 /* One launch of the spatial-region inversion (ph_region.hip): the pixels idx[q] (or q) of an
  * n_rows x n_cols scene, each inverted with its (2 n_spatial - 1)^2 neighbourhood, one
- * wavefront per pixel.  A wave's LDS block (lds_doubles doubles) holds the simplex
+ * wavefront per pixel.  A wave's block of lds_doubles doubles -- in LDS, or for problems whose
+ * block does not fit there at work + wave * lds_doubles in device memory -- holds the simplex
  * [(n_max + 1) n_max] from offset 0, then the arrays at the o_* offsets (in doubles):
  * o_x the trial point [n_max], o_d squared residual per pair,
  * o_q rB / r per pair, o_rm measured Rrs per pair, o_pw (440 / lambda)^Y per pair
  * [np_max each], o_k the last region's K [4 n_scenes], o_m Rrs at 490 / 550 / 640 nm per
  * (region, scene) [3 r_max n_scenes], o_pix the regions' pixel indices [r_max], o_bc the band
  * coefficients [4][8] (a0, a1, a_w, b_bw, exp(-0.015 (lambda - 440)), the bottoms'
- * reflectances), o_geo each scene's sec(sun) then sec(view) [2 PH_MAXD], o_hd each region's
+ * reflectances), o_geo each scene's sec(sun) then sec(view) [2 n_scenes], o_hd each region's
  * depth-penalty term [r_max], o_bqv and o_bd each (bottom, region)'s fraction and penalty term
  * [n_bottoms r_max each]. */
 typedef struct PhRegionArgs {
@@ -107,14 +111,15 @@ typedef struct PhRegionArgs {
     int r_max, n_max, np_max;      /* regions, parameters, (region, scene, band) pairs */
     int lds_doubles;
     int o_x, o_d, o_q, o_rm, o_pw, o_k, o_m, o_pix, o_bc, o_geo, o_hd, o_bqv, o_bd;
-    float nodata[PH_MAXD * 4];     /* no-data value of each (scene, band) grid */
-    double sec_sun[PH_MAXD], sec_view[PH_MAXD];
+    const double *scn;             /* per scene, device: sec(sun) [n_scenes], sec(view)
+                                      [n_scenes], R_sigma [n_scenes][4], the no-data value
+                                      of each (scene, band) grid [n_scenes][4] */
+    double *work;                  /* device blocks [grid][lds_doubles] when they live there */
     long n;                        /* pixels to invert */
     const float *rrs;              /* [n_scenes][4][n_rows * n_cols] */
     const int64_t *idx;            /* [n] pixel indices, or NULL: pixel q is q, screened */
     const float *hstart;           /* [n]: one depth start per pixel, or NULL: c.hstart */
     const float *pert;             /* [n]: n_sigma per pixel (R_sigma perturbation), or NULL */
-    double rsig[PH_MAXD * 4];      /* R_sigma of each (scene, band) */
     unsigned long long *queue;     /* [0] work counter, [1] watchdog flag; zero at launch */
     unsigned long long max_ticks;
     float *depth, *rrs_error, *model_error, *albedo, *k_min, *iod, *b_pct, *bottom_type;

@@ -68,7 +68,7 @@ struct PhReg {
     static constexpr int YC = MC + 1;
     double* g; // the wave's LDS block: the simplex, [vertex][coordinate], stride N
     int li, R, N, NP, nd, origin;
-    int nd_magic; // ceil(2^16 / nd): x / nd = (x nd_magic) >> 16 for x < 1024
+    int nd_magic; // ceil(2^16 / nd): x / nd = (x nd_magic) >> 16 for x < 64 nd, nd <= 32
     double rq;    // reqmin * N
     double xs[MC], xb[MC], xe[MC], step[MC], best[MC], xs0[MC], step0[MC];
     double yv[YC];
@@ -145,7 +145,7 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
     ph_wave_sync();
     const double* X = g + a.o_x;
     for (int j = li; j < NP; j += 64) {
-        const int r = ((j >> 2) * S.nd_magic) >> 16; // (j / 4) / nd, exact for j < 4096
+        const int r = ((j >> 2) * S.nd_magic) >> 16; // (j / 4) / nd: j / 4 < 64 nd
         const int s = (j >> 2) - r * nd;
         const int b = j & 3;
         const double* bc = g + a.o_bc + 8 * b; // a0, a1, aw, bbw, eg, bottoms of band b
@@ -166,7 +166,7 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
         const double P = 0.01 * fabs(X[off + 3 * s]);
         const double Gc = 0.01 * fabs(X[off + 1 + 3 * s]);
         const double Xc = 0.01 * fabs(X[off + 2 + 3 * s]);
-        const double ss = g[a.o_geo + s], sv = g[a.o_geo + PH_MAXD + s];
+        const double ss = g[a.o_geo + s], sv = g[a.o_geo + nd + s];
         double K, rB, rr;
         const double m = ph_band64_nb(P, Gc, Xc, rho, H, bc[0], bc[1], bc[2], bc[3], bc[4],
                                       g[a.o_pw + j], ss, sv, &K, &rB, &rr);
@@ -660,14 +660,19 @@ __device__ void ph_reg_write_nodata(const PhRegionArgs& a, long q, int n_regions
 
 } // namespace
 
-// One wavefront per pixel (block = 64 lanes), dynamic LDS a.lds_doubles doubles per block.
-template <int NBOT, int MC>
+// One wavefront per pixel (block = 64 lanes).  GM false: the wave's block of a.lds_doubles
+// doubles is its dynamic LDS; GM true: it is a.work + blockIdx.x * a.lds_doubles in device
+// memory (problems whose simplex does not fit the LDS), read and written by this wave alone.
+template <int NBOT, int MC, bool GM>
 __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
 {
     extern __shared__ double ph_reg_lds[];
     const PhConst& c = a.c;
     PhReg<MC> S;
-    S.g = ph_reg_lds;
+    if constexpr (GM)
+        S.g = a.work + (long)blockIdx.x * a.lds_doubles;
+    else
+        S.g = ph_reg_lds;
     S.li = (int)__lane_id();
     const int li = S.li;
     const int nd = a.n_scenes;
@@ -692,14 +697,9 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
         }
         g[a.o_bc + li] = v;
     }
-    if (li < 2 * PH_MAXD) {
-        double v = 0.0;
-        for (int t = 0; t < PH_MAXD; t++) {
-            if (li == t) v = a.sec_sun[t];
-            if (li == PH_MAXD + t) v = a.sec_view[t];
-        }
-        g[a.o_geo + li] = v;
-    }
+    for (int t = li; t < 2 * nd; t += 64) g[a.o_geo + t] = a.scn[t];
+    const double* scn_rsig = a.scn + 2 * nd;   // [scene][band]
+    const double* scn_nodata = a.scn + 6 * nd; // [scene][band], float values
 
     for (;;) {
         if (wall_clock64() - t0 > a.max_ticks) {
@@ -724,9 +724,9 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
         // negative (model/samodel.c:933-947)
         if (!a.idx) {
             bool bad = false;
-            if (li < 4 * nd) {
-                const float v = a.rrs[(long)li * npix + p];
-                bad = ph_approx_equal(v, a.nodata[li], 1.0e-6f) || v < 0.0f;
+            for (int t = li; t < 4 * nd; t += 64) {
+                const float v = a.rrs[(long)t * npix + p];
+                bad |= ph_approx_equal(v, (float)scn_nodata[t], 1.0e-6f) || v < 0.0f;
             }
             if (__ballot(bad) != 0ull) {
                 if (li == 0) ph_reg_write_nodata(a, q, 0);
@@ -750,7 +750,8 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
             centre = (iii == pi && jjj == pj);
             ok = true;
             for (int t = 0; t < 4 * nd; t++)
-                if (ph_approx_equal(a.rrs[(long)t * npix + cpix], a.nodata[t], 1.0e-6f)) ok = false;
+                if (ph_approx_equal(a.rrs[(long)t * npix + cpix], (float)scn_nodata[t], 1.0e-6f))
+                    ok = false;
         }
         const unsigned long long okm = __ballot(ok), cm = __ballot(centre);
         const int R = ph_uni(__popcll(okm));
@@ -790,7 +791,7 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
             if (a.pert && !ph_approx_equal(a.pert[q], 0.0f, 1.0e-6f)) {
                 const double ns_ = (double)a.pert[q];
 #pragma unroll
-                for (int bb = 0; bb < 4; bb++) r4[bb] += ns_ * a.rsig[s * 4 + bb];
+                for (int bb = 0; bb < 4; bb++) r4[bb] += ns_ * scn_rsig[s * 4 + bb];
             }
             g[a.o_rm + j] = ph_band_sel(r4, b);
             double r440 = ph_rrs_at(c, r4, 0);
@@ -948,55 +949,76 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
     }
 }
 
-template <int NBOT, int MC>
+template <int NBOT, int MC, bool GM>
 static int ph_region_launch_t(const PhRegionArgs* a, int grid, void* stream)
 {
-    const size_t lds = sizeof(double) * (size_t)a->lds_doubles;
+    const size_t lds = GM ? 0 : sizeof(double) * (size_t)a->lds_doubles;
     if (lds > 65536) {
-        const hipError_t e = hipFuncSetAttribute((const void*)ph_region_kernel<NBOT, MC>,
+        const hipError_t e = hipFuncSetAttribute((const void*)ph_region_kernel<NBOT, MC, GM>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    ph_region_kernel<NBOT, MC><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+    ph_region_kernel<NBOT, MC, GM><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
     return (int)hipGetLastError();
 }
 
-template <int NBOT, int MC>
+template <int NBOT, int MC, bool GM>
 static int ph_region_occ_t(const PhRegionArgs* a, int* bpc)
 {
-    const size_t lds = sizeof(double) * (size_t)a->lds_doubles;
+    const size_t lds = GM ? 0 : sizeof(double) * (size_t)a->lds_doubles;
     return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        bpc, (const void*)ph_region_kernel<NBOT, MC>, 64, lds);
+        bpc, (const void*)ph_region_kernel<NBOT, MC, GM>, 64, lds);
 }
 
-template <typename Fn>
-static int ph_region_dispatch(int nbot, int mc, Fn&& fn)
+template <int NBOT, typename Fn>
+static int ph_region_dispatch_mc(int mc, bool gm, Fn&& fn)
 {
-    if (mc == 1) {
-        if (nbot == 1) return fn(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
-        if (nbot == 2) return fn(std::integral_constant<int, 2>{}, std::integral_constant<int, 1>{});
-        if (nbot == 3) return fn(std::integral_constant<int, 3>{}, std::integral_constant<int, 1>{});
-    } else if (mc == 2) {
-        if (nbot == 1) return fn(std::integral_constant<int, 1>{}, std::integral_constant<int, 2>{});
-        if (nbot == 2) return fn(std::integral_constant<int, 2>{}, std::integral_constant<int, 2>{});
-        if (nbot == 3) return fn(std::integral_constant<int, 3>{}, std::integral_constant<int, 2>{});
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I4 = std::integral_constant<int, 4>;
+    using I6 = std::integral_constant<int, 6>;
+    using NB = std::integral_constant<int, NBOT>;
+    using T = std::true_type;
+    using F = std::false_type;
+    if (!gm) {
+        if (mc == 1) return fn(NB{}, I1{}, F{});
+        if (mc == 2) return fn(NB{}, I2{}, F{});
+        return PH_LAUNCH_UNSUPPORTED;
     }
+    if (mc <= 2) return fn(NB{}, I2{}, T{});
+    if (mc <= 4) return fn(NB{}, I4{}, T{});
+    if (mc <= 6) return fn(NB{}, I6{}, T{});
     return PH_LAUNCH_UNSUPPORTED;
+}
+
+// The instance for a launch: coordinate slots per lane from n_max (64 coordinates a slot), the
+// block in device memory when a->work is set.
+template <typename Fn>
+static int ph_region_dispatch(const PhRegionArgs* a, Fn&& fn)
+{
+    const int mc = (a->n_max + 63) / 64;
+    const bool gm = a->work != nullptr;
+    switch (a->c.n_bottoms) {
+    case 1: return ph_region_dispatch_mc<1>(mc, gm, fn);
+    case 2: return ph_region_dispatch_mc<2>(mc, gm, fn);
+    case 3: return ph_region_dispatch_mc<3>(mc, gm, fn);
+    default: return PH_LAUNCH_UNSUPPORTED;
+    }
 }
 
 extern "C" int ph_launch_region(const PhRegionArgs* a, int grid, void* stream)
 {
-    const int mc = a->n_max <= 64 ? 1 : (a->n_max <= 128 ? 2 : 0);
-    return ph_region_dispatch(a->c.n_bottoms, mc, [&](auto nb, auto m) {
-        return ph_region_launch_t<decltype(nb)::value, decltype(m)::value>(a, grid, stream);
+    return ph_region_dispatch(a, [&](auto nb, auto m, auto gm) {
+        return ph_region_launch_t<decltype(nb)::value, decltype(m)::value, decltype(gm)::value>(
+            a, grid, stream);
     });
 }
 
 extern "C" int ph_region_occupancy(const PhRegionArgs* a, int* blocks_per_cu)
 {
-    const int mc = a->n_max <= 64 ? 1 : (a->n_max <= 128 ? 2 : 0);
-    return ph_region_dispatch(a->c.n_bottoms, mc, [&](auto nb, auto m) {
-        return ph_region_occ_t<decltype(nb)::value, decltype(m)::value>(a, blocks_per_cu);
+    return ph_region_dispatch(a, [&](auto nb, auto m, auto gm) {
+        return ph_region_occ_t<decltype(nb)::value, decltype(m)::value, decltype(gm)::value>(
+            a, blocks_per_cu);
     });
 }

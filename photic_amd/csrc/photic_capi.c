@@ -670,8 +670,8 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
     if (rc) return rc;
     if (!cfg || !out || !nodata || !sec_sun || !sec_view)
         return ph_fail(PHOTIC_EINVAL, "cfg, out, nodata, sec_sun and sec_view are required");
-    if (n_scenes < 1 || n_scenes > PH_MAXD)
-        return ph_fail(PHOTIC_EINVAL, "n_scenes %d outside 1..%d", n_scenes, PH_MAXD);
+    if (n_scenes < 1 || n_scenes > PH_MAXS)
+        return ph_fail(PHOTIC_EINVAL, "n_scenes %d outside 1..%d", n_scenes, PH_MAXS);
     if (n_rows < 1 || n_cols < 1 || n < 0 || n_spatial < 0)
         return ph_fail(PHOTIC_EINVAL, "n_rows, n_cols >= 1, n >= 0 and n_spatial >= 0 required");
     if ((long)n_rows * n_cols > 2147483647L)
@@ -686,10 +686,11 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
     const int nbot = cfg->n_bottoms;
     const int r_max = (2 * ns - 1) * (2 * ns - 1);
     const int n_max = r_max * (1 + 2 * nbot) + 3 * n_scenes;
-    if (r_max > 64 || n_max > 128)
+    if (r_max > PH_REG_MAXR || n_max > PH_REG_MAXN)
         return ph_fail(PHOTIC_EUNSUPPORTED, "%d regions of %d bottoms and %d scenes need %d "
-                                            "parameters; the kernels hold at most 128",
-                       r_max, nbot, n_scenes, n_max);
+                                            "parameters; the kernels hold at most %d regions "
+                                            "and %d parameters",
+                       r_max, nbot, n_scenes, n_max, PH_REG_MAXR, PH_REG_MAXN);
     if (n == 0) return PHOTIC_OK;
     if (!rrs) return ph_fail(PHOTIC_EINVAL, "rrs is required");
 
@@ -721,24 +722,28 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
     PH_TAKE(o_m, 3 * r_max * n_scenes);
     PH_TAKE(o_pix, r_max);
     PH_TAKE(o_bc, 32);
-    PH_TAKE(o_geo, 2 * PH_MAXD);
+    PH_TAKE(o_geo, 2 * n_scenes);
     PH_TAKE(o_hd, r_max);
     PH_TAKE(o_bqv, r_max * nbot);
     PH_TAKE(o_bd, r_max * nbot);
 #undef PH_TAKE
     a.lds_doubles = off;
-    for (int k = 0; k < 4 * n_scenes; k++) a.nodata[k] = nodata[k];
+    /* the per-scene values the kernel reads from device memory: sec(sun), sec(view), R_sigma,
+     * the no-data values */
+    double scn[10 * PH_MAXS];
+    memset(scn, 0, sizeof(scn));
     for (int s = 0; s < n_scenes; s++) {
-        a.sec_sun[s] = sec_sun[s];
-        a.sec_view[s] = sec_view[s];
+        scn[s] = sec_sun[s];
+        scn[n_scenes + s] = sec_view[s];
     }
+    if (pert && !rsig) return ph_fail(PHOTIC_EINVAL, "a perturbation needs R_sigma");
+    for (int k = 0; pert && k < 4 * n_scenes; k++) scn[2 * n_scenes + k] = rsig[k];
+    for (int k = 0; k < 4 * n_scenes; k++) scn[6 * n_scenes + k] = (double)nodata[k];
     a.n = n;
     a.rrs = rrs;
     a.idx = idx;
     a.hstart = hstart;
-    if (pert && !rsig) return ph_fail(PHOTIC_EINVAL, "a perturbation needs R_sigma");
     a.pert = pert;
-    for (int k = 0; pert && k < 4 * n_scenes; k++) a.rsig[k] = rsig[k];
     a.depth = out->depth;
     a.rrs_error = out->rrs_error;
     a.model_error = out->model_error;
@@ -761,9 +766,12 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
             hipSuccess || max_lds <= 0)
         max_lds = 65536;
-    if ((long)a.lds_doubles * 8 > max_lds)
-        return ph_fail(PHOTIC_EUNSUPPORTED, "%d regions need %d bytes of LDS per pixel, the "
-                                            "device has %d", r_max, a.lds_doubles * 8, max_lds);
+    /* a wave's block lives in LDS when it fits there (and the problem fits two coordinate
+     * slots per lane), else in device memory, one block per wave of the grid; the results are
+     * the same either way.  PHOTIC_REGION_DEVICE_BLOCKS=1 forces device memory (a test hook). */
+    const char *force_dm = getenv("PHOTIC_REGION_DEVICE_BLOCKS");
+    const int in_device = (long)a.lds_doubles * 8 > max_lds || n_max > 128 ||
+                          (force_dm && atoi(force_dm) == 1);
     if (hipDeviceGetAttribute(&wrate, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
         wrate <= 0)
         wrate = 100000; /* kHz */
@@ -771,20 +779,36 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
     a.max_ticks = (unsigned long long)(limit_s * 1000.0 * (double)wrate);
     rc = ph_n_cu(&n_cu);
     if (rc) return rc;
+    /* occupancy of the chosen instance: a non-NULL work pointer selects the device-memory one
+     * (the real pointer is set below) */
+    a.work = in_device ? (double *)&a : NULL;
     rc = ph_launch_status(ph_region_occupancy(&a, &bpc), "occupancy");
+    a.work = NULL;
     if (rc) return rc;
     if (bpc < 1) return ph_fail(PHOTIC_EUNSUPPORTED, "region kernel does not fit a CU");
     long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;
     if (grid > n) grid = n;
 
     hipStream_t st = (hipStream_t)stream;
-    unsigned long long *queue = NULL;
-    rc = ph_hip(hipMallocAsync((void **)&queue, 2 * sizeof(unsigned long long), st),
-                "hipMallocAsync");
+    /* one allocation: the work counter and watchdog flag, the per-scene values, and with
+     * device-memory blocks one block per wave of the grid */
+    const size_t scn_off = 2 * sizeof(unsigned long long);
+    const size_t work_off = scn_off + sizeof(double) * 10 * (size_t)n_scenes;
+    const size_t bytes = work_off + (in_device ? sizeof(double) * (size_t)a.lds_doubles *
+                                                     (size_t)grid : 0);
+    char *buf = NULL;
+    rc = ph_hip(hipMallocAsync((void **)&buf, bytes, st), "hipMallocAsync");
     if (rc) return rc;
-    rc = ph_launch_status(ph_launch_zero_u64(queue, 2, st), "zero work counter");
+    unsigned long long *queue = (unsigned long long *)buf;
+    rc = ph_hip(hipMemcpyAsync(buf + scn_off, scn, sizeof(double) * 10 * (size_t)n_scenes,
+                               hipMemcpyHostToDevice, st),
+                "per-scene values");
+    if (rc == PHOTIC_OK)
+        rc = ph_launch_status(ph_launch_zero_u64(queue, 2, st), "zero work counter");
     if (rc == PHOTIC_OK) {
         a.queue = queue;
+        a.scn = (const double *)(buf + scn_off);
+        a.work = in_device ? (double *)(buf + work_off) : NULL;
         rc = ph_launch_status(ph_launch_region(&a, (int)grid, st), "region kernel");
     }
     if (rc == PHOTIC_OK && cfg->check_watchdog) {
@@ -795,7 +819,7 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
         if (rc == PHOTIC_OK && flag)
             rc = ph_fail(PHOTIC_EDEVICE, "region kernel stopped by its %.0f s watchdog", limit_s);
     }
-    int rc2 = ph_hip(hipFreeAsync(queue, st), "hipFreeAsync");
+    int rc2 = ph_hip(hipFreeAsync(buf, st), "hipFreeAsync");
     return rc ? rc : rc2;
 }
 

@@ -3,12 +3,13 @@
  * Grid in, grid out, like model/samodel.c:289-1763: the caller's row-pointer band grids are
  * screened for no-data on every scene (model/samodel.c:933-947), smoothed as
  * model/common.c:232-276 does when n_smoothing_radius > 1, packed into planar pixel arrays per
- * depth-start group, inverted on the GPU -- one scene by photic_invert_h, several (the
+ * depth-start group, inverted on the GPU -- one scene by photic_invert_h, up to eight (the
  * reference's N_s = nscenes run, model/samodel.c:1808-1815) by photic_invert_stack_h with
  * each scene's sun and view angles -- and the outputs scattered back into the caller's grids
  * with the reference's defaults and sign conventions.  With n_spatial > 1 each pixel is
  * inverted jointly with its (2 n_spatial - 1)^2 neighbours (model/samodel.c:2957-3027) by
- * photic_invert_regions, on the smoothed band grids held on the device.  When the caller
+ * photic_invert_regions, on the smoothed band grids held on the device; so is every pixel of
+ * a run of more than eight scenes, with one region.  When the caller
  * passes a depth_sigma grid, the depth-error pass of model/samodel.c:1376-1482 fills it
  * (sm_sigma_pass). */
 #define _POSIX_C_SOURCE 199309L /* clock_gettime */
@@ -23,6 +24,8 @@
 
 #include "photic.h"
 #include "ph_launch.h"
+
+#define SM_MAX_SCENES PH_MAXS /* the reference's MAX_SCENES (model/common.h:46) */
 
 /* PHOTIC_TIMING set in the environment: the drop-in prints the wall time of its phases to
  * stderr (host gather, device grids, inversion with the scatter, depth-error pass). */
@@ -333,7 +336,7 @@ static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene
     double *d64 = (double *)malloc(sizeof(double) * (size_t)n_s);
     int rc = (pix && ns && grp && hs && trial && table && gidx && gpos && gns && ghs && h && st &&
               d64) ? PHOTIC_OK : PHOTIC_ENOMEM;
-    double rsig[8 * 4];
+    double rsig[SM_MAX_SCENES * 4];
     for (int d = 0; d < nd; d++)
         for (int b = 0; b < 4; b++) rsig[4 * d + b] = scene_data[scene_indexes[d]].R_sigma[b];
 
@@ -504,16 +507,21 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
             if (t->wavelengths[b] != sc->wavelengths[b]) same_bands = 0;
     }
     const int nsr = n_spatial < 1 ? 1 : n_spatial;
-    const int n_par = (2 * nsr - 1) * (2 * nsr - 1) * (1 + 2 * n_bottoms) + 3 * nscenes;
-    if (nscenes > 8 || n_spatial < 0 || nb != 4 || n_bottoms < 1 || n_bottoms > 3 ||
-        !same_bands || n_par > 128) {
-        fprintf(stderr, "photic samodel: supported are 1..8 scenes of one grid with the same "
-                        "4 bands, 1..3 bottoms and at most 128 parameters (got %d scenes, "
-                        "n_spatial %d, %d bands, %d bottoms%s); outputs left at their "
-                        "defaults\n",
-                nscenes, n_spatial, nb, n_bottoms, same_bands ? "" : ", differing bands");
+    const int n_reg = (2 * nsr - 1) * (2 * nsr - 1);
+    const int n_par = n_reg * (1 + 2 * n_bottoms) + 3 * nscenes;
+    if (nscenes > SM_MAX_SCENES || n_spatial < 0 || nb != 4 || n_bottoms < 1 ||
+        n_bottoms > 3 || !same_bands || n_reg > PH_REG_MAXR || n_par > PH_REG_MAXN) {
+        fprintf(stderr, "photic samodel: supported are 1..%d scenes of one grid with the same "
+                        "4 bands, 1..3 bottoms, n_spatial <= 4 and at most %d parameters (got "
+                        "%d scenes, n_spatial %d, %d bands, %d bottoms%s); outputs left at "
+                        "their defaults\n",
+                SM_MAX_SCENES, PH_REG_MAXN, nscenes, n_spatial, nb, n_bottoms,
+                same_bands ? "" : ", differing bands");
         return PHOTIC_EUNSUPPORTED;
     }
+    /* one scene: the single-date kernel; up to PH_MAXD scenes: the stack kernel; more, or any
+     * neighbourhood: the region kernel (one region when n_spatial <= 1) */
+    const int use_regions = n_spatial > 1 || nscenes > PH_MAXD;
     const int nd = nscenes;
     const int r = n_smoothing_radius < 1 ? 1 : n_smoothing_radius;
 
@@ -533,7 +541,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     const char *wd = getenv("PHOTIC_WATCHDOG_S");
     if (wd && atof(wd) > 0.0) cfg.watchdog_s = atof(wd);
     /* each scene's subsurface sun and view angles, as secants (model/samodel.c:534-548) */
-    double sec_sun[8], sec_view[8];
+    double sec_sun[SM_MAX_SCENES], sec_view[SM_MAX_SCENES];
     for (int k = 0; k < nd; k++) {
         const struct photic_scene *t = &scene_data[scene_indexes[k]];
         double a = t->theta_w;
@@ -641,9 +649,9 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
 
     /* every (scene, band) grid, smoothed, and its no-data value, on the device for the
      * neighbourhood gather (n_spatial > 1) and the depth-error pass */
-    float *grid = NULL, *d_grid = NULL, nod[8 * 4];
+    float *grid = NULL, *d_grid = NULL, nod[SM_MAX_SCENES * 4];
     const double t1 = sm_ms();
-    if (!rc && (n_spatial > 1 || depth_sigma)) {
+    if (!rc && (use_regions || depth_sigma)) {
         grid = (float *)malloc(sizeof(float) * 4 * (size_t)nd * (size_t)(npix ? npix : 1));
         if (!grid) rc = PHOTIC_ENOMEM;
         for (int d = 0; !rc && d < nd; d++)
@@ -668,7 +676,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
             c.n_bottoms = 1;
             c.bottom_idx[0] = 0; /* sand */
         }
-        if (n_spatial > 1)
+        if (use_regions)
             rc = sm_run_regions(&c, d_grid, nod, nrows, ncols, nd, sec_sun, sec_view, n_spatial,
                                 &gr[k], depth, model_error, bottom_albedo, bottom_sand,
                                 bottom_seagrass, bottom_coral, K_min, bottom_type,

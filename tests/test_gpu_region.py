@@ -43,6 +43,9 @@ CASES = [
     (1, 3, 1, 6, 6, False),  # 25 regions, 78 parameters, 100 pairs
     (2, 2, 2, 6, 7, False),  # two scenes
     (1, 2, 1, 5, 5, False),  # 30 parameters: one coordinate per lane
+    (3, 3, 1, 5, 5, False),  # 25 regions of 3 bottoms: 178 parameters, blocks in device memory
+    (1, 1, 12, 5, 6, False), # twelve scenes, one region each: 39 parameters
+    (1, 2, 10, 5, 5, False), # ten scenes: 57 parameters, 360 pairs
 ]
 
 
@@ -104,12 +107,32 @@ def test_regions_pixel_list_and_unsupported():
     o = P.invert_regions(cfg, rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns, idx=bad)
     assert list(o["status"][1:3]) == [-1, -1] and list(o["n_regions"][1:3]) == [0, 0]
     assert o["depth"][0] == full["depth"][idx[0]] and o["depth"][3] == full["depth"][idx[1]]
-    # 5 x 5 regions of 3 bottoms: 178 parameters
+    # 9 x 9 regions (n_spatial 5), and 7 x 7 regions of 3 bottoms with 14 scenes (343 + 42 =
+    # 385 parameters): beyond the compiled kernels
     with pytest.raises(P.PhoticError):
-        P.invert_regions(P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=3), rrs, n_rows, n_cols,
-                         -9999.0, ss, sv, n_spatial=3)
+        P.invert_regions(P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=1), rrs, n_rows, n_cols,
+                         -9999.0, ss, sv, n_spatial=5)
+    r14 = np.ascontiguousarray(np.repeat(rrs, 14, axis=0))
+    with pytest.raises(P.PhoticError):
+        P.invert_regions(P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=3), r14, n_rows, n_cols,
+                         -9999.0, np.repeat(ss, 14), np.repeat(sv, 14), n_spatial=4)
     with pytest.raises(P.PhoticError):
         P.invert_regions(P.cfg_default(), rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=2)
+
+
+def test_regions_device_memory_blocks_match_lds(monkeypatch):
+    """The same problem with each wave's block in device memory instead of LDS (the layout the
+    kernels use for problems too large for LDS) gives every output bit for bit."""
+    nbot, ns, n_rows, n_cols = 3, 2, 6, 6
+    rrs, ss, sv = region_scene(91, n_rows, n_cols, 1)
+    cfg = P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=nbot, reqmin=1e-2, konvge=100,
+                        kcount=5000, check_watchdog=1)
+    lds = P.invert_regions(cfg, rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns)
+    monkeypatch.setenv("PHOTIC_REGION_DEVICE_BLOCKS", "1")
+    dev = P.invert_regions(cfg, rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns)
+    assert (lds["status"] >= 0).sum() > 20
+    for k in lds:
+        assert np.array_equal(lds[k], dev[k]), k
 
 
 def test_field_generator_matches_oracle():

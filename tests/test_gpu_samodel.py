@@ -154,19 +154,21 @@ def _call_scenes(bands, theta_w, theta_v, n_bottoms, nodata=-9999.0, emp=None,
     return rc, {k: v.array() for k, v in outs.items()}
 
 
-@pytest.mark.parametrize("nbot,with_emp", [(2, False), (1, True)])
-def test_samodel_dropin_several_scenes(nbot, with_emp):
-    """samodel() with three scenes: the reference's N_s = 3 run in its native layout, each
+@pytest.mark.parametrize("nbot,with_emp,nd", [(2, False, 3), (1, True, 3), (1, False, 10)])
+def test_samodel_dropin_several_scenes(nbot, with_emp, nd):
+    """samodel() with several scenes: the reference's N_s = nd run in its native layout, each
     scene with its own sun and view angles, a pixel unmodelled when any scene has no data,
     optionally an empirical depth grid (its own depth start, sand only beyond 8 m) -- against
-    the oracle's native multi-date inversion, pixel by pixel."""
-    nd, nrows, ncols = 3, 16, 24
+    the oracle's native multi-date inversion, pixel by pixel.  Up to eight scenes go through
+    the stack kernel, more through the region kernel with one region."""
+    nrows, ncols = (16, 24) if nd <= 8 else (8, 10)
     n = nrows * ncols
-    rrs, _, _, _ = O.gen_stack(71 + nbot, n, n_dates=nd, p_cloud=0.0, k_min=nd)
+    rrs, _, _, _ = O.gen_stack(71 + nbot + nd, n, n_dates=nd, p_cloud=0.0, k_min=nd)
     bands = rrs.reshape(nd, 4, nrows, ncols).copy()
     bands[1, :, 2, 3] = -9999.0        # no data on one scene
     bands[2, 1, 4, 5] = -0.0004        # negative on one band of one scene
-    tw, tv = (30.0, 38.0, 45.0), (0.0, 5.0, 10.0)
+    tw = tuple(30.0 + 15.0 * d / max(nd - 1, 1) for d in range(nd))
+    tv = tuple(5.0 * d for d in range(nd))
     emp = None
     if with_emp:
         emp = np.random.default_rng(5).uniform(-15.0, -0.5, (nrows, ncols)).astype(np.float32)
