@@ -107,8 +107,9 @@ def fit_seed_skip(P, net, cfg, stream, fixed):
         outs = P.DeviceOutputs(cfg, n_c, fields=("status", "n_evals"))
         P.invert_device(cfg, n_c, d_r.value, d_s.value,
                         d_seed.value if kind == "seeded" else None, outs, stream=stream)
+        # the launches run on a non-blocking stream: wait for them before the copies back
+        P.check(P.lib.photic_synchronize())
         ev[kind] = outs.fetch()
-    P.check(P.lib.photic_synchronize())
     h_pred = d_seed.to_array(np.float32, (5, n_c))[0]
     v = (ev["seeded"]["status"] >= 0) & (ev["plain"]["status"] >= 0)
     gain = (ev["seeded"]["n_evals"].astype(np.int64) - ev["plain"]["n_evals"])[v]
@@ -121,6 +122,11 @@ def fit_seed_skip(P, net, cfg, stream, fixed):
     return {"depth_m": h, "fitted": True, "fit_scene_seed": SKIP_FIT_SCENE, "fit_pixels": n_c,
             "evals_saved_per_pixel_on_fit_scene": saved[best] / max(int(v.sum()), 1) if h else 0.0,
             "skipped_frac_on_fit_scene": float((hv < h).mean()) if h else 0.0}
+
+
+def progress(msg):
+    """One line on stderr per finished phase (the JSON line alone goes to stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def host_cpus():
@@ -189,7 +195,7 @@ def main():
                                            "ordered"),
                     default="scene")
     ap.add_argument("--dropin-side", type=int, default=4096)
-    ap.add_argument("--ordered-cpu-sample", type=int, default=65536)
+    ap.add_argument("--ordered-cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--dropin-bottoms", type=int, default=2)
     ap.add_argument("--region-side", type=int, default=512)
     ap.add_argument("--region-spatial", type=int, default=2)
@@ -248,6 +254,7 @@ def main():
     del d_truth
     fields = ("depth", "rrs_error", "model_error", "status", "n_evals", "n_starts")
 
+    progress("scene of %d pixels generated" % n)
     # ---- the warm-start network, trained once on forward-model samples (not timed) ----
     net = None
     train_info = None
@@ -315,6 +322,8 @@ def main():
             "valid_pixels": int(v.sum()),
         }
 
+    if train_info:
+        progress("network trained, seed-skip depth %s" % train_info.get("seed_skip"))
     # ---- launch shapes: the autotuner times every candidate on the first 2^21 pixels and
     # keeps the fastest for the seeded and for the plain kernel (untimed) ----
     head = "plain" if net is None else "seeded"
@@ -330,6 +339,7 @@ def main():
             del d_st
         tuning["plain"] = P.autotune(cfg, n_t, r_t.value, s_t.value, None, stream)
 
+    progress("launch shapes chosen")
     # ---- warm-up: W whole-scene steps of the headline method, untimed ----
     for _ in range(args.warmup):
         run(head, cfg, n)
@@ -347,6 +357,7 @@ def main():
     k_ms = float(np.mean(kms))
     px_s = n / (ms_per_step / 1e3)
 
+    progress("%d timed steps, %.1f ms each" % (args.steps, ms_per_step))
     # ---- the plain GPU Nelder-Mead on the same scene, beside it ----
     comparisons = {}
     if net is not None:
@@ -400,6 +411,7 @@ def main():
                 full, early_exit=ee_eq, pixels=n,
                 seeded_median_rrs_error_pct=float(np.median(res["rrs_error"][v_all])),
                 chosen_on_pixels=n_eq, seeded_median_rrs_error_pct_on_those=seeded_med)
+            progress("plain method at equal misfit: early exit %g" % ee_eq)
             comparisons["plain_equal_misfit_sweep"] = [
                 {k: e[k] for k in ("early_exit", "pixels_per_s", "median_rrs_error_pct",
                                    "evals_per_pixel")} for e in sweep]
@@ -466,6 +478,7 @@ def main():
         ns = min(args.cpu_sample, n)
         rrs_h = d_rrs.to_array(np.float32, (4, n))[:, :ns].copy()
         ss_h = d_ss.to_array(np.float32, (n,))[:ns].copy()
+        progress("CPU baseline on %d pixels" % ns)
         cpx, threads, dt = cpu_baseline(rrs_h, ss_h, ns, 20)
         line["cpu_baseline"] = {
             "value": cpx, "unit": "pixels/s", "cores": threads, "kind": "port",
@@ -987,6 +1000,7 @@ def bench_ordered(args, P):
             net.seed_device(cfg, n, d_rrs.value, d_ss.value, d_seed.value, stream=stream)
         P.invert_device(cfg, n, d_rrs.value, d_ss.value, d_seed.value if d_seed else None, outs,
                         stream=stream)
+        P.check(P.lib.photic_synchronize())  # a non-blocking stream: wait before copying back
         res = outs.fetch()
         return summary(1e3 * (time.perf_counter() - t0), res)
 
@@ -1001,6 +1015,7 @@ def bench_ordered(args, P):
     for _ in range(args.warmup):
         ordered(P.order_cfg())
     ref = ordered(P.order_cfg())
+    progress("ordered inversion, %.1f ms" % ref["ms"])
     line["ordered_lut"] = ref
     line["ordered_lut_noise_angle"] = ordered(P.order_cfg(r_sigma=[2e-4] * 4))
     line["ordered_lut_restart_5"] = ordered(P.order_cfg(restart_misfit=5.0))
@@ -1009,6 +1024,7 @@ def bench_ordered(args, P):
     line["ms_per_step"] = ref["ms"]
     line["steps"], line["warmup"] = 1, args.warmup
     line["plain"] = plain_or_seeded(None)
+    progress("ordered variants and plain done")
     dx, dt = P.trainset_device(cfg, 101, args.train_samples)
     net = P.Mlp(init_seed=1)
     net.train_device(dx.value, dt.value, args.train_samples, batch=8192,
