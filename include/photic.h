@@ -198,6 +198,21 @@ int photic_invert_regions(const photic_cfg *cfg, int n_rows, int n_cols, int n_s
                           const float *hstart, const photic_outputs *out, int *n_regions,
                           void *stream);
 
+/* photic_invert_regions with a band set per scene, as the reference reads each scene's own
+ * band count and wavelengths and derives its coefficients from them (model/samodel.c:384-527):
+ * scene s has n_bands[s] (2..8) bands at wavelengths[...] (host; the scenes' lists one after
+ * another, ascending within a scene); rrs [sum n_bands][n_rows * n_cols] (device) and nodata
+ * [sum n_bands] (host) hold the scenes' band grids one after another.  The misfit runs over
+ * every (region, scene, band) in that order, each scene's K penalty over its own bands, IOD
+ * over every (scene, region, band).  cfg's own band list is not used; everything else is as
+ * photic_invert_regions. */
+int photic_invert_regions_bands(const photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
+                                const int *n_bands, const double *wavelengths, const float *rrs,
+                                const float *nodata, const double *sec_sun,
+                                const double *sec_view, int n_spatial, long n,
+                                const int64_t *idx, const float *hstart,
+                                const photic_outputs *out, int *n_regions, void *stream);
+
 /* Synthetic stack scene (Philox as photic_gen_scene, counter word 3 = date): H ~ U(0.5, 25),
  * B ~ U(0, 1) per pixel; per date sun zenith ~ U(20, 50) deg and log-normal P, G, X; a
  * Bernoulli(p_cloud) cloud mask with at least k_min clear dates; a date whose noisy Rrs goes

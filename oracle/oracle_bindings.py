@@ -73,6 +73,12 @@ def lib():
         _lib.orc_py_invert_regions.restype = l
         _lib.orc_py_invert_regions.argtypes = [v, i, i, i, i, v, v, v, v, i, i, v, d, i, i, i, v,
                                                v, l, v, v, i, v, v, C.POINTER(PyOut)]
+        _lib.orc_py_region_objective_bands.restype = d
+        _lib.orc_py_region_objective_bands.argtypes = [v, v, i, i, i, v, v, v, i, v, v, v]
+        _lib.orc_py_invert_regions_bands.restype = l
+        _lib.orc_py_invert_regions_bands.argtypes = [v, v, i, i, i, v, v, v, v, i, i, v, d, i,
+                                                     i, i, v, v, l, v, v, i, v, v,
+                                                     C.POINTER(PyOut)]
         _lib.orc_py_eval.argtypes = [i, i, v, v, i, i, v, v, v, v, v, v]
         _lib.orc_py_region_eval.argtypes = [v, i, i, i, i, v, v, v, i, v, v, v]
         _lib.orc_py_invert_ordered.restype = l
@@ -471,6 +477,47 @@ def invert_regions(rrs, n_rows, n_cols, idx, nodata, sec_sun, sec_view, n_spatia
     return out
 
 
+def invert_regions_bands(rrs_rows, wls, n_rows, n_cols, idx, nodata, sec_sun, sec_view,
+                         n_spatial=2, n_bottoms=3, bottom_idx=(0, 1, 2), hstart_px=None,
+                         reqmin=1e-2, konvge=100, kcount=5000, hstart=HSTART8, screen=False,
+                         screen_rrs=None):
+    """invert_regions with a band set per scene: wls[s] the wavelengths of scene s; rrs_rows
+    [sum len(wls[s])][n_rows * n_cols] the scenes' band grids one after another, nodata one
+    value per band grid (or a scalar)."""
+    rrs = np.ascontiguousarray(rrs_rows, np.float32)
+    nbs = np.asarray([len(w) for w in wls], np.int32)
+    nd = len(nbs)
+    rows = int(nbs.sum())
+    assert rrs.shape == (rows, n_rows * n_cols)
+    w = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64) for x in wls]))
+    ix = np.ascontiguousarray(idx, np.int64)
+    n = len(ix)
+    rmax = (2 * max(n_spatial, 1) - 1) ** 2
+    npar = rmax * (1 + 2 * n_bottoms) + 3 * nd
+    out = _region_out(npar, n)
+    out["n_regions"] = np.zeros(n, np.int32)
+    out["origin"] = np.zeros(n, np.int32)
+    o = PyOut(**{k: v.ctypes.data for k, v in out.items() if k not in ("n_regions", "origin")})
+    hs = np.asarray(hstart, np.float64)
+    bi = np.asarray(bottom_idx, np.int32)
+    nod = np.ascontiguousarray(np.broadcast_to(np.asarray(nodata, np.float32), (rows,)))
+    ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (nd,)))
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (nd,)))
+    hp = None if hstart_px is None else np.ascontiguousarray(hstart_px, np.float32)
+    sc = None
+    if screen_rrs is not None:
+        sc = np.ascontiguousarray(screen_rrs, np.float32).reshape(rrs.shape)
+    elif screen:
+        sc = rrs
+    lib().orc_py_invert_regions_bands(w.ctypes.data, nbs.ctypes.data, nd, n_rows, n_cols,
+                                      rrs.ctypes.data, nod.ctypes.data, ss.ctypes.data,
+                                      sv.ctypes.data, n_spatial, n_bottoms, bi.ctypes.data,
+                                      reqmin, konvge, kcount, len(hs), hs.ctypes.data, _p(hp), n,
+                                      ix.ctypes.data, _p(sc), npar, out["n_regions"].ctypes.data,
+                                      out["origin"].ctypes.data, C.byref(o))
+    return out
+
+
 def region_objective(x, rrs, origin=0, sec_sun=1.0, sec_view=1.0, n_bottoms=3,
                      bottom_idx=(0, 1, 2), wl=WL4):
     """Region objective of one vector x for region data rrs [R][nd][nb]; returns (E, terms)
@@ -486,6 +533,28 @@ def region_objective(x, rrs, origin=0, sec_sun=1.0, sec_view=1.0, n_bottoms=3,
     f = lib().orc_py_region_objective(w.ctypes.data, nb, nd, R, origin, r.ctypes.data,
                                       ss.ctypes.data, sv.ctypes.data, n_bottoms, bi.ctypes.data,
                                       xv.ctypes.data, terms.ctypes.data)
+    return f, terms
+
+
+def region_objective_bands(x, rrs_rows, wls, origin=0, sec_sun=1.0, sec_view=1.0,
+                           n_bottoms=3, bottom_idx=(0, 1, 2)):
+    """region_objective with a band set per scene: wls[s] the wavelengths of scene s, rrs_rows
+    [R][sum len(wls[s])] each region's values scene by scene; returns (E, terms)."""
+    r = np.ascontiguousarray(rrs_rows, np.float64)
+    nbs = np.asarray([len(w) for w in wls], np.int32)
+    nd = len(nbs)
+    R = r.shape[0]
+    assert r.shape[1] == int(nbs.sum())
+    w = np.ascontiguousarray(np.concatenate([np.asarray(q, np.float64) for q in wls]))
+    xv = np.ascontiguousarray(x, np.float64)
+    bi = np.asarray(bottom_idx, np.int32)
+    ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (nd,)))
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (nd,)))
+    terms = np.zeros(6)
+    f = lib().orc_py_region_objective_bands(w.ctypes.data, nbs.ctypes.data, nd, R, origin,
+                                            r.ctypes.data, ss.ctypes.data, sv.ctypes.data,
+                                            n_bottoms, bi.ctypes.data, xv.ctypes.data,
+                                            terms.ctypes.data)
     return f, terms
 
 

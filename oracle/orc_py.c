@@ -493,7 +493,11 @@ void orc_py_disc_mask(uint64_t seed, long n_rows, long n_cols, int n_discs, doub
  * model/samodel.c:933-947).  Pixels not modelled or with no region left get status -1.
  * x [stride][n] (stride >= the largest parameter count) receives |best| in the reference's
  * layout; n_regions [n] and origin [n] the gather's result.  Other outputs as orc_py_invert. */
-long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_cols,
+/* The same with a band set per scene: nbs[s] bands in scene s at wavelengths wl (the scenes'
+ * lists one after another); rrs, screen_rrs [sum nbs][n_rows * n_cols] and nodata [sum nbs]
+ * hold the scenes' band grids one after another. */
+long orc_py_invert_regions_bands(const double *wl, const int *nbs, int nd, int n_rows,
+                                 int n_cols,
                            const float *rrs, const float *nodata, const double *sec_sun,
                            const double *sec_view, int n_spatial, int n_bottoms,
                            const int *bottom_idx, double reqmin, int konvge, int kcount,
@@ -502,8 +506,14 @@ long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_c
                            int *n_regions,
                            int *origin, orc_py_out *o)
 {
-    orc_bands bands;
-    orc_band_tables(wl, nb, &bands);
+    orc_bands *bands = (orc_bands *)calloc((size_t)nd, sizeof(orc_bands));
+    const orc_bands *sb[ORC_RMAXD];
+    int nrow = 0;
+    for (int s = 0; s < nd; s++) {
+        orc_band_tables(wl + nrow, nbs[s], &bands[s]);
+        sb[s] = &bands[s];
+        nrow += nbs[s];
+    }
     long done = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : done)
     for (long p = 0; p < n; p++) {
@@ -511,13 +521,13 @@ long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_c
         orc_region_result *res = (orc_region_result *)calloc(1, sizeof(orc_region_result));
         const int i = (int)(idx[p] / n_cols), j = (int)(idx[p] % n_cols);
         int bad = 0;
-        for (int t = 0; screen_rrs && t < nd * nb; t++) {
+        for (int t = 0; screen_rrs && t < nrow; t++) {
             float v = screen_rrs[(long)t * n_rows * n_cols + idx[p]];
             if (orc_approx_equal(v, nodata[t], 1.0e-6f) || v < 0.0f)
                 bad = 1;
         }
-        int R = bad ? 0 : orc_region_gather(px, rrs, nodata, nd, nb, n_rows, n_cols, i, j,
-                                            n_spatial, 0.0f, NULL);
+        int R = bad ? 0 : orc_region_gather_scenes(px, rrs, nodata, nd, nbs, n_rows, n_cols, i,
+                                                   j, n_spatial, 0.0f, NULL);
         if (n_regions) n_regions[p] = R;
         if (origin) origin[p] = px->origin;
         if (o->x)
@@ -529,7 +539,7 @@ long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_c
             free(res);
             continue;
         }
-        orc_region_init(px, n_bottoms, bottom_idx, nd, &bands, sec_sun, sec_view);
+        orc_region_init_scenes(px, n_bottoms, bottom_idx, nd, sb, sec_sun, sec_view);
         orc_nm_cfg cfg;
         orc_default_cfg(&cfg, nd, reqmin, konvge, kcount);
         cfg.n_hstarts = n_hstarts;
@@ -562,7 +572,31 @@ long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_c
         free(res);
         done++;
     }
+    free(bands);
     return done;
+}
+
+long orc_py_invert_regions(const double *wl, int nb, int nd, int n_rows, int n_cols,
+                           const float *rrs, const float *nodata, const double *sec_sun,
+                           const double *sec_view, int n_spatial, int n_bottoms,
+                           const int *bottom_idx, double reqmin, int konvge, int kcount,
+                           int n_hstarts, const double *hstart, const float *hstart_px,
+                           long n, const long *idx, const float *screen_rrs, int x_stride,
+                           int *n_regions,
+                           int *origin, orc_py_out *o)
+{
+    int nbs[ORC_RMAXD];
+    double *w = (double *)malloc(sizeof(double) * (size_t)(nd * nb));
+    for (int s = 0; s < nd; s++) {
+        nbs[s] = nb;
+        for (int b = 0; b < nb; b++) w[s * nb + b] = wl[b];
+    }
+    long r = orc_py_invert_regions_bands(w, nbs, nd, n_rows, n_cols, rrs, nodata, sec_sun,
+                                         sec_view, n_spatial, n_bottoms, bottom_idx, reqmin,
+                                         konvge, kcount, n_hstarts, hstart, hstart_px, n, idx,
+                                         screen_rrs, x_stride, n_regions, origin, o);
+    free(w);
+    return r;
 }
 
 static orc_region_pixel *orc_py_region_px(const double *wl, int nb, int nd, int R, int origin,
@@ -607,6 +641,48 @@ double orc_py_region_objective(const double *wl, int nb, int nd, int R, int orig
         terms[5] = px->model_error;
     }
     free(px);
+    return f;
+}
+
+/* orc_py_region_objective with a band set per scene: nbs[s] bands in scene s at wl (the
+ * scenes' lists one after another); rrs [R][sum nbs] each region's values, scene by scene. */
+double orc_py_region_objective_bands(const double *wl, const int *nbs, int nd, int R,
+                                     int origin, const double *rrs, const double *sec_sun,
+                                     const double *sec_view, int n_bottoms,
+                                     const int *bottom_idx, const double *x, double *terms)
+{
+    orc_bands *bands = (orc_bands *)calloc((size_t)nd, sizeof(orc_bands));
+    const orc_bands *sb[ORC_RMAXD];
+    int rows = 0;
+    for (int s = 0; s < nd; s++) {
+        orc_band_tables(wl + rows, nbs[s], &bands[s]);
+        sb[s] = &bands[s];
+        rows += nbs[s];
+    }
+    orc_region_pixel *px = (orc_region_pixel *)calloc(1, sizeof(orc_region_pixel));
+    px->n_regions = R;
+    px->origin = origin;
+    for (int r = 0; r < R; r++) {
+        int at = 0;
+        for (int s = 0; s < nd; s++)
+            for (int b = 0; b < nbs[s]; b++)
+                px->rrs[r][s][b] = rrs[(long)r * rows + at++];
+    }
+    orc_region_init_scenes(px, n_bottoms, bottom_idx, nd, sb, sec_sun, sec_view);
+    double xv[ORC_RMAXN];
+    for (int i = 0; i < orc_region_n_params(px); i++)
+        xv[i] = x[i];
+    double f = orc_region_objective(xv, px);
+    if (terms) {
+        terms[0] = px->Rrs_error;
+        terms[1] = px->depth_error;
+        terms[2] = px->bottom_error;
+        terms[3] = px->K_error;
+        terms[4] = px->bottom_albedo;
+        terms[5] = px->model_error;
+    }
+    free(px);
+    free(bands);
     return f;
 }
 

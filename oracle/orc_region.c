@@ -56,9 +56,9 @@ int orc_region_n_params(const orc_region_pixel *px)
  * only -- negative values are kept) is dropped; the origin is the index the centre candidate
  * had when it was visited (the last such candidate at a clamped edge).  rsig [nd * nb] or NULL:
  * a perturbed gather adds n_sigma R_sigma to each kept value (model/samodel.c:3003-3007). */
-int orc_region_gather(orc_region_pixel *px, const float *rrs /*[nd][nb][n_rows*n_cols]*/,
-                      const float *nodata /*[nd*nb]*/, int nd, int nb, int n_rows, int n_cols,
-                      int i, int j, int n_spatial, float n_sigma, const double *rsig)
+int orc_region_gather_scenes(orc_region_pixel *px, const float *rrs, const float *nodata,
+                             int nd, const int *nbs, int n_rows, int n_cols, int i, int j,
+                             int n_spatial, float n_sigma, const double *rsig)
 {
     const long npix = (long)n_rows * n_cols;
     int ns = n_spatial == 0 ? 1 : n_spatial;
@@ -68,19 +68,21 @@ int orc_region_gather(orc_region_pixel *px, const float *rrs /*[nd][nb][n_rows*n
         int iii = i + ii < 0 ? 0 : (i + ii > n_rows - 1 ? n_rows - 1 : i + ii);
         for (int jj = 1 - ns; jj < ns; jj++) {
             int jjj = j + jj < 0 ? 0 : (j + jj > n_cols - 1 ? n_cols - 1 : j + jj);
-            int bad = 0;
-            for (int s = 0; s < nd && !bad; s++)
-                for (int b = 0; b < nb; b++) {
-                    float v = rrs[((long)s * nb + b) * npix + (long)iii * n_cols + jjj];
-                    if (!orc_approx_equal(v, nodata[s * nb + b], 1.0e-6f)) {
+            int bad = 0, row = 0;
+            for (int s = 0; s < nd && !bad; s++) {
+                for (int b = 0; b < nbs[s]; b++) {
+                    float v = rrs[(long)(row + b) * npix + (long)iii * n_cols + jjj];
+                    if (!orc_approx_equal(v, nodata[row + b], 1.0e-6f)) {
                         px->rrs[kr][s][b] = (double)v;
                         if (rsig && !orc_approx_equal(n_sigma, 0.0f, 1.0e-6f))
-                            px->rrs[kr][s][b] += n_sigma * rsig[s * nb + b];
+                            px->rrs[kr][s][b] += n_sigma * rsig[row + b];
                     } else {
                         bad = 1;
                         break;
                     }
                 }
+                row += nbs[s];
+            }
             if (i == iii && j == jjj)
                 px->origin = kr;
             if (!bad)
@@ -91,32 +93,55 @@ int orc_region_gather(orc_region_pixel *px, const float *rrs /*[nd][nb][n_rows*n
     return kr;
 }
 
+int orc_region_gather(orc_region_pixel *px, const float *rrs /*[nd][nb][n_rows*n_cols]*/,
+                      const float *nodata /*[nd*nb]*/, int nd, int nb, int n_rows, int n_cols,
+                      int i, int j, int n_spatial, float n_sigma, const double *rsig)
+{
+    int nbs[ORC_RMAXD];
+    for (int s = 0; s < nd; s++)
+        nbs[s] = nb;
+    return orc_region_gather_scenes(px, rrs, nodata, nd, nbs, n_rows, n_cols, i, j, n_spatial,
+                                    n_sigma, rsig);
+}
+
 /* Band tables, geometry and the per-region Rrs at 440/490/550/640/750 nm
  * (model/samodel.c:1785-1799); px->rrs, n_regions and origin already set. */
-void orc_region_init(orc_region_pixel *px, int n_bottoms, const int *bottom_idx, int n_dates,
-                     const orc_bands *bands, const double *sec_sun, const double *sec_view)
+void orc_region_init_scenes(orc_region_pixel *px, int n_bottoms, const int *bottom_idx,
+                            int n_dates, const orc_bands *const *sb, const double *sec_sun,
+                            const double *sec_view)
 {
     px->n_bottoms = n_bottoms;
     for (int k = 0; k < n_bottoms; k++)
         px->bottom_idx[k] = bottom_idx ? bottom_idx[k] : k;
     px->n_dates = n_dates;
-    px->bands = bands;
-    const int nb = bands->n_bands;
+    px->bands = sb[0];
     for (int s = 0; s < n_dates; s++) {
+        px->sb[s] = sb[s];
         px->sec_sun[s] = sec_sun[s];
         px->sec_view[s] = sec_view[s];
     }
     for (int r = 0; r < px->n_regions; r++)
         for (int s = 0; s < n_dates; s++) {
-            px->r440[r][s] = orc_interp_1d(bands->wl, px->rrs[r][s], nb, 440.0);
-            px->r490[r][s] = orc_interp_1d(bands->wl, px->rrs[r][s], nb, 490.0);
-            px->r550[r][s] = orc_interp_1d(bands->wl, px->rrs[r][s], nb, 550.0);
-            px->r640[r][s] = orc_interp_1d(bands->wl, px->rrs[r][s], nb, 640.0);
-            px->r750[r][s] = orc_interp_1d(bands->wl, px->rrs[r][s], nb, 750.0);
+            const orc_bands *t = sb[s];
+            const int nb = t->n_bands;
+            px->r440[r][s] = orc_interp_1d(t->wl, px->rrs[r][s], nb, 440.0);
+            px->r490[r][s] = orc_interp_1d(t->wl, px->rrs[r][s], nb, 490.0);
+            px->r550[r][s] = orc_interp_1d(t->wl, px->rrs[r][s], nb, 550.0);
+            px->r640[r][s] = orc_interp_1d(t->wl, px->rrs[r][s], nb, 640.0);
+            px->r750[r][s] = orc_interp_1d(t->wl, px->rrs[r][s], nb, 750.0);
             if (px->r440[r][s] < 0.0)
                 px->r440[r][s] = 0.0001;
         }
     px->n_evals = 0;
+}
+
+void orc_region_init(orc_region_pixel *px, int n_bottoms, const int *bottom_idx, int n_dates,
+                     const orc_bands *bands, const double *sec_sun, const double *sec_view)
+{
+    const orc_bands *sb[ORC_RMAXD];
+    for (int s = 0; s < n_dates; s++)
+        sb[s] = bands;
+    orc_region_init_scenes(px, n_bottoms, bottom_idx, n_dates, sb, sec_sun, sec_view);
 }
 
 /* samodel_error (model/samodel.c:2432-2759) over every region: the misfit over (region,
@@ -127,8 +152,6 @@ double orc_region_objective(double *x, void *ctx)
 {
     orc_region_pixel *px = (orc_region_pixel *)ctx;
     const int R = px->n_regions, nd = px->n_dates, nbot = px->n_bottoms;
-    const int nbd = px->bands->n_bands;
-    const orc_bands *t = px->bands;
     const int off = R + 2 * nbot * R;
     px->n_evals++;
 
@@ -136,6 +159,8 @@ double orc_region_objective(double *x, void *ctx)
     int n_total = 0;
     for (int r = 0; r < R; r++) {
         for (int s = 0; s < nd; s++) {
+            const orc_bands *t = px->sb[s];
+            const int nbd = t->n_bands;
             double H = fabs(x[r]);
             double P = 0.01 * fabs(x[off + 3 * s]);
             double G = 0.01 * fabs(x[off + 1 + 3 * s]);
@@ -235,7 +260,7 @@ double orc_region_objective(double *x, void *ctx)
     double Kerr = 0.0, Kmin = 1.0e4;
     for (int s = 0; s < nd; s++) {
         Kmin = 1.0e4;
-        for (int b = 0; b < nbd; b++) {
+        for (int b = 0; b < px->sb[s]->n_bands; b++) {
             double k = px->K[s][b];
             if (!orc_approx_equal((float)k, 0.0f, 1.0e-6f) && k < Kmin)
                 Kmin = k;
@@ -322,7 +347,7 @@ void orc_region_start_step(const orc_region_pixel *px, double h0, double *start,
 static void orc_region_outputs(orc_region_pixel *px, const double *best, orc_region_result *res)
 {
     const int R = px->n_regions, nd = px->n_dates, nbot = px->n_bottoms;
-    const int nbd = px->bands->n_bands, n = orc_region_n_params(px);
+    const int n = orc_region_n_params(px);
     res->n_params = n;
     res->n_regions = R;
     for (int i = 0; i < n; i++)
@@ -333,7 +358,7 @@ static void orc_region_outputs(orc_region_pixel *px, const double *best, orc_reg
 
     double kmin = 1.0e10;
     for (int s = 0; s < nd; s++)
-        for (int b = 0; b < nbd; b++) {
+        for (int b = 0; b < px->sb[s]->n_bands; b++) {
             double k = px->K[s][b];
             if (!orc_approx_equal((float)k, 0.0f, 1.0e-4f) && k < kmin)
                 kmin = k;
@@ -371,7 +396,7 @@ static void orc_region_outputs(orc_region_pixel *px, const double *best, orc_reg
     double iod = 0.0, nobs = 0.0;
     for (int s = 0; s < nd; s++)
         for (int r = 0; r < R; r++)
-            for (int b = 0; b < nbd; b++) {
+            for (int b = 0; b < px->sb[s]->n_bands; b++) {
                 iod += px->rrs_bot[r][s][b] / px->rrs_mod[r][s][b];
                 nobs += 1.0;
             }

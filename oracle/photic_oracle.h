@@ -195,7 +195,8 @@ typedef struct {
     int n_regions, origin;
     int n_bottoms, bottom_idx[ORC_RMAXBOT];
     int n_dates;
-    const orc_bands *bands;
+    const orc_bands *bands;            /* scene 0's bands */
+    const orc_bands *sb[ORC_RMAXD];    /* each scene's bands (model/samodel.c:384-527) */
     double sec_sun[ORC_RMAXD], sec_view[ORC_RMAXD];
     double rrs[ORC_MAXR][ORC_RMAXD][ORC_MAXB]; /* measured, [region][scene][band] */
     double r440[ORC_MAXR][ORC_RMAXD], r490[ORC_MAXR][ORC_RMAXD], r550[ORC_MAXR][ORC_RMAXD];
@@ -223,8 +224,17 @@ int orc_region_n_params(const orc_region_pixel *px);
 int orc_region_gather(orc_region_pixel *px, const float *rrs, const float *nodata, int nd,
                       int nb, int n_rows, int n_cols, int i, int j, int n_spatial,
                       float n_sigma, const double *rsig);
+/* The same with nbs[s] bands in scene s: rrs holds the scenes' band grids one after another
+ * ([sum nbs][n_rows * n_cols]), nodata and rsig one value per band grid. */
+int orc_region_gather_scenes(orc_region_pixel *px, const float *rrs, const float *nodata,
+                             int nd, const int *nbs, int n_rows, int n_cols, int i, int j,
+                             int n_spatial, float n_sigma, const double *rsig);
 void orc_region_init(orc_region_pixel *px, int n_bottoms, const int *bottom_idx, int n_dates,
                      const orc_bands *bands, const double *sec_sun, const double *sec_view);
+/* The same with each scene's own band set sb[s]. */
+void orc_region_init_scenes(orc_region_pixel *px, int n_bottoms, const int *bottom_idx,
+                            int n_dates, const orc_bands *const *sb, const double *sec_sun,
+                            const double *sec_view);
 double orc_region_objective(double *x, void *px);
 void orc_region_start_step(const orc_region_pixel *px, double h0, double *start, double *step);
 /* The region objective at x and the region outputs derived from it (a test hook). */

@@ -155,6 +155,8 @@ def _load():
                                  vp]),
         "photic_invert_regions": (i, [C.POINTER(Cfg), i, i, i, vp, vp, vp, vp, i, l, vp, vp,
                                       C.POINTER(Outputs), vp, vp]),
+        "photic_invert_regions_bands": (i, [C.POINTER(Cfg), i, i, i, vp, vp, vp, vp, vp, vp, i,
+                                            l, vp, vp, C.POINTER(Outputs), vp, vp]),
         "photic_autotune": (i, [C.POINTER(Cfg), l, vp, vp, vp, C.POINTER(Tuning), vp]),
         "photic_autotune_stack": (i, [C.POINTER(Cfg), l, i, vp, vp, vp, vp, C.POINTER(Tuning),
                                       vp]),
@@ -480,6 +482,36 @@ def invert_regions(cfg, rrs, n_rows, n_cols, nodata, sec_sun, sec_view, n_spatia
     invert_regions_device(cfg, n_rows, n_cols, nd, d_rrs.value, nodata, sec_sun, sec_view,
                           n_spatial, n, d_idx.value if d_idx is not None else None,
                           d_h.value if d_h is not None else None, outs)
+    check(lib.photic_synchronize())
+    return outs.fetch()
+
+
+def invert_regions_bands(cfg, rrs_rows, wavelengths, n_rows, n_cols, nodata, sec_sun, sec_view,
+                         n_spatial=2, idx=None, hstart=None, fields=None):
+    """photic_invert_regions_bands on host arrays: wavelengths[s] the band list of scene s;
+    rrs_rows [sum len(wavelengths[s])][n_rows * n_cols] float32, the scenes' band grids one
+    after another; nodata one value per band grid (or a scalar); sec_sun / sec_view
+    [n_scenes]; idx [n] int64 pixel indices or None; hstart [n] or None."""
+    rrs = np.ascontiguousarray(rrs_rows, dtype=np.float32)
+    nbs = np.asarray([len(w) for w in wavelengths], np.int32)
+    nd = len(nbs)
+    rows = int(nbs.sum())
+    assert rrs.shape == (rows, n_rows * n_cols)
+    wl = np.ascontiguousarray(np.concatenate([np.asarray(w, np.float64) for w in wavelengths]))
+    n = n_rows * n_cols if idx is None else len(idx)
+    d_rrs = DeviceBuffer.from_array(rrs)
+    d_idx = DeviceBuffer.from_array(np.asarray(idx, np.int64)) if idx is not None else None
+    d_h = DeviceBuffer.from_array(np.asarray(hstart, np.float32)) if hstart is not None else None
+    outs = RegionOutputs(cfg, n, nd, n_spatial, fields)
+    nod = np.ascontiguousarray(np.broadcast_to(np.asarray(nodata, np.float32), (rows,)))
+    ss = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_sun, np.float64), (nd,)))
+    sv = np.ascontiguousarray(np.broadcast_to(np.asarray(sec_view, np.float64), (nd,)))
+    check(lib.photic_invert_regions_bands(C.byref(cfg), n_rows, n_cols, nd, nbs.ctypes.data,
+                                          wl.ctypes.data, d_rrs.value, nod.ctypes.data,
+                                          ss.ctypes.data, sv.ctypes.data, n_spatial, n,
+                                          d_idx.value if d_idx is not None else None,
+                                          d_h.value if d_h is not None else None,
+                                          C.byref(outs.struct), outs.n_regions.value, None))
     check(lib.photic_synchronize())
     return outs.fetch()
 

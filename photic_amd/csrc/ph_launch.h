@@ -16,6 +16,8 @@
 #define PH_MAXS 32 /* scenes of a region problem (the reference's MAX_SCENES) */
 #define PH_REG_MAXR 64  /* regions of a pixel: (2 n_spatial - 1)^2, n_spatial <= 4 */
 #define PH_REG_MAXN 384 /* parameters of a region problem: 64 per lane slot, 6 slots */
+#define PH_REG_SLOT 10  /* doubles per band slot in PhRegionArgs::scn */
+#define PH_REG_SCENE 22 /* doubles per scene in PhRegionArgs::scn */
 
 #define PH_LAUNCH_UNSUPPORTED 100001
 
@@ -99,21 +101,29 @@ typedef struct PhStackArgs {
  * [(n_max + 1) n_max] from offset 0, then the arrays at the o_* offsets (in doubles):
  * o_x the trial point [n_max], o_d squared residual per pair,
  * o_q rB / r per pair, o_rm measured Rrs per pair, o_pw (440 / lambda)^Y per pair
- * [np_max each], o_k the last region's K [4 n_scenes], o_m Rrs at 490 / 550 / 640 nm per
- * (region, scene) [3 r_max n_scenes], o_pix the regions' pixel indices [r_max], o_bc the band
- * coefficients [4][8] (a0, a1, a_w, b_bw, exp(-0.015 (lambda - 440)), the bottoms'
- * reflectances), o_geo each scene's sec(sun) then sec(view) [2 n_scenes], o_hd each region's
+ * [np_max each; pair j = (region j / n_slots, band slot j % n_slots)], o_k the last region's
+ * K per band slot [n_slots], o_m Rrs at 490 / 550 / 640 nm per (region, scene) [3 r_max
+ * n_scenes], o_pix the regions' pixel indices [r_max], o_bc the band slots' coefficients
+ * [n_slots][PH_REG_SLOT] (as in scn), o_geo each scene's sec(sun) then sec(view) [2 n_scenes],
+ * o_sc each scene's first band slot and band count [2 n_scenes], o_hd each region's
  * depth-penalty term [r_max], o_bqv and o_bd each (bottom, region)'s fraction and penalty term
  * [n_bottoms r_max each]. */
 typedef struct PhRegionArgs {
     PhConst c;                     /* c.n_dates = n_scenes, c.n_bottoms native bottoms */
     int n_rows, n_cols, n_spatial, n_scenes;
+    int n_slots;                   /* band slots: the scenes' band counts summed */
     int r_max, n_max, np_max;      /* regions, parameters, (region, scene, band) pairs */
     int lds_doubles;
-    int o_x, o_d, o_q, o_rm, o_pw, o_k, o_m, o_pix, o_bc, o_geo, o_hd, o_bqv, o_bd;
-    const double *scn;             /* per scene, device: sec(sun) [n_scenes], sec(view)
-                                      [n_scenes], R_sigma [n_scenes][4], the no-data value
-                                      of each (scene, band) grid [n_scenes][4] */
+    int o_x, o_d, o_q, o_rm, o_pw, o_k, o_m, o_pix, o_bc, o_geo, o_sc, o_hd, o_bqv, o_bd;
+    const double *scn;             /* device tables: sec(sun) [n_scenes], sec(view)
+                                      [n_scenes]; per band slot (scene s's band b, the
+                                      scenes' bands one after another) R_sigma [n_slots] and
+                                      the grid's no-data value [n_slots]; per slot a0, a1,
+                                      a_w, b_bw, exp(-0.015 (lambda - 440)), the bottoms'
+                                      reflectances (3), 440 / lambda, 16 s + b
+                                      [n_slots][PH_REG_SLOT]; per scene its first slot, its
+                                      band count and the interpolation (i0, i1, w0, w1) at
+                                      440, 490, 550, 640, 750 nm [n_scenes][PH_REG_SCENE] */
     double *work;                  /* device blocks [grid][lds_doubles] when they live there */
     long n;                        /* pixels to invert */
     const float *rrs;              /* [n_scenes][4][n_rows * n_cols] */
@@ -226,13 +236,25 @@ int ph_launch_mlp_stack_mean(long n, int n_dates, const unsigned char *valid, fl
 /* photic_invert_regions with a perturbation: pert [n] (device) n_sigma per listed pixel, rsig
  * [4 n_scenes] (host) R_sigma per (scene, band); every region's values become
  * Rrs + n_sigma R_sigma (model/samodel.c:3003-3007); depth64 [n] (device) or NULL receives each
- * depth in double, as the reference keeps it.  For the drop-in's depth-error pass. */
+ * depth in double, as the reference keeps it.  For the drop-in's depth-error pass.  Every
+ * scene has cfg's bands. */
 int ph_invert_regions_pert(const struct photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
                            const float *rrs, const float *nodata, const double *sec_sun,
                            const double *sec_view, int n_spatial, long n, const int64_t *idx,
                            const float *hstart, const float *pert, const double *rsig,
                            const struct photic_outputs *out, int *n_regions, double *depth64,
                            void *stream);
+/* The same with a band set per scene (model/samodel.c:384-527): scene s has n_bands[s] (2..8)
+ * bands at wavelengths[...] (the scenes' lists one after another, ascending within a scene);
+ * rrs [sum n_bands][n_rows * n_cols] (device), nodata and rsig [sum n_bands] (host) hold the
+ * scenes' band grids one after another.  cfg's own band list is not used. */
+int ph_invert_regions_bands(const struct photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
+                            const int *n_bands, const double *wavelengths, const float *rrs,
+                            const float *nodata, const double *sec_sun, const double *sec_view,
+                            int n_spatial, long n, const int64_t *idx, const float *hstart,
+                            const float *pert, const double *rsig,
+                            const struct photic_outputs *out, int *n_regions, double *depth64,
+                            void *stream);
 
 /* Host side (photic_capi.c): the launch constants for a photic_cfg; exported for the test
  * suite's host build of the lane engine. */

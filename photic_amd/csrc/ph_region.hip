@@ -62,13 +62,24 @@ PH_DEV double ph_seq_sum(const double* v, int n)
     return s;
 }
 
+// j / d for 0 <= j < 2^23 and d >= 1 from the float reciprocal inv = 1 / d, corrected to the
+// exact quotient (the estimate is within one of it).
+PH_DEV int ph_div_small(int j, int d, float inv)
+{
+    int r = (int)((float)j * inv);
+    if (r * d > j) r--;
+    else if ((r + 1) * d <= j) r++;
+    return r;
+}
+
 template <int MC>
 struct PhReg {
     // vertex value k lives in lane k % 64, slot k / 64
     static constexpr int YC = MC + 1;
     double* g; // the wave's LDS block: the simplex, [vertex][coordinate], stride N
     int li, R, N, NP, nd, origin;
-    int nd_magic; // ceil(2^16 / nd): x / nd = (x nd_magic) >> 16 for x < 64 nd, nd <= 32
+    int p1;       // band slots of one region: the scenes' band counts summed
+    float p1inv;  // 1 / p1
     double rq;    // reqmin * N
     double xs[MC], xb[MC], xe[MC], step[MC], best[MC], xs0[MC], step0[MC];
     double yv[YC];
@@ -145,10 +156,11 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
     ph_wave_sync();
     const double* X = g + a.o_x;
     for (int j = li; j < NP; j += 64) {
-        const int r = ((j >> 2) * S.nd_magic) >> 16; // (j / 4) / nd: j / 4 < 64 nd
-        const int s = (j >> 2) - r * nd;
-        const int b = j & 3;
-        const double* bc = g + a.o_bc + 8 * b; // a0, a1, aw, bbw, eg, bottoms of band b
+        // pair j = (region r, band slot): the slot is band b of scene s
+        const int r = ph_div_small(j, S.p1, S.p1inv);
+        const int slot = j - r * S.p1;
+        const double* bc = g + a.o_bc + PH_REG_SLOT * slot; // a0, a1, aw, bbw, eg, bottoms, ...
+        const int s = (int)bc[9] >> 4;
         const double H = fabs(X[r]);
         double Bk[NBOT], q[NBOT], qsum = 0.0;
 #pragma unroll
@@ -172,7 +184,7 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
                                       g[a.o_pw + j], ss, sv, &K, &rB, &rr);
         const double d = m - g[a.o_rm + j];
         g[a.o_d + j] = d * d;
-        if (r == R - 1) g[a.o_k + 4 * s + b] = K; // the reference keeps the last region's K
+        if (r == R - 1) g[a.o_k + slot] = K; // the reference keeps the last region's K
         if (side_on) g[a.o_q + j] = rB / rr;
     }
     // lane r < R: region r's bottom fractions |B_k| |q_k| / sum |q| (model/samodel.c:2648-2686),
@@ -195,9 +207,9 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
     if (li < nd) {
         const double H = fabs(X[S.origin]);
         const double kmm = 0.275, kmn = 0.185;
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const double K = g[a.o_k + 4 * li + b];
+        const int s0 = (int)g[a.o_sc + 2 * li], nbs = (int)g[a.o_sc + 2 * li + 1];
+        for (int b = 0; b < nbs; b++) {
+            const double K = g[a.o_k + s0 + b];
             if (!ph_approx_equal((float)K, 0.0f, 1.0e-6f) && K < kmin_s) kmin_s = K;
         }
         double thr = -1.0;
@@ -287,18 +299,20 @@ PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const 
     e /= 120.0;
     if (side_on) {
         double kmin_all = 1.0e10;
-        for (int j = 0; j < 4 * nd; j++) {
+        for (int j = 0; j < S.p1; j++) {
             const double K = g[a.o_k + j];
             if (!ph_approx_equal((float)K, 0.0f, 1.0e-4f) && K < kmin_all) kmin_all = K;
         }
         // IOD in (scene, region, band) order (model/samodel.c:2051-2064)
         double iod = 0.0, nobs = 0.0;
-        for (int s = 0; s < nd; s++)
+        for (int s = 0; s < nd; s++) {
+            const int s0 = (int)g[a.o_sc + 2 * s], nbs = (int)g[a.o_sc + 2 * s + 1];
             for (int r = 0; r < R; r++)
-                for (int b = 0; b < 4; b++) {
-                    iod += g[a.o_q + (r * nd + s) * 4 + b];
+                for (int b = 0; b < nbs; b++) {
+                    iod += g[a.o_q + r * S.p1 + s0 + b];
                     nobs += 1.0;
                 }
+        }
         // the albedo of the last band evaluated: region R - 1's sum q_k B_k
         double Bk[NBOT], q[NBOT], qsum = 0.0, alb = 0.0;
 #pragma unroll
@@ -677,29 +691,25 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
     const int li = S.li;
     const int nd = a.n_scenes;
     S.nd = nd;
-    S.nd_magic = (65536 + nd - 1) / nd;
+    const int P1 = a.n_slots;
+    S.p1 = P1;
+    S.p1inv = 1.0f / (float)P1;
     const long npix = (long)a.n_rows * a.n_cols;
     // watchdog: the wall clock at the wave's last progress (a pixel taken); a wave that
     // finishes no pixel within max_ticks gives up and raises the flag
     unsigned long long t0 = wall_clock64();
     double* g = S.g;
-    // the launch's band coefficients ([band][a0, a1, aw, bbw, eg, bottoms]) and each scene's
-    // geometry, once per wave
-    if (li < 32) {
-        const int b = li >> 3, w = li & 7;
-        double v = 0.0;
-        for (int bb = 0; bb < 4; bb++) {
-            if (bb != b) continue;
-            v = w == 0 ? c.a0[bb] : (w == 1 ? c.a1[bb] : (w == 2 ? c.aw[bb] : (w == 3 ? c.bbw[bb] : c.eg[bb])));
-#pragma unroll
-            for (int k = 0; k < NBOT; k++)
-                if (w == 5 + k) v = c.bot[c.bottom_idx[k]][bb];
-        }
-        g[a.o_bc + li] = v;
+    // the per-call tables (ph_launch.h, PhRegionArgs::scn): each band slot's coefficients,
+    // each scene's geometry and its band slots, once per wave
+    const double* scn_rsig = a.scn + 2 * nd;                              // [slot]
+    const double* scn_nodata = scn_rsig + P1;                             // [slot], floats
+    const double* scn_slot = scn_nodata + P1;                             // [slot][PH_REG_SLOT]
+    const double* scn_scene = scn_slot + PH_REG_SLOT * P1;                // [scene][PH_REG_SCENE]
+    for (int t = li; t < PH_REG_SLOT * P1; t += 64) g[a.o_bc + t] = scn_slot[t];
+    for (int t = li; t < 2 * nd; t += 64) {
+        g[a.o_geo + t] = a.scn[t];
+        g[a.o_sc + t] = scn_scene[PH_REG_SCENE * (t >> 1) + (t & 1)]; // first slot, band count
     }
-    for (int t = li; t < 2 * nd; t += 64) g[a.o_geo + t] = a.scn[t];
-    const double* scn_rsig = a.scn + 2 * nd;   // [scene][band]
-    const double* scn_nodata = a.scn + 6 * nd; // [scene][band], float values
 
     for (;;) {
         if (wall_clock64() - t0 > a.max_ticks) {
@@ -724,7 +734,7 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
         // negative (model/samodel.c:933-947)
         if (!a.idx) {
             bool bad = false;
-            for (int t = li; t < 4 * nd; t += 64) {
+            for (int t = li; t < P1; t += 64) {
                 const float v = a.rrs[(long)t * npix + p];
                 bad |= ph_approx_equal(v, (float)scn_nodata[t], 1.0e-6f) || v < 0.0f;
             }
@@ -749,7 +759,7 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
             cpix = (long)iii * a.n_cols + jjj;
             centre = (iii == pi && jjj == pj);
             ok = true;
-            for (int t = 0; t < 4 * nd; t++)
+            for (int t = 0; t < P1; t++)
                 if (ph_approx_equal(a.rrs[(long)t * npix + cpix], (float)scn_nodata[t], 1.0e-6f))
                     ok = false;
         }
@@ -764,7 +774,7 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
             continue;
         }
         const int N = ph_uni(R * (1 + 2 * NBOT) + 3 * nd);
-        const int NP = ph_uni(R * nd * 4);
+        const int NP = ph_uni(R * P1); // (region, band slot) pairs
         S.R = R;
         S.N = N;
         S.NP = NP;
@@ -779,34 +789,46 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
         // region's 440/490 ratio, model/samodel.c:2898-2903); per (region, scene) Rrs at
         // 490, 550 and 640 nm for the starts ----
         for (int j = li; j < NP; j += 64) {
-            const int r = j / (4 * nd);
-            const int s = (j >> 2) - r * nd;
-            const int b = j & 3;
+            // pair j = (region r, band slot qs), slot qs band b of scene s
+            const int r = ph_div_small(j, P1, S.p1inv);
+            const int qs = j - r * P1;
+            const int sb = (int)g[a.o_bc + PH_REG_SLOT * qs + 9];
+            const int s = sb >> 4, b = sb & 15;
+            const double* sc = scn_scene + PH_REG_SCENE * s;
+            const int row0 = (int)sc[0], nbs = (int)sc[1];
             const long rp = (long)g[a.o_pix + r];
-            double r4[4];
+            double rv[PH_MAXB];
 #pragma unroll
-            for (int bb = 0; bb < 4; bb++) r4[bb] = (double)a.rrs[((long)s * 4 + bb) * npix + rp];
+            for (int bb = 0; bb < PH_MAXB; bb++)
+                rv[bb] = bb < nbs ? (double)a.rrs[(long)(row0 + bb) * npix + rp] : 0.0;
             // a perturbed run adds n_sigma R_sigma to every region's values after the no-data
             // test (model/samodel.c:3003-3007)
             if (a.pert && !ph_approx_equal(a.pert[q], 0.0f, 1.0e-6f)) {
                 const double ns_ = (double)a.pert[q];
 #pragma unroll
-                for (int bb = 0; bb < 4; bb++) r4[bb] += ns_ * scn_rsig[s * 4 + bb];
+                for (int bb = 0; bb < PH_MAXB; bb++)
+                    if (bb < nbs) rv[bb] += ns_ * scn_rsig[row0 + bb];
             }
-            g[a.o_rm + j] = ph_band_sel(r4, b);
-            double r440 = ph_rrs_at(c, r4, 0);
-            const double r490 = ph_rrs_at(c, r4, 1);
+            // Rrs at 440 / 490 / 550 / 640 nm by the scene's own interpolation weights
+            // (model/common.c:298-335): ph_rrs_at's operations
+            auto rrs_at = [&](int k) {
+                const double* w = sc + 2 + 4 * k; // i0, i1, w0, w1
+                return ph_sel(rv, (int)w[0]) * w[2] + ph_sel(rv, (int)w[1]) * w[3];
+            };
+            g[a.o_rm + j] = ph_sel(rv, b);
+            double r440 = rrs_at(0);
+            const double r490 = rrs_at(1);
             if (r440 < 0.0) r440 = 0.0001;
             const double chi = r440 / r490;
             double Y = 3.44 * (1.0 - 3.17 * PH_EXP(-2.01 * chi));
             if (Y < 0.0) Y = 0.0;
             if (Y > 2.5) Y = 2.5;
-            g[a.o_pw + j] = PH_POW(440.0 / ph_band_sel(c.wl, b), Y);
+            g[a.o_pw + j] = PH_POW(g[a.o_bc + PH_REG_SLOT * qs + 8], Y); // (440 / lambda)^Y
             if (b == 0) {
                 const int t = r * nd + s;
                 g[a.o_m + 3 * t] = r490;
-                g[a.o_m + 3 * t + 1] = ph_rrs_at(c, r4, 2);
-                g[a.o_m + 3 * t + 2] = ph_rrs_at(c, r4, 3);
+                g[a.o_m + 3 * t + 1] = rrs_at(2);
+                g[a.o_m + 3 * t + 2] = rrs_at(3);
             }
         }
         ph_wave_sync();

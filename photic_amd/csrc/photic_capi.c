@@ -659,6 +659,18 @@ int photic_invert_regions(const photic_cfg *cfg, int n_rows, int n_cols, int n_s
                                   stream);
 }
 
+int photic_invert_regions_bands(const photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
+                                const int *n_bands, const double *wavelengths, const float *rrs,
+                                const float *nodata, const double *sec_sun,
+                                const double *sec_view, int n_spatial, long n,
+                                const int64_t *idx, const float *hstart,
+                                const photic_outputs *out, int *n_regions, void *stream)
+{
+    return ph_invert_regions_bands(cfg, n_rows, n_cols, n_scenes, n_bands, wavelengths, rrs,
+                                   nodata, sec_sun, sec_view, n_spatial, n, idx, hstart, NULL,
+                                   NULL, out, n_regions, NULL, stream);
+}
+
 int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
                            const float *rrs, const float *nodata, const double *sec_sun,
                            const double *sec_view, int n_spatial, long n, const int64_t *idx,
@@ -666,22 +678,54 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
                            const photic_outputs *out, int *n_regions, double *depth64,
                            void *stream)
 {
-    int rc = ph_have_device();
-    if (rc) return rc;
-    if (!cfg || !out || !nodata || !sec_sun || !sec_view)
-        return ph_fail(PHOTIC_EINVAL, "cfg, out, nodata, sec_sun and sec_view are required");
+    if (!cfg) return ph_fail(PHOTIC_EINVAL, "cfg is NULL");
     if (n_scenes < 1 || n_scenes > PH_MAXS)
         return ph_fail(PHOTIC_EINVAL, "n_scenes %d outside 1..%d", n_scenes, PH_MAXS);
+    if (cfg->n_bands < 2 || cfg->n_bands > PH_MAXB)
+        return ph_fail(PHOTIC_EINVAL, "n_bands %d outside 2..%d", cfg->n_bands, PH_MAXB);
+    /* every scene with cfg's bands */
+    int nbs[PH_MAXS];
+    double wl[PH_MAXS * PH_MAXB];
+    for (int s = 0; s < n_scenes; s++) {
+        nbs[s] = cfg->n_bands;
+        for (int b = 0; b < cfg->n_bands; b++) wl[s * cfg->n_bands + b] = cfg->wavelengths[b];
+    }
+    return ph_invert_regions_bands(cfg, n_rows, n_cols, n_scenes, nbs, wl, rrs, nodata, sec_sun,
+                                   sec_view, n_spatial, n, idx, hstart, pert, rsig, out,
+                                   n_regions, depth64, stream);
+}
+
+int ph_invert_regions_bands(const photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
+                            const int *n_bands, const double *wavelengths, const float *rrs,
+                            const float *nodata, const double *sec_sun, const double *sec_view,
+                            int n_spatial, long n, const int64_t *idx, const float *hstart,
+                            const float *pert, const double *rsig, const photic_outputs *out,
+                            int *n_regions, double *depth64, void *stream)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    if (!cfg || !out || !nodata || !sec_sun || !sec_view || !n_bands || !wavelengths)
+        return ph_fail(PHOTIC_EINVAL, "cfg, out, nodata, sec_sun, sec_view, n_bands and "
+                                      "wavelengths are required");
+    if (n_scenes < 1 || n_scenes > PH_MAXS)
+        return ph_fail(PHOTIC_EINVAL, "n_scenes %d outside 1..%d", n_scenes, PH_MAXS);
+    int n_slots = 0;
+    for (int s = 0; s < n_scenes; s++) {
+        if (n_bands[s] < 2 || n_bands[s] > PH_MAXB)
+            return ph_fail(PHOTIC_EINVAL, "scene %d: %d bands, outside 2..%d", s, n_bands[s],
+                           PH_MAXB);
+        n_slots += n_bands[s];
+    }
     if (n_rows < 1 || n_cols < 1 || n < 0 || n_spatial < 0)
         return ph_fail(PHOTIC_EINVAL, "n_rows, n_cols >= 1, n >= 0 and n_spatial >= 0 required");
     if ((long)n_rows * n_cols > 2147483647L)
         return ph_fail(PHOTIC_EINVAL, "scene above 2^31 - 1 pixels");
     if (!idx && n != (long)n_rows * n_cols)
         return ph_fail(PHOTIC_EINVAL, "without idx, n must be n_rows * n_cols");
-    if (cfg->mode != PHOTIC_MODE_NATIVE || cfg->n_bands != 4 || cfg->precision != PHOTIC_FP64 ||
+    if (cfg->mode != PHOTIC_MODE_NATIVE || cfg->precision != PHOTIC_FP64 ||
         cfg->n_bottoms < 1 || cfg->n_bottoms > 3)
         return ph_fail(PHOTIC_EUNSUPPORTED, "regions are compiled for the native layout with "
-                                            "1..3 bottoms, 4 bands, FP64");
+                                            "1..3 bottoms, FP64");
     const int ns = n_spatial == 0 ? 1 : n_spatial;
     const int nbot = cfg->n_bottoms;
     const int r_max = (2 * ns - 1) * (2 * ns - 1);
@@ -703,9 +747,10 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
     a.n_cols = n_cols;
     a.n_spatial = n_spatial;
     a.n_scenes = n_scenes;
+    a.n_slots = n_slots;
     a.r_max = r_max;
     a.n_max = n_max;
-    a.np_max = r_max * n_scenes * 4;
+    a.np_max = r_max * n_slots;
     /* the wave's LDS block: the simplex, then each array at an even offset */
     int off = (n_max + 1) * n_max;
 #define PH_TAKE(field, count)          \
@@ -718,27 +763,64 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
     PH_TAKE(o_q, a.np_max);
     PH_TAKE(o_rm, a.np_max);
     PH_TAKE(o_pw, a.np_max);
-    PH_TAKE(o_k, 4 * n_scenes);
+    PH_TAKE(o_k, n_slots);
     PH_TAKE(o_m, 3 * r_max * n_scenes);
     PH_TAKE(o_pix, r_max);
-    PH_TAKE(o_bc, 32);
+    PH_TAKE(o_bc, PH_REG_SLOT * n_slots);
     PH_TAKE(o_geo, 2 * n_scenes);
+    PH_TAKE(o_sc, 2 * n_scenes);
     PH_TAKE(o_hd, r_max);
     PH_TAKE(o_bqv, r_max * nbot);
     PH_TAKE(o_bd, r_max * nbot);
 #undef PH_TAKE
     a.lds_doubles = off;
-    /* the per-scene values the kernel reads from device memory: sec(sun), sec(view), R_sigma,
-     * the no-data values */
-    double scn[10 * PH_MAXS];
-    memset(scn, 0, sizeof(scn));
-    for (int s = 0; s < n_scenes; s++) {
-        scn[s] = sec_sun[s];
-        scn[n_scenes + s] = sec_view[s];
-    }
+    /* the tables the kernel reads from device memory (PhRegionArgs::scn): geometry, then per
+     * band slot R_sigma, the no-data value and the band's coefficients (model/samodel.c:
+     * 518-603: each scene's own wavelengths), then per scene its slots and interpolation
+     * weights (model/common.c:298-335) */
     if (pert && !rsig) return ph_fail(PHOTIC_EINVAL, "a perturbation needs R_sigma");
-    for (int k = 0; pert && k < 4 * n_scenes; k++) scn[2 * n_scenes + k] = rsig[k];
-    for (int k = 0; k < 4 * n_scenes; k++) scn[6 * n_scenes + k] = (double)nodata[k];
+    const size_t scn_n = (size_t)(2 * n_scenes + (2 + PH_REG_SLOT) * n_slots +
+                                  PH_REG_SCENE * n_scenes);
+    double scn[2 * PH_MAXS + (2 + PH_REG_SLOT) * PH_MAXS * PH_MAXB + PH_REG_SCENE * PH_MAXS];
+    memset(scn, 0, sizeof(double) * scn_n);
+    {
+        double *ss = scn, *sr = scn + 2 * n_scenes, *sn = sr + n_slots, *sl = sn + n_slots;
+        double *sc = sl + PH_REG_SLOT * n_slots;
+        int slot = 0;
+        for (int s = 0; s < n_scenes && !rc; s++) {
+            ss[s] = sec_sun[s];
+            ss[n_scenes + s] = sec_view[s];
+            photic_cfg one = *cfg;
+            one.n_bands = n_bands[s];
+            for (int b = 0; b < n_bands[s]; b++) one.wavelengths[b] = wavelengths[slot + b];
+            PhConst cs;
+            rc = ph_make_const(&one, 1, 0, &cs);
+            if (rc) break;
+            double *st = sc + PH_REG_SCENE * s;
+            st[0] = slot;
+            st[1] = n_bands[s];
+            for (int k = 0; k < 5; k++) {
+                st[2 + 4 * k] = cs.ip_i0[k];
+                st[3 + 4 * k] = cs.ip_i1[k];
+                st[4 + 4 * k] = cs.ip_w0[k];
+                st[5 + 4 * k] = cs.ip_w1[k];
+            }
+            for (int b = 0; b < n_bands[s]; b++, slot++) {
+                sr[slot] = rsig ? rsig[slot] : 0.0;
+                sn[slot] = (double)nodata[slot];
+                double *t = sl + PH_REG_SLOT * slot;
+                t[0] = cs.a0[b];
+                t[1] = cs.a1[b];
+                t[2] = cs.aw[b];
+                t[3] = cs.bbw[b];
+                t[4] = cs.eg[b];
+                for (int k = 0; k < nbot; k++) t[5 + k] = cs.bot[cfg->bottom_idx[k]][b];
+                t[8] = 440.0 / cs.wl[b];
+                t[9] = 16 * s + b;
+            }
+        }
+    }
+    if (rc) return rc;
     a.n = n;
     a.rrs = rrs;
     a.idx = idx;
@@ -793,16 +875,16 @@ int ph_invert_regions_pert(const photic_cfg *cfg, int n_rows, int n_cols, int n_
     /* one allocation: the work counter and watchdog flag, the per-scene values, and with
      * device-memory blocks one block per wave of the grid */
     const size_t scn_off = 2 * sizeof(unsigned long long);
-    const size_t work_off = scn_off + sizeof(double) * 10 * (size_t)n_scenes;
+    const size_t work_off = scn_off + sizeof(double) * scn_n;
     const size_t bytes = work_off + (in_device ? sizeof(double) * (size_t)a.lds_doubles *
                                                      (size_t)grid : 0);
     char *buf = NULL;
     rc = ph_hip(hipMallocAsync((void **)&buf, bytes, st), "hipMallocAsync");
     if (rc) return rc;
     unsigned long long *queue = (unsigned long long *)buf;
-    rc = ph_hip(hipMemcpyAsync(buf + scn_off, scn, sizeof(double) * 10 * (size_t)n_scenes,
+    rc = ph_hip(hipMemcpyAsync(buf + scn_off, scn, sizeof(double) * scn_n,
                                hipMemcpyHostToDevice, st),
-                "per-scene values");
+                "per-call tables");
     if (rc == PHOTIC_OK)
         rc = ph_launch_status(ph_launch_zero_u64(queue, 2, st), "zero work counter");
     if (rc == PHOTIC_OK) {

@@ -65,9 +65,17 @@ static float sm_smoothed(float **a, int i, int j, int nrows, int ncols, int r, f
 typedef struct {
     long n;
     long *idx;      /* pixel index i * ncols + j */
-    float *rrs;     /* [nd][4][n] */
+    float *rrs;     /* [rows][n]: the scenes' band values one after another */
     float *hstart;  /* [n] or NULL */
 } sm_group;
+
+/* The scenes' band sets: scene d has nb[d] bands at wl[off[d] .. off[d] + nb[d]), the band
+ * grids of all scenes one after another (rows in all). */
+typedef struct {
+    int rows;
+    int nb[SM_MAX_SCENES], off[SM_MAX_SCENES];
+    double wl[SM_MAX_SCENES * PH_MAXB];
+} sm_bands;
 
 /* nd > 1: the scenes' sec(sun) and sec(view), one value per scene */
 static int sm_run_group(const photic_cfg *cfg, sm_group *g, int nd, const double *sec_sun,
@@ -156,7 +164,8 @@ static int sm_run_group(const photic_cfg *cfg, sm_group *g, int nd, const double
  * device (ph_invert_regions_pert; pert [n] host n_sigma per pixel, or NULL); h receives the
  * host outputs [9][n] (depth, rrs_error, albedo, k_min, iod, bottom_type, b_pct[3]) and st
  * [n] the status. */
-static int sm_invert_regions(const photic_cfg *cfg, const float *d_grid, const float *nodata,
+static int sm_invert_regions(const photic_cfg *cfg, const sm_bands *bd, const float *d_grid,
+                             const float *nodata,
                              int nrows, int ncols, int nd, const double *sec_sun,
                              const double *sec_view, int n_spatial, long n, const long *idx,
                              const float *hstart, const float *pert, const double *rsig,
@@ -198,9 +207,9 @@ static int sm_invert_regions(const photic_cfg *cfg, const float *d_grid, const f
         o.bottom_type = d_out + 5 * n;
         o.b_pct = d_out + 6 * n;
         o.status = d_st;
-        rc = ph_invert_regions_pert(cfg, nrows, ncols, nd, d_grid, nodata, sec_sun, sec_view,
-                                    n_spatial, n, d_idx, d_h, d_pert, rsig, &o, NULL, d_d64,
-                                    NULL);
+        rc = ph_invert_regions_bands(cfg, nrows, ncols, nd, bd->nb, bd->wl, d_grid, nodata,
+                                     sec_sun, sec_view, n_spatial, n, d_idx, d_h, d_pert, rsig,
+                                     &o, NULL, d_d64, NULL);
         if (!rc) rc = photic_synchronize();
     }
     if (!rc) rc = photic_memcpy_dtoh(h, d_out, sizeof(float) * 9 * (size_t)n);
@@ -218,7 +227,8 @@ static int sm_invert_regions(const photic_cfg *cfg, const float *d_grid, const f
 
 /* n_spatial > 1: each depth-start group's pixel list inverted with its neighbourhoods, the
  * outputs scattered into the caller's grids. */
-static int sm_run_regions(const photic_cfg *cfg, const float *d_grid, const float *nodata,
+static int sm_run_regions(const photic_cfg *cfg, const sm_bands *bd, const float *d_grid,
+                          const float *nodata,
                           int nrows, int ncols, int nd, const double *sec_sun,
                           const double *sec_view, int n_spatial, sm_group *g, float **depth,
                           float **model_error, float **bottom_albedo, float **bottom_sand,
@@ -231,7 +241,7 @@ static int sm_run_regions(const photic_cfg *cfg, const float *d_grid, const floa
     int *st = (int *)malloc(sizeof(int) * (size_t)n);
     int rc = (h && st) ? PHOTIC_OK : PHOTIC_ENOMEM;
     if (!rc)
-        rc = sm_invert_regions(cfg, d_grid, nodata, nrows, ncols, nd, sec_sun, sec_view,
+        rc = sm_invert_regions(cfg, bd, d_grid, nodata, nrows, ncols, nd, sec_sun, sec_view,
                                n_spatial, n, g->idx, g->hstart, NULL, NULL, h, st, NULL);
     for (long p = 0; !rc && p < n; p++) {
         if (st[p] < 0) continue; /* no region left: the defaults stay */
@@ -297,7 +307,8 @@ static void sm_uniforms(uint32_t c0, uint32_t c1, uint32_t c2, double *u0, doubl
     *u1 = (double)((((uint64_t)o[2] << 32) | o[3]) >> 11) * 0x1p-53;
 }
 
-static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene_data,
+static int sm_sigma_pass(const photic_cfg *cfg, const sm_bands *bd,
+                         const struct photic_scene *scene_data,
                          const int *scene_indexes, int nd, int nrows, int ncols, int n_spatial,
                          int empirical_depth_present, struct photic_geogrid empirical_depths,
                          const double *sec_sun, const double *sec_view, const float *d_grid,
@@ -336,9 +347,10 @@ static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene
     double *d64 = (double *)malloc(sizeof(double) * (size_t)n_s);
     int rc = (pix && ns && grp && hs && trial && table && gidx && gpos && gns && ghs && h && st &&
               d64) ? PHOTIC_OK : PHOTIC_ENOMEM;
-    double rsig[SM_MAX_SCENES * 4];
+    double rsig[SM_MAX_SCENES * PH_MAXB];
     for (int d = 0; d < nd; d++)
-        for (int b = 0; b < 4; b++) rsig[4 * d + b] = scene_data[scene_indexes[d]].R_sigma[b];
+        for (int b = 0; b < bd->nb[d]; b++)
+            rsig[bd->off[d] + b] = scene_data[scene_indexes[d]].R_sigma[b];
 
     /* the samples: a pixel in the interval, its n_sigma, its depth-start group (each sample
      * from its own Philox counters, so the samples are independent of one another) */
@@ -416,7 +428,7 @@ static int sm_sigma_pass(const photic_cfg *cfg, const struct photic_scene *scene
             c.n_bottoms = 1;
             c.bottom_idx[0] = 0; /* sand */
         }
-        rc = sm_invert_regions(&c, d_grid, nodata, nrows, ncols, nd, sec_sun, sec_view,
+        rc = sm_invert_regions(&c, bd, d_grid, nodata, nrows, ncols, nd, sec_sun, sec_view,
                                n_spatial < 1 ? 1 : n_spatial, m, gidx, k ? ghs : NULL, gns, rsig,
                                h, st, d64);
         for (long p = 0; !rc && p < m; p++) trial[gpos[p]] = st[p] >= 0 ? d64[p] : 0.0;
@@ -484,6 +496,11 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     if (!scene_data || !gridded_data || !scene_indexes || nscenes < 1) return PHOTIC_EINVAL;
     const struct photic_scene *sc = &scene_data[scene_indexes[0]];
     const int nrows = sc->nrows, ncols = sc->ncols, nb = sc->n_bands;
+    if (nscenes > SM_MAX_SCENES) {
+        fprintf(stderr, "photic samodel: at most %d scenes (got %d); outputs left unset\n",
+                SM_MAX_SCENES, nscenes);
+        return PHOTIC_EUNSUPPORTED;
+    }
 
     /* the reference's defaults for every pixel (model/samodel.c:819-841) */
     for (int i = 0; i < nrows; i++)
@@ -499,36 +516,50 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
             bottom_type[i][j] = -9999.0f;
             index_optical_depth[i][j] = 0.0f;
         }
-    int same_bands = 1;
-    for (int k = 1; k < nscenes && same_bands; k++) {
+    /* each scene's band set (model/samodel.c:384-527): 2..8 bands at ascending wavelengths,
+     * every scene on the first one's grid */
+    sm_bands bd;
+    memset(&bd, 0, sizeof(bd));
+    int bands_ok = 1, same_bands = 1;
+    for (int k = 0; k < nscenes; k++) {
         const struct photic_scene *t = &scene_data[scene_indexes[k]];
-        if (t->n_bands != nb || t->nrows != nrows || t->ncols != ncols) same_bands = 0;
-        for (int b = 0; same_bands && b < nb && b < 4; b++)
+        if (t->n_bands < 2 || t->n_bands > PH_MAXB || t->nrows != nrows || t->ncols != ncols) {
+            bands_ok = 0;
+            break;
+        }
+        bd.nb[k] = t->n_bands;
+        bd.off[k] = bd.rows;
+        for (int b = 0; b < t->n_bands; b++) {
+            bd.wl[bd.rows + b] = (double)t->wavelengths[b];
+            if (b > 0 && !(t->wavelengths[b] > t->wavelengths[b - 1])) bands_ok = 0;
+        }
+        bd.rows += t->n_bands;
+        if (t->n_bands != 4 || t->n_bands != nb) same_bands = 0;
+        for (int b = 0; same_bands && b < 4; b++)
             if (t->wavelengths[b] != sc->wavelengths[b]) same_bands = 0;
     }
     const int nsr = n_spatial < 1 ? 1 : n_spatial;
     const int n_reg = (2 * nsr - 1) * (2 * nsr - 1);
     const int n_par = n_reg * (1 + 2 * n_bottoms) + 3 * nscenes;
-    if (nscenes > SM_MAX_SCENES || n_spatial < 0 || nb != 4 || n_bottoms < 1 ||
-        n_bottoms > 3 || !same_bands || n_reg > PH_REG_MAXR || n_par > PH_REG_MAXN) {
-        fprintf(stderr, "photic samodel: supported are 1..%d scenes of one grid with the same "
-                        "4 bands, 1..3 bottoms, n_spatial <= 4 and at most %d parameters (got "
-                        "%d scenes, n_spatial %d, %d bands, %d bottoms%s); outputs left at "
-                        "their defaults\n",
-                SM_MAX_SCENES, PH_REG_MAXN, nscenes, n_spatial, nb, n_bottoms,
-                same_bands ? "" : ", differing bands");
+    if (!bands_ok || n_spatial < 0 || n_bottoms < 1 || n_bottoms > 3 || n_reg > PH_REG_MAXR ||
+        n_par > PH_REG_MAXN) {
+        fprintf(stderr, "photic samodel: supported are 1..%d scenes of one grid with 2..%d "
+                        "ascending bands each, 1..3 bottoms, n_spatial <= 4 and at most %d "
+                        "parameters (got %d scenes, n_spatial %d, %d bottoms); outputs left "
+                        "at their defaults\n",
+                SM_MAX_SCENES, PH_MAXB, PH_REG_MAXN, nscenes, n_spatial, n_bottoms);
         return PHOTIC_EUNSUPPORTED;
     }
-    /* one scene: the single-date kernel; up to PH_MAXD scenes: the stack kernel; more, or any
-     * neighbourhood: the region kernel (one region when n_spatial <= 1) */
-    const int use_regions = n_spatial > 1 || nscenes > PH_MAXD;
+    /* one scene: the single-date kernel; up to PH_MAXD scenes: the stack kernel -- both for
+     * scenes sharing the 4-band set they are compiled for; more scenes, other band sets or
+     * any neighbourhood: the region kernel (one region when n_spatial <= 1) */
+    const int use_regions = n_spatial > 1 || nscenes > PH_MAXD || !same_bands;
     const int nd = nscenes;
+    const int rows = bd.rows;
     const int r = n_smoothing_radius < 1 ? 1 : n_smoothing_radius;
 
-    double wl[4];
-    for (int b = 0; b < 4; b++) wl[b] = (double)sc->wavelengths[b];
     photic_cfg cfg;
-    photic_cfg_default(&cfg, 4, wl); /* the reference's Nelder-Mead settings */
+    photic_cfg_default(&cfg, bd.nb[0], bd.wl); /* the reference's Nelder-Mead settings */
     cfg.mode = PHOTIC_MODE_NATIVE;
     cfg.n_bottoms = n_bottoms;
     /* the pixels are screened below on their raw values (model/samodel.c:933-947); the
@@ -563,7 +594,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     int rc = PHOTIC_OK;
     for (int k = 0; k < 3 && !rc; k++) {
         gr[k].idx = (long *)malloc(sizeof(long) * (size_t)(npix ? npix : 1));
-        gr[k].rrs = (float *)malloc(sizeof(float) * 4 * nd * (size_t)(npix ? npix : 1));
+        gr[k].rrs = (float *)malloc(sizeof(float) * rows * (size_t)(npix ? npix : 1));
         gr[k].hstart = k ? (float *)malloc(sizeof(float) * (size_t)(npix ? npix : 1)) : NULL;
         if (!gr[k].idx || !gr[k].rrs || (k && !gr[k].hstart)) rc = PHOTIC_ENOMEM;
     }
@@ -577,7 +608,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     if (!rc) {
         code = (signed char *)malloc((size_t)(npix ? npix : 1));
         hrow = (float *)malloc(sizeof(float) * (size_t)(npix ? npix : 1));
-        vrow = (float *)malloc(sizeof(float) * 4 * nd * (size_t)(npix ? npix : 1));
+        vrow = (float *)malloc(sizeof(float) * rows * (size_t)(npix ? npix : 1));
         cnt = (long *)calloc(3 * (size_t)nrows + 3, sizeof(long));
         if (!code || !hrow || !vrow || !cnt) rc = PHOTIC_ENOMEM;
     }
@@ -589,7 +620,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
                 int bad = 0;
                 for (int d = 0; d < nd; d++) {
                     const struct photic_scene *t = &scene_data[scene_indexes[d]];
-                    for (int b = 0; b < 4; b++) {
+                    for (int b = 0; b < bd.nb[d]; b++) {
                         const struct photic_geogrid *g = &gridded_data[t->band_indexes[b]];
                         float refl = g->array[i][j];
                         if (sm_feq(refl, g->nodata_value, 1.0e-6f) || refl < 0.0f) bad = 1;
@@ -599,7 +630,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
                         /* one region: a smoothed no-data value drops it (with several, the
                          * region gather on the device decides) */
                         if (n_spatial <= 1 && sm_feq(sv, g->nodata_value, 1.0e-6f)) bad = 1;
-                        vrow[(long)(4 * d + b) * npix + p] = sv;
+                        vrow[(long)(bd.off[d] + b) * npix + p] = sv;
                     }
                 }
                 code[p] = -1;
@@ -631,7 +662,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
                 sm_group *g = &gr[k];
                 const long m = at[k]++;
                 g->idx[m] = p;
-                for (int q = 0; q < 4 * nd; q++)
+                for (int q = 0; q < rows; q++)
                     g->rrs[(long)q * npix + m] = vrow[(long)q * npix + p];
                 if (k) g->hstart[m] = hrow[p];
             }
@@ -643,31 +674,31 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     free(cnt);
     /* compact the planar (scene, band) rows to n per group */
     for (int k = 0; k < 3 && !rc; k++)
-        for (int q = 1; q < 4 * nd; q++)
+        for (int q = 1; q < rows; q++)
             memmove(gr[k].rrs + (long)q * gr[k].n, gr[k].rrs + (long)q * npix,
                     sizeof(float) * (size_t)gr[k].n);
 
     /* every (scene, band) grid, smoothed, and its no-data value, on the device for the
      * neighbourhood gather (n_spatial > 1) and the depth-error pass */
-    float *grid = NULL, *d_grid = NULL, nod[SM_MAX_SCENES * 4];
+    float *grid = NULL, *d_grid = NULL, nod[SM_MAX_SCENES * PH_MAXB];
     const double t1 = sm_ms();
     if (!rc && (use_regions || depth_sigma)) {
-        grid = (float *)malloc(sizeof(float) * 4 * (size_t)nd * (size_t)(npix ? npix : 1));
+        grid = (float *)malloc(sizeof(float) * rows * (size_t)(npix ? npix : 1));
         if (!grid) rc = PHOTIC_ENOMEM;
         for (int d = 0; !rc && d < nd; d++)
-            for (int b = 0; b < 4; b++) {
+            for (int b = 0; b < bd.nb[d]; b++) {
                 const struct photic_geogrid *g =
                     &gridded_data[scene_data[scene_indexes[d]].band_indexes[b]];
-                float *dst = grid + (long)(4 * d + b) * npix;
-                nod[4 * d + b] = g->nodata_value;
+                float *dst = grid + (long)(bd.off[d] + b) * npix;
+                nod[bd.off[d] + b] = g->nodata_value;
                 for (int i = 0; i < nrows; i++)
                     for (int j = 0; j < ncols; j++)
                         dst[(long)i * ncols + j] =
                             r > 1 ? sm_smoothed(g->array, i, j, nrows, ncols, r, g->nodata_value)
                                   : g->array[i][j];
             }
-        if (!rc) rc = photic_device_malloc((void **)&d_grid, sizeof(float) * 4 * nd * npix);
-        if (!rc) rc = photic_memcpy_htod(d_grid, grid, sizeof(float) * 4 * nd * npix);
+        if (!rc) rc = photic_device_malloc((void **)&d_grid, sizeof(float) * rows * npix);
+        if (!rc) rc = photic_memcpy_htod(d_grid, grid, sizeof(float) * rows * npix);
     }
     const double t2 = sm_ms();
     for (int k = 0; k < 3 && !rc; k++) {
@@ -677,7 +708,8 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
             c.bottom_idx[0] = 0; /* sand */
         }
         if (use_regions)
-            rc = sm_run_regions(&c, d_grid, nod, nrows, ncols, nd, sec_sun, sec_view, n_spatial,
+            rc = sm_run_regions(&c, &bd, d_grid, nod, nrows, ncols, nd, sec_sun, sec_view,
+                                n_spatial,
                                 &gr[k], depth, model_error, bottom_albedo, bottom_sand,
                                 bottom_seagrass, bottom_coral, K_min, bottom_type,
                                 index_optical_depth);
@@ -688,7 +720,7 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     }
     const double t3 = sm_ms();
     if (!rc && depth_sigma)
-        rc = sm_sigma_pass(&cfg, scene_data, scene_indexes, nd, nrows, ncols, n_spatial,
+        rc = sm_sigma_pass(&cfg, &bd, scene_data, scene_indexes, nd, nrows, ncols, n_spatial,
                            empirical_depth_present, empirical_depths, sec_sun, sec_view, d_grid,
                            nod, depth, depth_sigma);
     if (getenv("PHOTIC_TIMING"))

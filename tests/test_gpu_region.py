@@ -151,3 +151,77 @@ def test_field_generator_matches_oracle():
     # smooth: neighbouring depths differ by well under a metre
     H = otruth[0].reshape(96, 80)
     assert np.abs(np.diff(H, axis=0)).max() < 1.0 and np.abs(np.diff(H, axis=1)).max() < 1.0
+
+
+WL5 = (443.0, 490.0, 560.0, 620.0, 665.0)
+WL6 = (412.0, 443.0, 490.0, 531.0, 555.0, 670.0)
+
+
+def _band_set_scene(seed, n_rows, n_cols, wls):
+    """Scenes of the same pixels seen through different band sets: rows [sum nb][n], with a
+    no-data pixel on every scene and a negative value on the last scene."""
+    n = n_rows * n_cols
+    rows = np.concatenate([O.gen(seed, n, wl=w)[0] for w in wls]).astype(np.float32)
+    rows[:, 1 * n_cols + 1] = -9999.0
+    rows[-1, 2 * n_cols + 2] = -0.0005
+    return np.ascontiguousarray(rows)
+
+
+@pytest.mark.parametrize("wls,nbot,ns", [((O.WL4, WL5), 2, 2), ((WL6,), 1, 2),
+                                         ((WL5, O.WL4, WL6), 1, 1)])
+def test_regions_band_sets_against_oracle(wls, nbot, ns):
+    """Scenes with their own band sets (the reference's per-scene bands, model/samodel.c:
+    384-527): statuses, region counts and outputs as the oracle's, at least 98% bit for bit,
+    the rest within the bar or flat."""
+    n_rows, n_cols = 5, 6
+    rows = _band_set_scene(300 + nbot + ns, n_rows, n_cols, wls)
+    nd = len(wls)
+    ss = [1.1 + 0.05 * s for s in range(nd)]
+    sv = [1.0 + 0.02 * s for s in range(nd)]
+    n = n_rows * n_cols
+    cfg = P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=nbot, reqmin=1e-2, konvge=100,
+                        kcount=5000, check_watchdog=1)
+    print("band sets %s, %d bottoms, n_spatial %d" % ([len(w) for w in wls], nbot, ns),
+          flush=True)
+    g = P.invert_regions_bands(cfg, rows, wls, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns)
+    o = O.invert_regions_bands(rows, wls, n_rows, n_cols, np.arange(n), -9999.0, ss, sv,
+                               n_spatial=ns, n_bottoms=nbot, bottom_idx=(0, 1, 2), screen=True)
+    assert np.array_equal(g["status"], o["status"])
+    assert np.array_equal(g["n_regions"], o["n_regions"])
+    v = o["status"] >= 0
+    # not modelled exactly where a band of a scene is no-data or negative (the screen)
+    assert np.array_equal(~v, (rows < 0).any(axis=0)) and (~v).sum() >= 2
+    exact = np.all(g["x"] == o["x"], axis=0)
+    flat = np.abs(g["model_error"] - o["model_error"]) <= 1e-6 * np.abs(o["model_error"])
+    # the bar: H within 1 cm, the other parameters within 1e-4 relative, where both converge
+    conv = (g["status"] == 0) & (o["status"] == 0)
+    rel = np.abs(g["x"] - o["x"]) <= 1e-4 * np.abs(o["x"])
+    within = conv & (np.abs(g["depth"] - o["depth"]) < 0.01) & np.all(rel, axis=0)
+    for p in np.flatnonzero(v & ~exact):
+        print("  pixel %d: depth %.6f / %.6f, misfit %.9g / %.9g, status %d / %d, regions %d"
+              % (p, g["depth"][p], o["depth"][p], g["model_error"][p], o["model_error"][p],
+                 g["status"][p], o["status"][p], g["n_regions"][p]), flush=True)
+    # a pixel may part from the oracle where the oracle's C-library exp/log/pow is not
+    # correctly rounded and a comparison flips: at most one here, and then on a rough misfit
+    # surface (E_Rrs near 96% for the six-band scene's corner pixel, where the kernel ends
+    # 2 cm away at a lower misfit), so it must end no worse than the oracle
+    part = v & ~(exact | flat | within)
+    assert (v & ~exact).sum() <= 1
+    assert np.all(g["model_error"][part] <= o["model_error"][part])
+    sel = v & exact
+    for k in ("depth", "rrs_error", "model_error", "k_min", "iod", "albedo", "bottom_type"):
+        assert np.array_equal(g[k][sel], o[k][sel]), k
+
+
+def test_regions_band_sets_with_shared_bands_equal_the_shared_call():
+    """Every scene given the same four bands through photic_invert_regions_bands: bit for bit
+    photic_invert_regions."""
+    nbot, ns, n_rows, n_cols, nd = 2, 2, 5, 5, 2
+    rrs, ss, sv = region_scene(55, n_rows, n_cols, nd)
+    cfg = P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=nbot, reqmin=1e-2, konvge=100,
+                        kcount=5000, check_watchdog=1)
+    a = P.invert_regions(cfg, rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns)
+    b = P.invert_regions_bands(cfg, rrs.reshape(nd * 4, -1), [O.WL4] * nd, n_rows, n_cols,
+                               -9999.0, ss, sv, n_spatial=ns)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k

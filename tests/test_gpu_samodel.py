@@ -384,3 +384,70 @@ def test_samodel_watchdog_fails_the_call(nd, n_spatial, monkeypatch):
     monkeypatch.delenv("PHOTIC_WATCHDOG_S")
     rc, out = _call_scenes(bands, (30.0, 40.0)[:nd], (0.0, 0.0)[:nd], 2, n_spatial=n_spatial)
     assert rc == 0 and np.any(out["btype"] != -9999.0)
+
+
+def _call_band_sets(scene_grids, wls, theta_w, theta_v, n_bottoms, n_spatial, nodata=-9999.0):
+    """samodel() over scenes with their own band sets: scene_grids[s] [nb_s][nrows][ncols] at
+    wavelengths wls[s] (integers, as the reference's scene structure holds them)."""
+    nd = len(scene_grids)
+    nrows, ncols = scene_grids[0].shape[1:]
+    flat = [g for sg in scene_grids for g in sg]
+    grids = [RowGrid(np.ascontiguousarray(g, np.float32)) for g in flat]
+    gd = (Geogrid * len(flat))()
+    for q in range(len(flat)):
+        gd[q].nrows, gd[q].ncols, gd[q].nodata_value = nrows, ncols, nodata
+        gd[q].array = grids[q].pp
+    scenes = (Scene * nd)()
+    at = 0
+    for d in range(nd):
+        sc = scenes[d]
+        sc.n_bands, sc.nrows, sc.ncols = len(wls[d]), nrows, ncols
+        for b, wl in enumerate(wls[d]):
+            sc.band_indexes[b] = at
+            sc.wavelengths[b] = int(wl)
+            at += 1
+        sc.theta_w, sc.theta_v = theta_w[d], theta_v[d]
+    idx = (C.c_int * nd)(*range(nd))
+    empg = Geogrid()
+    outs = {k: RowGrid(np.zeros((nrows, ncols))) for k in
+            ("depth", "sigma", "err", "alb", "sand", "grass", "coral", "kmin", "btype", "iod")}
+    P.lib.photic_samodel.restype = C.c_int
+    rc = P.lib.photic_samodel(scenes, gd, idx, nd, 0, empg, 1, n_spatial, n_bottoms,
+                              *[outs[k].pp for k in ("depth", "sigma", "err", "alb", "sand",
+                                                     "grass", "coral", "kmin", "btype", "iod")])
+    return rc, {k: v.array() for k, v in outs.items()}
+
+
+@pytest.mark.parametrize("wls,n_spatial", [(((443, 482, 561, 655), (443, 490, 560, 620, 665)), 1),
+                                           (((412, 443, 490, 531, 555, 670),), 2)])
+def test_samodel_dropin_band_sets(wls, n_spatial):
+    """samodel() with each scene's own band set (a five- beside a four-band scene, or one
+    six-band scene): every pixel through the region kernel, against the oracle's region
+    inversion with per-scene bands, pixel by pixel."""
+    nrows, ncols = 6, 8
+    n = nrows * ncols
+    nd = len(wls)
+    scene_grids = [O.gen(60 + s, n, wl=tuple(float(w) for w in wls[s]))[0]
+                   .reshape(len(wls[s]), nrows, ncols).copy() for s in range(nd)]
+    scene_grids[0][:, 2, 3] = -9999.0    # no data on every band of a scene
+    scene_grids[-1][1, 4, 5] = -0.0004   # negative on one band
+    tw = tuple(30.0 + 8.0 * s for s in range(nd))
+    tv = tuple(3.0 * s for s in range(nd))
+    rc, out = _call_band_sets(scene_grids, wls, tw, tv, 2, n_spatial)
+    assert rc == 0
+    rows = np.concatenate([g.reshape(len(g), n) for g in scene_grids]).astype(np.float32)
+    ss = np.array([O.sec_deg(t) for t in tw])
+    sv = np.array([O.sec_deg(t) for t in tv])
+    o = O.invert_regions_bands(rows, [tuple(float(w) for w in ws) for ws in wls], nrows, ncols,
+                               np.arange(n), -9999.0, ss, sv, n_spatial=n_spatial, n_bottoms=2,
+                               bottom_idx=(0, 1, 2), screen=True)
+    ok = o["status"] >= 0
+    assert not ok[2 * ncols + 3] and not ok[4 * ncols + 5]
+    assert np.array_equal(out["btype"].reshape(-1) != -9999.0, ok)
+    depth = -out["depth"].reshape(-1)
+    same = depth[ok] == o["depth"][ok]
+    err = out["err"].reshape(-1)[ok]
+    flat = np.abs(err - o["rrs_error"][ok]) <= 1e-6 * np.abs(o["rrs_error"][ok])
+    print("band sets %s, n_spatial %d: %d modelled, identical depth %.4f" %
+          ([len(w) for w in wls], n_spatial, ok.sum(), same.mean()))
+    assert np.all(same | flat) and same.mean() >= 0.98

@@ -248,3 +248,53 @@ def test_smoothing_matches_reference(r):
         assert np.array_equal(s, g)
     if r == 2:
         assert s[0, 0] == -9999.0
+
+
+WL5 = (443.0, 490.0, 560.0, 620.0, 665.0)
+
+
+def test_scene_band_sets_match_the_shared_band_path():
+    """Every scene given the same four bands through the per-scene entry gives the shared-band
+    objective bit for bit."""
+    rng = np.random.default_rng(3)
+    px = _pixels(30, 9 * 2).reshape(9, 2, 4).astype(np.float32).astype(np.float64)
+    assert np.all(px > 0.0)
+    for t in range(5):
+        x = _native_x(rng, 2, 2)
+        f1, t1 = O.region_objective(x, px, origin=4, sec_sun=[1.1, 1.2], sec_view=[1.0, 1.05],
+                                    n_bottoms=2)
+        f2, t2 = O.region_objective_bands(x, px.reshape(9, 8), [O.WL4, O.WL4], origin=4,
+                                          sec_sun=[1.1, 1.2], sec_view=[1.0, 1.05], n_bottoms=2)
+        assert np.array_equal([f1], [f2], equal_nan=True)
+        assert np.array_equal(t1, t2, equal_nan=True)
+
+
+@pytest.mark.parametrize("H", [1.5, 3.5, 20.0])
+def test_scene_band_sets_enter_the_misfit_scene_by_scene(H):
+    """Two scenes with their own band sets (four and five bands): the joint misfit is formed
+    from each scene alone. E_Rrs = 100 sqrt(err / n) / (tot / n) over all (scene, band) pairs,
+    so each scene's squared residual sum follows from its own E_Rrs and the joint E_Rrs from
+    their sum over nine pairs; the K penalty is the sum of each scene's own (each over its own
+    bands' K). A scene evaluated with the other's band table or band count fails this."""
+    a = O.gen(41, 1)[0][:, 0].astype(np.float64)
+    b = O.gen(41, 1, wl=WL5)[0][:, 0].astype(np.float64)
+    rng = np.random.default_rng(int(H * 10))
+    for _ in range(4):
+        x = np.concatenate([[H], rng.uniform(5, 60, 1), rng.uniform(0.1, 2.0, 1)])
+        pa = [rng.uniform(0.2, 3.0), rng.uniform(0.1, 2.0), rng.uniform(0.02, 0.6)]
+        pb = [rng.uniform(0.2, 3.0), rng.uniform(0.1, 2.0), rng.uniform(0.02, 0.6)]
+        _, ta = O.region_objective_bands(np.concatenate([x, pa]), a[None], [O.WL4],
+                                         sec_sun=1.1, n_bottoms=1)
+        _, tb = O.region_objective_bands(np.concatenate([x, pb]), b[None], [WL5],
+                                         sec_sun=1.2, sec_view=1.04, n_bottoms=1)
+        _, tab = O.region_objective_bands(np.concatenate([x, pa, pb]),
+                                          np.concatenate([a, b])[None], [O.WL4, WL5],
+                                          sec_sun=[1.1, 1.2], sec_view=[1.0, 1.04], n_bottoms=1)
+
+        def err(e, r):
+            return (e / 100.0 * r.sum() / len(r)) ** 2 * len(r)
+
+        n = 9
+        e_ab = 100.0 * np.sqrt((err(ta[0], a) + err(tb[0], b)) / n) / ((a.sum() + b.sum()) / n)
+        assert tab[0] == pytest.approx(e_ab, rel=1e-12)
+        assert tab[3] == pytest.approx(ta[3] + tb[3], rel=1e-14, abs=1e-300)
