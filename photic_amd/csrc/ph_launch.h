@@ -183,7 +183,9 @@ int ph_launch_gen_stack(const PhConst *c, uint64_t seed, long pix0, long n, int 
 
 /* spatial regions (ph_region.hip) */
 int ph_launch_region(const PhRegionArgs *a, int grid, void *stream);
-int ph_region_occupancy(const PhRegionArgs *a, int *blocks_per_cu);
+/* Blocks per CU of the instance a launch would use: device_blocks selects the form whose blocks
+ * live in device memory (ph_launch_region takes it when a->work is set). */
+int ph_region_occupancy(const PhRegionArgs *a, int device_blocks, int *blocks_per_cu);
 
 /* the ordered inversion (ph_order.hip) */
 int ph_launch_order_key(const PhInvertArgs *a, int noise, const double *nsig, float *key, int *pix,

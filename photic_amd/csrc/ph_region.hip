@@ -1015,12 +1015,11 @@ static int ph_region_dispatch_mc(int mc, bool gm, Fn&& fn)
 }
 
 // The instance for a launch: coordinate slots per lane from n_max (64 coordinates a slot), the
-// block in device memory when a->work is set.
+// block in device memory when gm is set.
 template <typename Fn>
-static int ph_region_dispatch(const PhRegionArgs* a, Fn&& fn)
+static int ph_region_dispatch(const PhRegionArgs* a, bool gm, Fn&& fn)
 {
     const int mc = (a->n_max + 63) / 64;
-    const bool gm = a->work != nullptr;
     switch (a->c.n_bottoms) {
     case 1: return ph_region_dispatch_mc<1>(mc, gm, fn);
     case 2: return ph_region_dispatch_mc<2>(mc, gm, fn);
@@ -1031,15 +1030,15 @@ static int ph_region_dispatch(const PhRegionArgs* a, Fn&& fn)
 
 extern "C" int ph_launch_region(const PhRegionArgs* a, int grid, void* stream)
 {
-    return ph_region_dispatch(a, [&](auto nb, auto m, auto gm) {
+    return ph_region_dispatch(a, a->work != nullptr, [&](auto nb, auto m, auto gm) {
         return ph_region_launch_t<decltype(nb)::value, decltype(m)::value, decltype(gm)::value>(
             a, grid, stream);
     });
 }
 
-extern "C" int ph_region_occupancy(const PhRegionArgs* a, int* blocks_per_cu)
+extern "C" int ph_region_occupancy(const PhRegionArgs* a, int device_blocks, int* blocks_per_cu)
 {
-    return ph_region_dispatch(a, [&](auto nb, auto m, auto gm) {
+    return ph_region_dispatch(a, device_blocks != 0, [&](auto nb, auto m, auto gm) {
         return ph_region_occ_t<decltype(nb)::value, decltype(m)::value, decltype(gm)::value>(
             a, blocks_per_cu);
     });

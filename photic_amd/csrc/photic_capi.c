@@ -861,11 +861,7 @@ int ph_invert_regions_bands(const photic_cfg *cfg, int n_rows, int n_cols, int n
     a.max_ticks = (unsigned long long)(limit_s * 1000.0 * (double)wrate);
     rc = ph_n_cu(&n_cu);
     if (rc) return rc;
-    /* occupancy of the chosen instance: a non-NULL work pointer selects the device-memory one
-     * (the real pointer is set below) */
-    a.work = in_device ? (double *)&a : NULL;
-    rc = ph_launch_status(ph_region_occupancy(&a, &bpc), "occupancy");
-    a.work = NULL;
+    rc = ph_launch_status(ph_region_occupancy(&a, in_device, &bpc), "occupancy");
     if (rc) return rc;
     if (bpc < 1) return ph_fail(PHOTIC_EUNSUPPORTED, "region kernel does not fit a CU");
     long grid = cfg->grid > 0 ? cfg->grid : (long)n_cu * bpc;

@@ -496,11 +496,6 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
     if (!scene_data || !gridded_data || !scene_indexes || nscenes < 1) return PHOTIC_EINVAL;
     const struct photic_scene *sc = &scene_data[scene_indexes[0]];
     const int nrows = sc->nrows, ncols = sc->ncols, nb = sc->n_bands;
-    if (nscenes > SM_MAX_SCENES) {
-        fprintf(stderr, "photic samodel: at most %d scenes (got %d); outputs left unset\n",
-                SM_MAX_SCENES, nscenes);
-        return PHOTIC_EUNSUPPORTED;
-    }
 
     /* the reference's defaults for every pixel (model/samodel.c:819-841) */
     for (int i = 0; i < nrows; i++)
@@ -516,6 +511,11 @@ int photic_samodel(struct photic_scene scene_data[], struct photic_geogrid gridd
             bottom_type[i][j] = -9999.0f;
             index_optical_depth[i][j] = 0.0f;
         }
+    if (nscenes > SM_MAX_SCENES) {
+        fprintf(stderr, "photic samodel: at most %d scenes (got %d); outputs left at their "
+                        "defaults\n", SM_MAX_SCENES, nscenes);
+        return PHOTIC_EUNSUPPORTED;
+    }
     /* each scene's band set (model/samodel.c:384-527): 2..8 bands at ascending wavelengths,
      * every scene on the first one's grid */
     sm_bands bd;
