@@ -403,6 +403,10 @@ int photic_kernel_info(const photic_cfg *cfg, int *vgprs, int *scratch_bytes,
 int photic_device_count(void);
 int photic_device_malloc(void **ptr, size_t bytes);
 int photic_device_free(void *ptr);
+/* Page-locked host memory: device-to-host copies into it run at the link's rate instead
+ * of going through the runtime's staging buffer. */
+int photic_host_malloc(void **ptr, size_t bytes);
+int photic_host_free(void *ptr);
 int photic_memcpy_htod(void *dst, const void *src, size_t bytes);
 int photic_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int photic_memset_zero(void *dst, size_t bytes);

@@ -114,6 +114,8 @@ def _load():
         "photic_device_count": (i, []),
         "photic_device_malloc": (i, [C.POINTER(vp), C.c_size_t]),
         "photic_device_free": (i, [vp]),
+        "photic_host_malloc": (i, [C.POINTER(vp), C.c_size_t]),
+        "photic_host_free": (i, [vp]),
         "photic_memcpy_htod": (i, [vp, vp, C.c_size_t]),
         "photic_memcpy_dtoh": (i, [vp, vp, C.c_size_t]),
         "photic_memset_zero": (i, [vp, C.c_size_t]),
@@ -232,7 +234,13 @@ class DeviceBuffer:
         return buf
 
     def to_array(self, dtype, shape):
-        out = np.empty(shape, dtype=dtype)
+        return self.copy_to(np.empty(shape, dtype=dtype))
+
+    def copy_to(self, out):
+        """Copy the buffer's first out.nbytes bytes into the C-contiguous host array out."""
+        if not out.flags.c_contiguous or out.nbytes > self.nbytes:
+            raise ValueError("copy_to needs a C-contiguous array of at most %d bytes"
+                             % self.nbytes)
         if out.nbytes:
             check(lib.photic_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes))
         return out
@@ -251,6 +259,37 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class _HostMemory:
+    """One page-locked host allocation; frees itself."""
+
+    def __init__(self, nbytes):
+        self.ptr = C.c_void_p()
+        check(lib.photic_host_malloc(C.byref(self.ptr), max(nbytes, 1)))
+
+    def __del__(self):
+        try:
+            if self.ptr is not None and self.ptr.value:
+                lib.photic_host_free(self.ptr)
+            self.ptr = None
+        except Exception:
+            pass
+
+
+class PinnedBuffer:
+    """Page-locked host memory seen as a numpy array (.array); device-to-host copies into it
+    skip the runtime's staging copy. The memory is freed once no array taken from it is
+    left."""
+
+    def __init__(self, dtype, shape):
+        dtype = np.dtype(dtype)
+        shape = tuple(int(v) for v in np.atleast_1d(shape))
+        nbytes = dtype.itemsize * int(np.prod(shape))
+        mem = _HostMemory(nbytes)
+        raw = (C.c_char * max(nbytes, 1)).from_address(mem.ptr.value)
+        raw._owner = mem  # every array viewing raw keeps the allocation alive
+        self.array = np.frombuffer(raw, dtype=np.uint8, count=nbytes).view(dtype).reshape(shape)
 
 
 _OUT_FIELDS = {
@@ -281,11 +320,27 @@ class DeviceOutputs:
             self.bufs[name] = (b, dt, rows)
             setattr(self.struct, name, b.value)
 
-    def fetch(self):
+    def _shape(self, rows):
+        return (self.n,) if rows == 1 else (rows, self.n)
+
+    def pinned(self):
+        """Page-locked host arrays for every field, shaped as fetch() returns them."""
+        return {name: PinnedBuffer(dt, self._shape(rows)).array
+                for name, (b, dt, rows) in self.bufs.items()}
+
+    def fetch(self, into=None):
+        """The fields as host arrays: new ones, or copied into the arrays of `into` (a dict
+        such as pinned() returns)."""
         out = {}
         for name, (b, dt, rows) in self.bufs.items():
-            shape = (self.n,) if rows == 1 else (rows, self.n)
-            out[name] = b.to_array(dt, shape)
+            if into is None:
+                out[name] = b.to_array(dt, self._shape(rows))
+            else:
+                dst = into[name]
+                if dst.dtype != np.dtype(dt) or dst.shape != self._shape(rows):
+                    raise ValueError("fetch into %s: need %s %s" % (name, np.dtype(dt),
+                                                                    self._shape(rows)))
+                out[name] = b.copy_to(dst)
         return out
 
 

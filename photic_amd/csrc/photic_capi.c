@@ -1527,6 +1527,15 @@ int photic_device_malloc(void **ptr, size_t bytes)
 
 int photic_device_free(void *ptr) { return ph_hip(hipFree(ptr), "hipFree"); }
 
+int photic_host_malloc(void **ptr, size_t bytes)
+{
+    int rc = ph_have_device();
+    if (rc) return rc;
+    return ph_hip(hipHostMalloc(ptr, bytes ? bytes : 1, 0), "hipHostMalloc");
+}
+
+int photic_host_free(void *ptr) { return ph_hip(hipHostFree(ptr), "hipHostFree"); }
+
 int photic_memcpy_htod(void *dst, const void *src, size_t bytes)
 {
     return ph_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");

@@ -33,6 +33,13 @@ def test_scene_scale_parity():
     P.invert_device(cfg, n, d_rrs.value, d_ss.value, None, outs)
     P.check(P.lib.photic_synchronize())
     g = outs.fetch()
+    # the same outputs copied into page-locked host arrays
+    pin = outs.pinned()
+    gp = outs.fetch(into=pin)
+    for k in g:
+        assert gp[k] is pin[k]
+        assert np.array_equal(gp[k].view(np.uint8), g[k].view(np.uint8)), k
+    del gp, pin
     rrs = d_rrs.to_array(np.float32, (4, n))
     ss = d_ss.to_array(np.float32, (n,))
     masked = np.all(rrs == -9999.0, axis=0)
