@@ -69,7 +69,8 @@ __global__ void __launch_bounds__(256, ph_invert_waves<NBOT>()) ph_invert_kernel
         }
         // ---- refill idle lanes ----
         unsigned long long idle = __ballot(S.phase == PH_IDLE);
-        if (idle != 0ull && !exhausted) {
+        const int n_idle = __popcll(idle);
+        if (idle != 0ull && !exhausted && (n_idle >= a.refill_min || n_idle == 64)) {
             if (q_next >= q_end) {
                 unsigned long long base = 0;
                 if (lane == __ffsll((long long)idle) - 1)

@@ -60,6 +60,7 @@ typedef struct PhInvertArgs {
     const float *hstart;   /* [n_pix]: one depth start per pixel (c.n_hstarts == 1), or NULL */
     unsigned long long *queue; /* [0] work counter, [1] watchdog flag; zero at launch */
     int chunk;
+    int refill_min; /* idle lanes a wave gathers before it sets up new pixels (<= 1: any) */
     unsigned long long max_ticks; /* watchdog: wall-clock ticks a wave may go without progress */
     float *depth, *rrs_error, *model_error, *albedo, *k_min, *iod;
     float *b_pct; /* [3][n_pix]: sand, seagrass, coral percent */

@@ -373,6 +373,9 @@ static int ph_invert_single(const photic_cfg *cfg, long n_pix, const float *rrs,
     a.seed = seed;
     a.hstart = hstart;
     a.chunk = cfg->chunk > 0 ? cfg->chunk : (tu->chunk > 0 ? tu->chunk : 64);
+    /* a wave sets up new pixels once two lanes are idle: the set-up (five correctly rounded
+     * pows) then runs for two pixels at a time (+1% at 4096^2, seeded and plain alike) */
+    a.refill_min = 2;
     a.depth = out->depth;
     a.rrs_error = out->rrs_error;
     a.model_error = out->model_error;
