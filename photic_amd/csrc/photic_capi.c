@@ -881,6 +881,8 @@ int ph_invert_regions_bands(const photic_cfg *cfg, int n_rows, int n_cols, int n
     rc = ph_hip(hipMallocAsync((void **)&buf, bytes, st), "hipMallocAsync");
     if (rc) return rc;
     unsigned long long *queue = (unsigned long long *)buf;
+    /* scn is pageable (this frame's array): HIP copies from pageable memory synchronously,
+     * so the array may go out of scope as soon as the call returns */
     rc = ph_hip(hipMemcpyAsync(buf + scn_off, scn, sizeof(double) * scn_n,
                                hipMemcpyHostToDevice, st),
                 "per-call tables");
