@@ -646,6 +646,9 @@ PH_INLINE void ph_nm_begin(PhNM<N>& S, const PhConst& c)
 {
     S.icount = 0;
     S.numres = 0;
+    S.ihi = 0;
+    S.ylo = 0.0;
+    S.yr = 0.0;
     S.jcount = S.kstart < 0 ? c.seed_konvge : c.konvge;
     S.del = 1.0;
     S.phase = PH_BUILD;
@@ -685,63 +688,35 @@ PH_INLINE void ph_nm_consume(PhNM<N>& S, const PhConst& c, double f)
     if (ph < PH_BUILD || ph > PH_FACT_MINUS) return;
     S.icount++;
 
-    // ---- classify (asa047.c:176-393, 455-479) ----
-    bool wr = false, wr_xs = false;   // vertex write: vertex wj = wr_xs ? xs : xe, value wf
-    int wj = 0;
-    double wf = f;
-    int act = A_NONE;
-    bool mv_phi = false;              // A_MOVE: src is vertex ihi (else xs)
-    double mv_coef = 0.5;
-    int mv_phase = PH_EXPAND;
-    if (ph == PH_BUILD) {
-        wr = true;
-        wj = S.sub;
-        const int next = (S.sub == N) ? 0 : S.sub + 1;
-        S.sub = next;
-        act = (next < N) ? A_BUILD_NEXT : A_AFTER_BUILD;
-    } else if (ph == PH_REFLECT) {
-        S.ystar = f;
-        if (f < S.ylo) {
-            act = A_MOVE; mv_coef = 2.0; mv_phase = PH_EXPAND;
-        } else {
-            int l = 0;
-            ph_for<N + 1>([&](auto i) PH_BODY { l += (f < S.y[i]) ? 1 : 0; });
-            if (l > 1) {
-                wr = true; wj = S.ihi; wr_xs = true; wf = f;
-                act = A_ITER_END;
-            } else if (l == 0) {
-                act = A_MOVE; mv_phi = true; mv_phase = PH_CONTRACT_IN;
-            } else {
-                act = A_MOVE; mv_phase = PH_CONTRACT_OUT;
-            }
-        }
-    } else if (ph == PH_EXPAND) {
-        wr = true; wj = S.ihi;
-        wr_xs = S.ystar < f;
-        wf = ph_pick(wr_xs, S.ystar, f);
-        act = A_ITER_END;
-    } else if (ph == PH_CONTRACT_IN) {
-        if (ph_sel(S.y, S.ihi) < f) {
-            S.sub = 0;
-            act = A_SHRINK_NEXT;
-        } else {
-            wr = true; wj = S.ihi;
-            act = A_ITER_END;
-        }
-    } else if (ph == PH_CONTRACT_OUT) {
-        wr = true; wj = S.ihi;
-        wr_xs = !(f <= S.ystar);
-        wf = ph_pick(wr_xs, S.ystar, f);
-        act = A_ITER_END;
-    } else if (ph == PH_SHRINK) {
-        wr = true; wj = S.sub;
-        S.sub = S.sub + 1;
-        act = (S.sub <= N) ? A_SHRINK_NEXT : A_AFTER_BUILD;
-    } else if (ph == PH_FACT_PLUS) {
-        act = (ph_trunc10(f) < S.yr) ? A_FACT_FAIL : A_FACT_MINUS;
-    } else { // PH_FACT_MINUS
-        act = (ph_trunc10(f) < S.yr) ? A_FACT_FAIL : A_FACT_NEXT;
-    }
+    // ---- classify (asa047.c:176-393, 455-479), as selects: lanes of a wave sit in every
+    // phase at once, so branches would all run anyway ----
+    const bool bB = ph == PH_BUILD, bR = ph == PH_REFLECT, bE = ph == PH_EXPAND;
+    const bool bCI = ph == PH_CONTRACT_IN, bCO = ph == PH_CONTRACT_OUT, bS = ph == PH_SHRINK;
+    const bool bF = ph >= PH_FACT_PLUS; // PH_FACT_PLUS or PH_FACT_MINUS
+    int l = 0;
+    ph_for<N + 1>([&](auto i) PH_BODY { l += (f < S.y[i]) ? 1 : 0; });
+    const bool r_lo = f < S.ylo;
+    const bool ci_shrink = ph_sel(S.y, S.ihi) < f;
+    const bool fact_fail = ph_trunc10(f) < S.yr;
+    const int bnext = (S.sub == N) ? 0 : S.sub + 1;
+    const int snext = S.sub + 1;
+    if (bR) S.ystar = f;
+    const bool wr = bB || (bR && !r_lo && l > 1) || bE || (bCI && !ci_shrink) || bCO || bS;
+    const int wj = (bB || bS) ? S.sub : S.ihi;
+    const bool wr_xs = bR || (bE && S.ystar < f) || (bCO && !(f <= S.ystar));
+    const double wf = ph_pick(wr_xs, S.ystar, f);
+    int act = A_ITER_END; // bR (l > 1), bE, bCO, bCI without a shrink
+    if (bB) act = bnext < N ? A_BUILD_NEXT : A_AFTER_BUILD;
+    if (bR && (r_lo || l <= 1)) act = A_MOVE;
+    if (bCI && ci_shrink) act = A_SHRINK_NEXT;
+    if (bS) act = snext <= N ? A_SHRINK_NEXT : A_AFTER_BUILD;
+    if (bF) act = fact_fail ? A_FACT_FAIL : (ph == PH_FACT_PLUS ? A_FACT_MINUS : A_FACT_NEXT);
+    const bool mv_phi = !r_lo && l == 0;
+    const double mv_coef = r_lo ? 2.0 : 0.5;
+    const int mv_phase = r_lo ? PH_EXPAND : (l == 0 ? PH_CONTRACT_IN : PH_CONTRACT_OUT);
+    if (bB) S.sub = bnext;
+    if (bCI && ci_shrink) S.sub = 0;
+    if (bS) S.sub = snext;
 
     // ---- the vertex write ----
     if (wr) {
