@@ -647,6 +647,7 @@ PH_INLINE void ph_nm_begin(PhNM<N>& S, const PhConst& c)
     S.icount = 0;
     S.numres = 0;
     S.ihi = 0;
+    S.ilo = 0;
     S.ylo = 0.0;
     S.yr = 0.0;
     S.jcount = S.kstart < 0 ? c.seed_konvge : c.konvge;
@@ -724,64 +725,47 @@ PH_INLINE void ph_nm_consume(PhNM<N>& S, const PhConst& c, double f)
         ph_for<N + 1>([&](auto k) PH_BODY { S.y[k] = ph_pick(wj == (int)k, wf, S.y[k]); });
     }
 
-    // ---- follow-up ----
-    if (act == A_BUILD_NEXT) {
-        const int j = S.sub;
-        ph_for<N>([&](auto i) PH_BODY {
-            S.xe[i] = (j == (int)i) ? S.xs[i] + S.m.step(i) * S.del : +S.xs[i];
-        });
-        S.phase = PH_BUILD;
-        return;
-    }
-    if (act == A_MOVE) {
-        ph_for<N>([&](auto i) PH_BODY {
-            const double v = mv_phi ? S.m.p(S.ihi, i) : +S.xs[i];
-            S.xe[i] = S.xb[i] + mv_coef * (v - S.xb[i]);
-        });
-        S.phase = mv_phase;
-        return;
-    }
-    if (act == A_SHRINK_NEXT) {
-        const int j = S.sub, lo = S.ilo;
-        ph_for<N>([&](auto i) PH_BODY { S.xe[i] = (S.m.p(j, i) + S.m.p(lo, i)) * 0.5; });
-        S.phase = PH_SHRINK;
-        return;
-    }
-    if (act == A_FACT_FAIL) {
-        // asa047.c:488-493: restart from the probe point with a simplex of relative size
-        // eps; jcount and icount carry over
-        S.ifault = 2;
-        ph_for<N>([&](auto i) PH_BODY {
-            S.xs[i] = S.xb[i];
-            S.xe[i] = S.xs[i];
-        });
-        S.del = eps;
-        S.numres++;
-        S.phase = PH_BUILD;
-        S.sub = N;
-        return;
-    }
-    if (act == A_FACT_MINUS || act == A_FACT_NEXT) {
+    // ---- follow-ups that only set the next point (asa047.c:174-194, 263-393, 455-493): one
+    // read of at most two rows of the lane's arrays (row N + 1 is the steps), the formulas as
+    // selects ----
+    const bool fBN = act == A_BUILD_NEXT, fMV = act == A_MOVE, fSN = act == A_SHRINK_NEXT;
+    const bool fFF = act == A_FACT_FAIL, fFM = act == A_FACT_MINUS, fFN = act == A_FACT_NEXT;
+    if (fBN || fMV || fSN || fFF || fFM || fFN) {
+        // BUILD_NEXT: vertex sub = start + step * del on coordinate sub; MOVE: xb + coef (v -
+        // xb), v the worst vertex or the reflection; SHRINK_NEXT: (vertex sub + best vertex) /
+        // 2; FACT_FAIL: restart from the probe point with a simplex of relative size eps
+        // (jcount and icount carry over); FACT_MINUS / FACT_NEXT: the probe point moved by
+        // step * eps on coordinates i0 and i1
+        const int ra = (fBN || fFM || fFN) ? N + 1 : (fSN ? S.sub : S.ihi);
+        const int rb = S.ilo;
         const int i0 = S.sub;
-        const bool minus = (act == A_FACT_MINUS);
-        const int i1 = minus ? -1 : i0 + 1;
+        const int i1 = fFM ? -1 : i0 + 1;
         ph_for<N>([&](auto i) PH_BODY {
-            const double d = S.m.step(i) * eps;
-            double v = S.xb[i];
-            if (i0 == (int)i) v = minus ? v - d - d : v + d;
-            if (i1 == (int)i) v = v + d;
-            S.xb[i] = v;
-            S.xe[i] = v;
+            const double a = S.m.p(ra, i), b = S.m.p(rb, i);
+            const double xs = S.xs[i], xb = S.xb[i];
+            const double bn = (i0 == (int)i) ? xs + a * S.del : +xs;
+            const double v = mv_phi ? a : xs;
+            const double mv = xb + mv_coef * (v - xb);
+            const double sn = (a + b) * 0.5;
+            const double d = a * eps;
+            double fv = xb;
+            if (i0 == (int)i) fv = fFM ? fv - d - d : fv + d;
+            if (i1 == (int)i) fv = fv + d;
+            S.xe[i] = fBN ? bn : (fMV ? mv : (fSN ? sn : (fFF ? xb : fv)));
+            if (fFF) S.xs[i] = xb;
+            if (fFM || fFN) S.xb[i] = fv;
         });
-        if (minus) {
-            S.phase = PH_FACT_MINUS;
-            return;
+        const bool run_done = fFN && i1 >= N;
+        int nph = fBN || fFF ? (int)PH_BUILD : (fMV ? mv_phase : (fSN ? (int)PH_SHRINK : (fFM ? (int)PH_FACT_MINUS : (int)PH_FACT_PLUS)));
+        if (!run_done) S.phase = nph;
+        if (fFF) {
+            S.ifault = 2;
+            S.del = eps;
+            S.numres++;
+            S.sub = N;
         }
-        if (i1 < N) {
-            S.sub = i1;
-            S.phase = PH_FACT_PLUS;
-            return;
-        }
+        if (fFN && !run_done) S.sub = i1;
+        if (!run_done) return;
         S.ifault = 0;
         act = A_RUN_DONE;
     }
