@@ -24,6 +24,8 @@
 
 typedef short ph_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float ph_f32x4 __attribute__((ext_vector_type(4)));
+typedef float ph_f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 ph_bf16x2 __attribute__((ext_vector_type(2)));
 
 #define MLP_IN 5
 #define MLP_HID 128
@@ -64,23 +66,30 @@ __device__ __forceinline__ float ph_bf2f(unsigned short h)
     return __uint_as_float((uint32_t)h << 16);
 }
 
+// Four floats rounded to BF16 (to nearest, ties to even, as ph_f2bf for finite inputs), two
+// at a time by the conversion instruction gfx950 has for it.
 __device__ __forceinline__ ph_bf16x4 ph_pack4(float a, float b, float c, float d)
 {
+    const ph_bf16x2 lo = __builtin_convertvector((ph_f32x2){a, b}, ph_bf16x2);
+    const ph_bf16x2 hi = __builtin_convertvector((ph_f32x2){c, d}, ph_bf16x2);
+    const unsigned int ul = __builtin_bit_cast(unsigned int, lo);
+    const unsigned int uh = __builtin_bit_cast(unsigned int, hi);
     ph_bf16x4 v;
-    v[0] = (short)ph_f2bf(a);
-    v[1] = (short)ph_f2bf(b);
-    v[2] = (short)ph_f2bf(c);
-    v[3] = (short)ph_f2bf(d);
+    v[0] = (short)(ul & 0xffffu);
+    v[1] = (short)(ul >> 16);
+    v[2] = (short)(uh & 0xffffu);
+    v[3] = (short)(uh >> 16);
     return v;
 }
 
-// GELU, tanh form, and its derivative
+// GELU, tanh form, and its derivative.  tanh u = 1 - 2 / (e^2u + 1) with the hardware
+// reciprocal (about 1 ulp): the activations are rounded to BF16 next, 2^-8 relative.
 __device__ __forceinline__ void ph_gelu(float z, float* g, float* dg)
 {
     const float c = 0.7978845608028654f, a = 0.044715f;
     float u = c * fmaf(a * z * z, z, z);
     float e = __expf(2.0f * u);
-    float t = 1.0f - 2.0f / (e + 1.0f);
+    float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
     *g = 0.5f * z * (1.0f + t);
     *dg = 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * fmaf(3.0f * a, z * z, 1.0f);
 }
