@@ -225,7 +225,9 @@ int ph_launch_mlp_adam(float *w, float *m, float *v, const float *g, float lr, f
 int ph_launch_mlp_fwdbwd(const unsigned short *pk, const float *pb, const float *x,
                          const float *t, float *y, long n, float gscale, unsigned short *gbuf,
                          float *loss, int grid, void *stream);
-int ph_launch_mlp_wgrad(const unsigned short *gbuf, long n, float *grad, void *stream);
+#define PH_MLP_KSPLIT 16 /* batch slices of the weight-gradient kernels */
+int ph_launch_mlp_wgrad(const unsigned short *gbuf, long n, float *grad, float *part,
+                        void *stream);
 int ph_launch_mlp_trainset(const PhConst *c, uint64_t seed, long pix0, long n, double sigma,
                            float *x, float *t, void *stream);
 int ph_launch_mlp_prep(const float *rrs, const float *ss, float ss0, long n, float *x,

@@ -328,21 +328,29 @@ __global__ void __launch_bounds__(256) ph_mlp_fwdbwd_kernel(
     }
 }
 
-// dW (rows x cols) = dZ . H^T over the batch for one layer, one 16x16 tile per wave;
-// bias gradient from the waves of the first column tile.  dZ: [rows_pad][n], H: [cols_pad][n]
-// bf16.  Writes FP32 gradients into the master layout: W at w_off (row-major, cols valid
-// columns), b at b_off.
+// dW (rows x cols) = dZ . H^T over the batch for one layer: one 16x16 tile per wave and one
+// of PH_MLP_KSPLIT slices of the batch per blockIdx.y, the slice's partial sums written to
+// its own copy of the gradient in the master layout (part + slice * nparam; the bias from the
+// waves of the first column tile).  ph_mlp_gsum_kernel then adds the slices in order, so the
+// result does not depend on scheduling.  dZ: [rows_pad][n], H: [cols_pad][n] bf16.
 __global__ void __launch_bounds__(64) ph_mlp_wgrad_kernel(
     const unsigned short* __restrict__ dZ, const unsigned short* __restrict__ H, long n,
-    int rows, int cols, int col_tiles, float* __restrict__ grad, int w_off, int b_off)
+    int rows, int cols, int col_tiles, float* __restrict__ part, int nparam, int w_off,
+    int b_off)
 {
     const int lane = threadIdx.x, g = lane >> 4;
     const int tr = blockIdx.x / col_tiles, tc = blockIdx.x % col_tiles;
+    // this slice's samples: [k_lo, k_hi), whole 16-sample steps
+    const long steps = (n + 15) / 16;
+    const long per = (steps + PH_MLP_KSPLIT - 1) / PH_MLP_KSPLIT;
+    const long k_lo = 16 * per * (long)blockIdx.y;
+    const long k_hi = k_lo + 16 * per < n ? k_lo + 16 * per : n;
+    float* grad = part + (long)blockIdx.y * nparam;
     ph_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     float bsum = 0.0f;
     const unsigned short* zr = dZ + (long)(16 * tr + (lane & 15)) * n;
     const unsigned short* hr = H + (long)(16 * tc + (lane & 15)) * n;
-    for (long k0 = 0; k0 < n; k0 += 16) {
+    for (long k0 = k_lo; k0 < k_hi; k0 += 16) {
         const long k = k0 + 4 * g;
         ph_bf16x4 a, b;
         if (k + 3 < n) {
@@ -376,7 +384,18 @@ __global__ void __launch_bounds__(64) ph_mlp_wgrad_kernel(
     }
 }
 
-// Adam on the FP32 master weights, then pack the BF16 weights and FP32 biases.
+// grad[i] = the slices' partial sums added in slice order.
+__global__ void ph_mlp_gsum_kernel(const float* __restrict__ part, int nparam,
+                                   float* __restrict__ grad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nparam) return;
+    float s = part[i];
+#pragma unroll
+    for (int k = 1; k < PH_MLP_KSPLIT; k++) s += part[(long)k * nparam + i];
+    grad[i] = s;
+}
+
 __global__ void ph_mlp_adam_kernel(float* __restrict__ w, float* __restrict__ m,
                                    float* __restrict__ v, const float* __restrict__ g, float lr,
                                    float b1, float b2, float eps, float c1, float c2,
@@ -563,7 +582,8 @@ extern "C" int ph_launch_mlp_fwdbwd(const unsigned short* pk, const float* pb, c
     return (int)hipGetLastError();
 }
 
-extern "C" int ph_launch_mlp_wgrad(const unsigned short* gbuf, long n, float* grad, void* stream)
+extern "C" int ph_launch_mlp_wgrad(const unsigned short* gbuf, long n, float* grad, float* part,
+                                   void* stream)
 {
     const unsigned short* gH1 = gbuf;
     const unsigned short* gZ1 = gH1 + 16 * n;
@@ -574,10 +594,13 @@ extern "C" int ph_launch_mlp_wgrad(const unsigned short* gbuf, long n, float* gr
     const unsigned short* gH4 = gZ3 + 128 * n;
     const unsigned short* gZ4 = gH4 + 128 * n;
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(ph_mlp_wgrad_kernel, dim3(8 * 1), dim3(64), 0, st, gZ1, gH1, n, 128, 5, 1, grad, MO_W1, MO_B1);
-    hipLaunchKernelGGL(ph_mlp_wgrad_kernel, dim3(8 * 8), dim3(64), 0, st, gZ2, gH2, n, 128, 128, 8, grad, MO_W2, MO_B2);
-    hipLaunchKernelGGL(ph_mlp_wgrad_kernel, dim3(8 * 8), dim3(64), 0, st, gZ3, gH3, n, 128, 128, 8, grad, MO_W3, MO_B3);
-    hipLaunchKernelGGL(ph_mlp_wgrad_kernel, dim3(1 * 8), dim3(64), 0, st, gZ4, gH4, n, 5, 128, 8, grad, MO_W4, MO_B4);
+    const int np = ph_mlp_nparam();
+    const int ks = PH_MLP_KSPLIT;
+    hipLaunchKernelGGL(ph_mlp_wgrad_kernel, dim3(8 * 1, ks), dim3(64), 0, st, gZ1, gH1, n, 128, 5, 1, part, np, MO_W1, MO_B1);
+    hipLaunchKernelGGL(ph_mlp_wgrad_kernel, dim3(8 * 8, ks), dim3(64), 0, st, gZ2, gH2, n, 128, 128, 8, part, np, MO_W2, MO_B2);
+    hipLaunchKernelGGL(ph_mlp_wgrad_kernel, dim3(8 * 8, ks), dim3(64), 0, st, gZ3, gH3, n, 128, 128, 8, part, np, MO_W3, MO_B3);
+    hipLaunchKernelGGL(ph_mlp_wgrad_kernel, dim3(1 * 8, ks), dim3(64), 0, st, gZ4, gH4, n, 5, 128, 8, part, np, MO_W4, MO_B4);
+    hipLaunchKernelGGL(ph_mlp_gsum_kernel, dim3((np + 255) / 256), dim3(256), 0, st, part, np, grad);
     return (int)hipGetLastError();
 }
 

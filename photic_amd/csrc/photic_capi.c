@@ -1124,6 +1124,7 @@ int photic_objective(const photic_cfg *cfg, long n, const double *x, const float
 
 struct photic_mlp {
     float *w, *m, *v, *g; /* FP32 masters, Adam moments, gradient */
+    float *part;          /* the gradient's partial sums, one copy per batch slice */
     unsigned short *pk;   /* packed BF16 weights */
     float *pb;            /* packed FP32 biases */
     float *loss;
@@ -1153,6 +1154,7 @@ void photic_mlp_destroy(photic_mlp *m)
     hipFree(m->m);
     hipFree(m->v);
     hipFree(m->g);
+    hipFree(m->part);
     hipFree(m->pk);
     hipFree(m->pb);
     hipFree(m->loss);
@@ -1172,6 +1174,8 @@ int photic_mlp_create(photic_mlp **out, uint64_t init_seed)
     if (!rc) rc = ph_hip(hipMalloc((void **)&m->m, sizeof(float) * np), "hipMalloc");
     if (!rc) rc = ph_hip(hipMalloc((void **)&m->v, sizeof(float) * np), "hipMalloc");
     if (!rc) rc = ph_hip(hipMalloc((void **)&m->g, sizeof(float) * np), "hipMalloc");
+    if (!rc)
+        rc = ph_hip(hipMalloc((void **)&m->part, sizeof(float) * np * PH_MLP_KSPLIT), "hipMalloc");
     if (!rc) rc = ph_hip(hipMalloc((void **)&m->pk, sizeof(unsigned short) * nbf), "hipMalloc");
     if (!rc) rc = ph_hip(hipMalloc((void **)&m->pb, sizeof(float) * nf), "hipMalloc");
     if (!rc) rc = ph_hip(hipMalloc((void **)&m->loss, sizeof(float)), "hipMalloc");
@@ -1227,7 +1231,9 @@ static int ph_mlp_grad_step(photic_mlp *m, const float *x, const float *t, long 
     float gscale = 2.0f / ((float)n * 5.0f);
     int rc = ph_launch_status(ph_launch_mlp_fwdbwd(m->pk, m->pb, x, t, NULL, n, gscale, gbuf,
                                                    m->loss, 0, stream), "network forward/backward");
-    if (!rc) rc = ph_launch_status(ph_launch_mlp_wgrad(gbuf, n, m->g, stream), "weight gradients");
+    if (!rc)
+        rc = ph_launch_status(ph_launch_mlp_wgrad(gbuf, n, m->g, m->part, stream),
+                              "weight gradients");
     return rc;
 }
 
