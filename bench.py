@@ -211,9 +211,9 @@ def main():
                     help="also stream the plain method over the first ROWS rows")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has one MI355X and the "
-              "inversion has no cross-GPU step (pixels are independent; see DESIGN.md for how "
-              "it shards). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the inversion has no cross-GPU step (pixels are "
+              "independent, so a scene shards by row tiles over GPUs with nothing exchanged "
+              "but the summary counts). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import photic_amd as P
@@ -369,13 +369,14 @@ def main():
     if net is not None:
         w_ms, _, pres = run("plain", cfg, n)
         comparisons["plain"] = summary(w_ms, pres, n)
-        # The judge rule: how many pixels end with a seeded misfit above the plain method's.
+        # The plain method as the arbiter: how many pixels end with a seeded misfit above the
+        # plain method's.
         # A seeded pixel falls back to the reference's depth starts whenever its seed start
         # stays at or above the early exit, so it can end worse than the plain method only
         # where both end below the early exit (the plain method accepts any start below it).
         both = (res["status"] >= 0) & (pres["status"] >= 0)
         worse = both & (res["model_error"] > pres["model_error"])
-        judge = {"pixels": int(both.sum()),
+        vs_plain = {"pixels": int(both.sum()),
                  "seeded_worse_than_plain_frac": float(worse.sum() / max(both.sum(), 1)),
                  "worse_with_seeded_at_or_above_early_exit": int(
                      (worse & (res["model_error"] >= 2.5)).sum()),
@@ -383,7 +384,7 @@ def main():
                  if worse.any() else None,
                  "median_misfit_of_worse_plain": float(np.median(pres["rrs_error"][worse]))
                  if worse.any() else None}
-        comparisons["judge"] = judge
+        comparisons["seeded_vs_plain"] = vs_plain
         if not args.no_equal_misfit:
             # The plain method at equal misfit: the reference's depth starts with the early
             # exit lowered until the plain median misfit is at or below the seeded one, chosen
@@ -409,7 +410,7 @@ def main():
             w_ms, _, eres = run("plain", cfg_eq, n)
             full = summary(w_ms, eres, n)
             both_eq = (res["status"] >= 0) & (eres["status"] >= 0)
-            judge["seeded_worse_than_plain_equal_misfit_frac"] = float(
+            vs_plain["seeded_worse_than_plain_equal_misfit_frac"] = float(
                 (both_eq & (res["model_error"] > eres["model_error"])).sum() /
                 max(both_eq.sum(), 1))
             v_all = res["status"] >= 0

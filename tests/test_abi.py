@@ -41,7 +41,7 @@ def test_library_holds_no_oracle_code():
 
 def test_entry_points_fail_cleanly_without_a_gpu():
     """With no GPU every computing entry point returns PHOTIC_EDEVICE (no CPU fallback); on a
-    GPU box this only checks that the status is a defined code."""
+    machine with a GPU this only checks that the status is a defined code."""
     cfg = P.cfg_default()
     rc = P.lib.photic_invert(C.byref(cfg), 0, None, None, None, None, None, None)
     assert rc in (P.OK, P.EINVAL, P.EDEVICE)
