@@ -97,9 +97,12 @@ typedef struct PhStackArgs {
 
 /* One launch of the spatial-region inversion (ph_region.hip): the pixels idx[q] (or q) of an
  * n_rows x n_cols scene, each inverted with its (2 n_spatial - 1)^2 neighbourhood, one
- * wavefront per pixel.  A wave's block of lds_doubles doubles -- in LDS, or for problems whose
- * block does not fit there at work + wave * lds_doubles in device memory -- holds the simplex
- * [(n_max + 1) n_max] from offset 0, then the arrays at the o_* offsets (in doubles):
+ * wavefront per pixel.  A wave's block of lds_doubles doubles in LDS holds the simplex
+ * [(n_max + 1) n_max] from offset 0, then the arrays at the o_* offsets (in doubles).  For
+ * problems whose block does not fit the LDS (the device-memory instances), the simplex is
+ * at work + wave * work_doubles in device memory and the arrays in LDS (lds_doubles, the
+ * o_* offsets counted from the LDS base) when they fit there, else the whole block is in
+ * device memory (tables_in_lds 0, work_doubles the block).  The arrays:
  * o_x the trial point [n_max], o_d squared residual per pair,
  * o_q rB / r per pair, o_rm measured Rrs per pair, o_pw (440 / lambda)^Y per pair
  * [np_max each; pair j = (region j / n_slots, band slot j % n_slots)], o_k the last region's
@@ -115,6 +118,8 @@ typedef struct PhRegionArgs {
     int n_slots;                   /* band slots: the scenes' band counts summed */
     int r_max, n_max, np_max;      /* regions, parameters, (region, scene, band) pairs */
     int lds_doubles;
+    int work_doubles;              /* device-memory instances: doubles per wave in work */
+    int tables_in_lds;             /* device-memory instances: the arrays are in LDS */
     int o_x, o_d, o_q, o_rm, o_pw, o_k, o_m, o_pix, o_bc, o_geo, o_sc, o_hd, o_bqv, o_bd;
     const double *scn;             /* device tables: sec(sun) [n_scenes], sec(view)
                                       [n_scenes]; per band slot (scene s's band b, the
@@ -125,7 +130,7 @@ typedef struct PhRegionArgs {
                                       [n_slots][PH_REG_SLOT]; per scene its first slot, its
                                       band count and the interpolation (i0, i1, w0, w1) at
                                       440, 490, 550, 640, 750 nm [n_scenes][PH_REG_SCENE] */
-    double *work;                  /* device blocks [grid][lds_doubles] when they live there */
+    double *work;                  /* device blocks [grid][work_doubles] when they live there */
     long n;                        /* pixels to invert */
     const float *rrs;              /* [n_scenes][4][n_rows * n_cols] */
     const int64_t *idx;            /* [n] pixel indices, or NULL: pixel q is q, screened */

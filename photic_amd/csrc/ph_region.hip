@@ -76,7 +76,8 @@ template <int MC>
 struct PhReg {
     // vertex value k lives in lane k % 64, slot k / 64
     static constexpr int YC = MC + 1;
-    double* g; // the wave's LDS block: the simplex, [vertex][coordinate], stride N
+    double* g; // the wave's simplex, [vertex][coordinate], stride N
+    double* t; // the wave's arrays, at the o_* offsets
     int li, R, N, NP, nd, origin;
     int p1;       // band slots of one region: the scenes' band counts summed
     float p1inv;  // 1 / p1
@@ -146,7 +147,7 @@ template <int NBOT, int MC>
 PH_DEV double ph_reg_objective(const PhReg<MC>& S, const PhRegionArgs& a, const PhConst& c,
                                bool side_on, PhRegSide* side)
 {
-    double* g = S.g;
+    double* g = S.t;
     const int li = S.li, R = S.R, NP = S.NP, nd = S.nd;
     const int off = R + 2 * NBOT * R;
     ph_wave_sync(); // the last evaluation's readers are done with the buffers
@@ -675,18 +676,22 @@ __device__ void ph_reg_write_nodata(const PhRegionArgs& a, long q, int n_regions
 } // namespace
 
 // One wavefront per pixel (block = 64 lanes).  GM false: the wave's block of a.lds_doubles
-// doubles is its dynamic LDS; GM true: it is a.work + blockIdx.x * a.lds_doubles in device
-// memory (problems whose simplex does not fit the LDS), read and written by this wave alone.
+// doubles is its dynamic LDS; GM true (problems whose block does not fit the LDS): its simplex
+// is at a.work + blockIdx.x * a.work_doubles in device memory, read and written by this wave
+// alone, and its arrays in LDS when a.tables_in_lds, else after the simplex in device memory.
 template <int NBOT, int MC, bool GM>
 __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
 {
     extern __shared__ double ph_reg_lds[];
     const PhConst& c = a.c;
     PhReg<MC> S;
-    if constexpr (GM)
-        S.g = a.work + (long)blockIdx.x * a.lds_doubles;
-    else
+    if constexpr (GM) {
+        S.g = a.work + (long)blockIdx.x * a.work_doubles;
+        S.t = a.tables_in_lds ? ph_reg_lds : S.g;
+    } else {
         S.g = ph_reg_lds;
+        S.t = ph_reg_lds;
+    }
     S.li = (int)__lane_id();
     const int li = S.li;
     const int nd = a.n_scenes;
@@ -698,7 +703,7 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
     // watchdog: the wall clock at the wave's last progress (a pixel taken); a wave that
     // finishes no pixel within max_ticks gives up and raises the flag
     unsigned long long t0 = wall_clock64();
-    double* g = S.g;
+    double* g = S.t;
     // the per-call tables (ph_launch.h, PhRegionArgs::scn): each band slot's coefficients,
     // each scene's geometry and its band slots, once per wave
     const double* scn_rsig = a.scn + 2 * nd;                              // [slot]
@@ -974,7 +979,7 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
 template <int NBOT, int MC, bool GM>
 static int ph_region_launch_t(const PhRegionArgs* a, int grid, void* stream)
 {
-    const size_t lds = GM ? 0 : sizeof(double) * (size_t)a->lds_doubles;
+    const size_t lds = GM && !a->tables_in_lds ? 0 : sizeof(double) * (size_t)a->lds_doubles;
     if (lds > 65536) {
         const hipError_t e = hipFuncSetAttribute((const void*)ph_region_kernel<NBOT, MC, GM>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -988,7 +993,7 @@ static int ph_region_launch_t(const PhRegionArgs* a, int grid, void* stream)
 template <int NBOT, int MC, bool GM>
 static int ph_region_occ_t(const PhRegionArgs* a, int* bpc)
 {
-    const size_t lds = GM ? 0 : sizeof(double) * (size_t)a->lds_doubles;
+    const size_t lds = GM && !a->tables_in_lds ? 0 : sizeof(double) * (size_t)a->lds_doubles;
     return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
         bpc, (const void*)ph_region_kernel<NBOT, MC, GM>, 64, lds);
 }

@@ -852,11 +852,31 @@ int ph_invert_regions_bands(const photic_cfg *cfg, int n_rows, int n_cols, int n
             hipSuccess || max_lds <= 0)
         max_lds = 65536;
     /* a wave's block lives in LDS when it fits there (and the problem fits two coordinate
-     * slots per lane), else in device memory, one block per wave of the grid; the results are
-     * the same either way.  PHOTIC_REGION_DEVICE_BLOCKS=1 forces device memory (a test hook). */
+     * slots per lane); else its simplex is in device memory, one per wave of the grid, and the
+     * other arrays stay in LDS when they fit there (their offsets then counted from the LDS
+     * base) or follow the simplex.  The results are the same in every form.
+     * PHOTIC_REGION_DEVICE_BLOCKS=1 forces the device-memory simplex, =2 the whole block into
+     * device memory (test hooks). */
     const char *force_dm = getenv("PHOTIC_REGION_DEVICE_BLOCKS");
-    const int in_device = (long)a.lds_doubles * 8 > max_lds || n_max > 128 ||
-                          (force_dm && atoi(force_dm) == 1);
+    const int force = force_dm ? atoi(force_dm) : 0;
+    const int in_device = (long)a.lds_doubles * 8 > max_lds || n_max > 128 || force == 1 ||
+                          force == 2;
+    a.work_doubles = 0;
+    a.tables_in_lds = 0;
+    if (in_device) {
+        const int simplex = (n_max + 1) * n_max; /* even: the arrays stay 16-byte aligned */
+        const long tables = (long)a.lds_doubles - simplex;
+        if (tables * 8 <= max_lds && force != 2) {
+            a.tables_in_lds = 1;
+            a.work_doubles = simplex;
+            a.lds_doubles = (int)tables;
+            int *o[] = {&a.o_x, &a.o_d, &a.o_q, &a.o_rm, &a.o_pw, &a.o_k, &a.o_m,
+                        &a.o_pix, &a.o_bc, &a.o_geo, &a.o_sc, &a.o_hd, &a.o_bqv, &a.o_bd};
+            for (size_t i = 0; i < sizeof(o) / sizeof(o[0]); i++) *o[i] -= simplex;
+        } else {
+            a.work_doubles = a.lds_doubles;
+        }
+    }
     if (hipDeviceGetAttribute(&wrate, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
         wrate <= 0)
         wrate = 100000; /* kHz */
@@ -875,7 +895,7 @@ int ph_invert_regions_bands(const photic_cfg *cfg, int n_rows, int n_cols, int n
      * device-memory blocks one block per wave of the grid */
     const size_t scn_off = 2 * sizeof(unsigned long long);
     const size_t work_off = scn_off + sizeof(double) * scn_n;
-    const size_t bytes = work_off + (in_device ? sizeof(double) * (size_t)a.lds_doubles *
+    const size_t bytes = work_off + (in_device ? sizeof(double) * (size_t)a.work_doubles *
                                                      (size_t)grid : 0);
     char *buf = NULL;
     rc = ph_hip(hipMallocAsync((void **)&buf, bytes, st), "hipMallocAsync");
