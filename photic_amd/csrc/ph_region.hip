@@ -676,11 +676,11 @@ __device__ void ph_reg_write_nodata(const PhRegionArgs& a, long q, int n_regions
 } // namespace
 
 // One wavefront per pixel (block = 64 lanes).  GM false: the wave's block of a.lds_doubles
-// doubles is its dynamic LDS; GM true (problems whose block does not fit the LDS): its simplex
-// is at a.work + blockIdx.x * a.work_doubles in device memory, read and written by this wave
-// alone, and its arrays in LDS when a.tables_in_lds, else after the simplex in device memory.
+// doubles is its dynamic LDS; GM true: its simplex is at a.work + blockIdx.x * a.work_doubles
+// in device memory, read and written by this wave alone, and its arrays in LDS when
+// a.tables_in_lds, else after the simplex in device memory.
 template <int NBOT, int MC, bool GM>
-__global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
+__device__ __forceinline__ void ph_region_body(const PhRegionArgs& a)
 {
     extern __shared__ double ph_reg_lds[];
     const PhConst& c = a.c;
@@ -976,17 +976,48 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
     }
 }
 
+// The kernels: blocks in LDS (one wave per SIMD at two coordinate slots, where the simplexes
+// fill the LDS), and simplexes in device memory with up to four slots compiled for two waves
+// per SIMD -- the LDS then holds only the small arrays, and the second wave hides the first
+// one's latency (n_spatial 2, 3 bottoms: 21.4 k px/s with the blocks in LDS, 23.9 k with the
+// simplexes in device memory at two waves; six slots keep their registers unspilled).  Both
+// take the launch arguments as their only argument (ph_kernarg).
+template <int NBOT, int MC, bool GM>
+__global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
+{
+    ph_region_body<NBOT, MC, GM>(a);
+}
+
+template <int NBOT, int MC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+ph_region_kernel_dm2(PhRegionArgs a)
+{
+    ph_region_body<NBOT, MC, true>(a);
+}
+
+template <int NBOT, int MC, bool GM>
+static const void* ph_region_fn()
+{
+    if constexpr (GM && MC <= 4)
+        return (const void*)ph_region_kernel_dm2<NBOT, MC>;
+    else
+        return (const void*)ph_region_kernel<NBOT, MC, GM>;
+}
+
 template <int NBOT, int MC, bool GM>
 static int ph_region_launch_t(const PhRegionArgs* a, int grid, void* stream)
 {
     const size_t lds = GM && !a->tables_in_lds ? 0 : sizeof(double) * (size_t)a->lds_doubles;
     if (lds > 65536) {
-        const hipError_t e = hipFuncSetAttribute((const void*)ph_region_kernel<NBOT, MC, GM>,
+        const hipError_t e = hipFuncSetAttribute(ph_region_fn<NBOT, MC, GM>(),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    ph_region_kernel<NBOT, MC, GM><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+    if constexpr (GM && MC <= 4)
+        ph_region_kernel_dm2<NBOT, MC><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+    else
+        ph_region_kernel<NBOT, MC, GM><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
     return (int)hipGetLastError();
 }
 
@@ -994,8 +1025,8 @@ template <int NBOT, int MC, bool GM>
 static int ph_region_occ_t(const PhRegionArgs* a, int* bpc)
 {
     const size_t lds = GM && !a->tables_in_lds ? 0 : sizeof(double) * (size_t)a->lds_doubles;
-    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        bpc, (const void*)ph_region_kernel<NBOT, MC, GM>, 64, lds);
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, ph_region_fn<NBOT, MC, GM>(),
+                                                             64, lds);
 }
 
 template <int NBOT, typename Fn>

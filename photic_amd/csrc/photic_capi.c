@@ -851,16 +851,18 @@ int ph_invert_regions_bands(const photic_cfg *cfg, int n_rows, int n_cols, int n
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
             hipSuccess || max_lds <= 0)
         max_lds = 65536;
-    /* a wave's block lives in LDS when it fits there (and the problem fits two coordinate
-     * slots per lane); else its simplex is in device memory, one per wave of the grid, and the
-     * other arrays stay in LDS when they fit there (their offsets then counted from the LDS
-     * base) or follow the simplex.  The results are the same in every form.
-     * PHOTIC_REGION_DEVICE_BLOCKS=1 forces the device-memory simplex, =2 the whole block into
-     * device memory (test hooks). */
+    /* a wave's block lives in LDS when it fits there and the problem fits one coordinate slot
+     * per lane; else its simplex is in device memory, one per wave of the grid, and the other
+     * arrays stay in LDS when they fit there (their offsets then counted from the LDS base) or
+     * follow the simplex.  Problems of two slots (65-128 parameters) fill the LDS with a
+     * single wave per SIMD in the first form and run faster in the second, at two waves per
+     * SIMD.  The results are the same in every form.  PHOTIC_REGION_DEVICE_BLOCKS (test
+     * hooks): 0 keeps blocks of up to two slots in LDS where they fit, 1 forces the
+     * device-memory simplex, 2 the whole block into device memory. */
     const char *force_dm = getenv("PHOTIC_REGION_DEVICE_BLOCKS");
-    const int force = force_dm ? atoi(force_dm) : 0;
-    const int in_device = (long)a.lds_doubles * 8 > max_lds || n_max > 128 || force == 1 ||
-                          force == 2;
+    const int force = force_dm && *force_dm ? atoi(force_dm) : -1;
+    const int fits_lds = (long)a.lds_doubles * 8 <= max_lds && n_max <= 128;
+    const int in_device = force == 1 || force == 2 || !fits_lds || (force != 0 && n_max > 64);
     a.work_doubles = 0;
     a.tables_in_lds = 0;
     if (in_device) {
