@@ -639,6 +639,18 @@ def _region_flops(n_regions, n_evals, nd=1):
     return float((n_evals.astype(np.int64) * (R * nd * 46 * 4 + 15 * nd + 6)).sum())
 
 
+def region_kernel_name(nbot, n_max):
+    """The instance the library launches for a one-scene problem of n_max parameters: blocks in
+    LDS up to 64, simplexes in device memory at two waves per SIMD up to 256 (two or four
+    coordinate slots), then six slots."""
+    mc = (n_max + 63) // 64
+    if mc == 1:
+        return "ph_region_kernel<%d, 1, false>" % nbot
+    if mc <= 4:
+        return "ph_region_kernel_dm2<%d, %d>" % (nbot, 2 if mc == 2 else 4)
+    return "ph_region_kernel<%d, 6, true>" % nbot
+
+
 def bench_regions(args, P):
     """Spatial regions: every pixel of a side x side coherent scene with its neighbourhood."""
     side, ns, nbot = args.region_side, args.region_spatial, args.region_bottoms
@@ -728,8 +740,7 @@ def bench_regions(args, P):
                            pixels_per_s=n / (ms1.value / 1e3)),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "ph_region_kernel<%d, %d>" % (nbot, 1 if (2 * ns - 1) ** 2 *
-                                                              (1 + 2 * nbot) + 3 <= 64 else 2),
+                     "kernel": region_kernel_name(nbot, (2 * ns - 1) ** 2 * (1 + 2 * nbot) + 3),
                      "kernel_ms": k_ms,
                      "flop_per_eval": "R (46 x 4) + 15 + 6 per pixel of R regions",
                      "flop_per_launch": flops},

@@ -122,13 +122,15 @@ def test_regions_pixel_list_and_unsupported():
 
 @pytest.mark.parametrize("form", ["1", "2"])
 def test_regions_device_memory_blocks_match_lds(monkeypatch, form):
-    """The same problem with each wave's simplex in device memory and its other arrays in LDS
-    (form 1), or its whole block in device memory (form 2) -- the layouts the kernels use for
-    problems too large for LDS -- gives every output bit for bit."""
+    """The same problem (66 parameters) with each wave's block in LDS, with its simplex in
+    device memory and its other arrays in LDS (form 1, the layout the library picks for it),
+    or with its whole block in device memory (form 2, for problems whose arrays do not fit the
+    LDS) gives every output bit for bit."""
     nbot, ns, n_rows, n_cols = 3, 2, 6, 6
     rrs, ss, sv = region_scene(91, n_rows, n_cols, 1)
     cfg = P.cfg_default(mode=P.MODE_NATIVE, n_bottoms=nbot, reqmin=1e-2, konvge=100,
                         kcount=5000, check_watchdog=1)
+    monkeypatch.setenv("PHOTIC_REGION_DEVICE_BLOCKS", "0")  # blocks of up to 128 in LDS
     lds = P.invert_regions(cfg, rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns)
     monkeypatch.setenv("PHOTIC_REGION_DEVICE_BLOCKS", form)
     dev = P.invert_regions(cfg, rrs, n_rows, n_cols, -9999.0, ss, sv, n_spatial=ns)
