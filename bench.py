@@ -197,6 +197,8 @@ def main():
     ap.add_argument("--dropin-side", type=int, default=4096)
     ap.add_argument("--ordered-cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--dropin-bottoms", type=int, default=2)
+    ap.add_argument("--dropin-spatial", type=int, default=1,
+                    help="the drop-in's n_spatial (the reference's default is 2)")
     ap.add_argument("--region-side", type=int, default=512)
     ap.add_argument("--region-spatial", type=int, default=2)
     ap.add_argument("--region-bottoms", type=int, default=3)
@@ -771,7 +773,7 @@ def bench_regions(args, P):
 def bench_dropin(args, P):
     """The drop-in samodel() end to end: host grids in, host grids out."""
     from photic_amd.samodel_types import Geogrid, RowGrid, Scene
-    side, nbot = args.dropin_side, args.dropin_bottoms
+    side, nbot, ns = args.dropin_side, args.dropin_bottoms, args.dropin_spatial
     n = side * side
     theta = 35.0
     sec_sun = 1.0 / np.cos(np.radians(theta))
@@ -803,7 +805,7 @@ def bench_dropin(args, P):
     def step():
         outs = {k: RowGrid(np.zeros((side, side), np.float32)) for k in names}
         t0 = time.perf_counter()
-        rc = P.lib.photic_samodel(scenes, gd, idx, 1, 0, Geogrid(), 1, 1, nbot,
+        rc = P.lib.photic_samodel(scenes, gd, idx, 1, 0, Geogrid(), 1, ns, nbot,
                                   *[outs[k].pp for k in names])
         wall = 1e3 * (time.perf_counter() - t0)
         P.check(rc)
@@ -822,11 +824,16 @@ def bench_dropin(args, P):
     v = err > 0.0
     dH = depth[v].astype(np.float64) - truth_H[v]
 
-    # the same inversion with the scene already in HBM (photic_invert, no copies)
-    dev = P.DeviceOutputs(cfg, n, fields=("depth", "status", "n_evals"))
+    # the same inversion with the scene already in HBM (photic_invert or, with regions,
+    # photic_invert_regions; no copies)
     P.check(P.lib.photic_synchronize())
     t0 = time.perf_counter()
-    P.invert_device(cfg, n, d_rrs.value, None, None, dev)
+    if ns <= 1:
+        dev = P.DeviceOutputs(cfg, n, fields=("depth", "status", "n_evals"))
+        P.invert_device(cfg, n, d_rrs.value, None, None, dev)
+    else:
+        dev = P.invert_regions_device(cfg, side, side, 1, d_rrs.value, -9999.0, sec_sun, 1.0,
+                                      ns, n)
     P.check(P.lib.photic_synchronize())
     dev_ms = 1e3 * (time.perf_counter() - t0)
     r = dev.fetch()
@@ -838,10 +845,10 @@ def bench_dropin(args, P):
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (Philox-seeded smooth parameter fields, no real imagery)",
-        "config": {"workload": "samodel() drop-in, %dx%d pixels, one scene, n_spatial 1, "
+        "config": {"workload": "samodel() drop-in, %dx%d pixels, one scene, n_spatial %d, "
                                "%d bottoms, reqmin 1e-2 konvge 100 kcount 5000, with the "
                                "depth-error pass, row-pointer float grids on the host"
-                               % (side, side, nbot),
+                               % (side, side, ns, nbot),
                    "pixels": n, "seed": 1, "noise_sigma": 2e-4, "theta_sun_deg": theta,
                    "parallelism": "1 GPU, persistent work-queue kernel"},
         "modelled_frac": float(v.mean()),
