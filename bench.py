@@ -644,12 +644,13 @@ def _region_flops(n_regions, n_evals, nd=1):
 def region_kernel_name(nbot, n_max):
     """The instance the library launches for a one-scene problem of n_max parameters: blocks in
     LDS up to 64, simplexes in device memory at two waves per SIMD up to 256 (two or four
-    coordinate slots), then six slots."""
+    coordinate slots; a one-scene problem's other arrays fit in LDS, the last template
+    argument 1), then six slots."""
     mc = (n_max + 63) // 64
     if mc == 1:
         return "ph_region_kernel<%d, 1, false>" % nbot
     if mc <= 4:
-        return "ph_region_kernel_dm2<%d, %d>" % (nbot, 2 if mc == 2 else 4)
+        return "ph_region_kernel_dm2<%d, %d, 1>" % (nbot, 2 if mc == 2 else 4)
     return "ph_region_kernel<%d, 6, true>" % nbot
 
 

@@ -678,8 +678,10 @@ __device__ void ph_reg_write_nodata(const PhRegionArgs& a, long q, int n_regions
 // One wavefront per pixel (block = 64 lanes).  GM false: the wave's block of a.lds_doubles
 // doubles is its dynamic LDS; GM true: its simplex is at a.work + blockIdx.x * a.work_doubles
 // in device memory, read and written by this wave alone, and its arrays in LDS when
-// a.tables_in_lds, else after the simplex in device memory.
-template <int NBOT, int MC, bool GM>
+// a.tables_in_lds, else after the simplex in device memory.  TL fixes that choice at compile
+// time (1: LDS, 2: device memory; 0: read a.tables_in_lds), so that the arrays' accesses
+// compile to LDS instructions with 32-bit addresses rather than to flat ones.
+template <int NBOT, int MC, bool GM, int TL = 0>
 __device__ __forceinline__ void ph_region_body(const PhRegionArgs& a)
 {
     extern __shared__ double ph_reg_lds[];
@@ -687,7 +689,9 @@ __device__ __forceinline__ void ph_region_body(const PhRegionArgs& a)
     PhReg<MC> S;
     if constexpr (GM) {
         S.g = a.work + (long)blockIdx.x * a.work_doubles;
-        S.t = a.tables_in_lds ? ph_reg_lds : S.g;
+        if constexpr (TL == 1) S.t = ph_reg_lds;
+        else if constexpr (TL == 2) S.t = S.g;
+        else S.t = a.tables_in_lds ? ph_reg_lds : S.g;
     } else {
         S.g = ph_reg_lds;
         S.t = ph_reg_lds;
@@ -988,20 +992,23 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
     ph_region_body<NBOT, MC, GM>(a);
 }
 
-template <int NBOT, int MC>
+// TL: 1 arrays in LDS, 2 arrays in device memory after the simplex
+template <int NBOT, int MC, int TL>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
 ph_region_kernel_dm2(PhRegionArgs a)
 {
-    ph_region_body<NBOT, MC, true>(a);
+    ph_region_body<NBOT, MC, true, TL>(a);
 }
 
 template <int NBOT, int MC, bool GM>
-static const void* ph_region_fn()
+static const void* ph_region_fn(const PhRegionArgs* a)
 {
-    if constexpr (GM && MC <= 4)
-        return (const void*)ph_region_kernel_dm2<NBOT, MC>;
-    else
+    if constexpr (GM && MC <= 4) {
+        if (a->tables_in_lds) return (const void*)ph_region_kernel_dm2<NBOT, MC, 1>;
+        return (const void*)ph_region_kernel_dm2<NBOT, MC, 2>;
+    } else {
         return (const void*)ph_region_kernel<NBOT, MC, GM>;
+    }
 }
 
 template <int NBOT, int MC, bool GM>
@@ -1009,15 +1016,19 @@ static int ph_region_launch_t(const PhRegionArgs* a, int grid, void* stream)
 {
     const size_t lds = GM && !a->tables_in_lds ? 0 : sizeof(double) * (size_t)a->lds_doubles;
     if (lds > 65536) {
-        const hipError_t e = hipFuncSetAttribute(ph_region_fn<NBOT, MC, GM>(),
+        const hipError_t e = hipFuncSetAttribute(ph_region_fn<NBOT, MC, GM>(a),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    if constexpr (GM && MC <= 4)
-        ph_region_kernel_dm2<NBOT, MC><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
-    else
+    if constexpr (GM && MC <= 4) {
+        if (a->tables_in_lds)
+            ph_region_kernel_dm2<NBOT, MC, 1><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+        else
+            ph_region_kernel_dm2<NBOT, MC, 2><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+    } else {
         ph_region_kernel<NBOT, MC, GM><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+    }
     return (int)hipGetLastError();
 }
 
@@ -1025,7 +1036,7 @@ template <int NBOT, int MC, bool GM>
 static int ph_region_occ_t(const PhRegionArgs* a, int* bpc)
 {
     const size_t lds = GM && !a->tables_in_lds ? 0 : sizeof(double) * (size_t)a->lds_doubles;
-    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, ph_region_fn<NBOT, MC, GM>(),
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, ph_region_fn<NBOT, MC, GM>(a),
                                                              64, lds);
 }
 
