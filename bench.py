@@ -648,10 +648,10 @@ def region_kernel_name(nbot, n_max):
     argument 1), then six slots."""
     mc = (n_max + 63) // 64
     if mc == 1:
-        return "ph_region_kernel<%d, 1, false>" % nbot
+        return "ph_region_kernel<%d, 1, false, 0>" % nbot
     if mc <= 4:
         return "ph_region_kernel_dm2<%d, %d, 1>" % (nbot, 2 if mc == 2 else 4)
-    return "ph_region_kernel<%d, 6, true>" % nbot
+    return "ph_region_kernel<%d, 6, true, 1>" % nbot
 
 
 def bench_regions(args, P):

@@ -677,21 +677,21 @@ __device__ void ph_reg_write_nodata(const PhRegionArgs& a, long q, int n_regions
 
 // One wavefront per pixel (block = 64 lanes).  GM false: the wave's block of a.lds_doubles
 // doubles is its dynamic LDS; GM true: its simplex is at a.work + blockIdx.x * a.work_doubles
-// in device memory, read and written by this wave alone, and its arrays in LDS when
-// a.tables_in_lds, else after the simplex in device memory.  TL fixes that choice at compile
-// time (1: LDS, 2: device memory; 0: read a.tables_in_lds), so that the arrays' accesses
-// compile to LDS instructions with 32-bit addresses rather than to flat ones.
-template <int NBOT, int MC, bool GM, int TL = 0>
+// in device memory, read and written by this wave alone, and its arrays in LDS (TL 1, the
+// launch's a.tables_in_lds) or after the simplex in device memory (TL 2).  The choice is made
+// at compile time so that the arrays' accesses compile to LDS instructions with 32-bit
+// addresses: through a pointer that may address either they would be flat ones.
+template <int NBOT, int MC, bool GM, int TL>
 __device__ __forceinline__ void ph_region_body(const PhRegionArgs& a)
 {
+    static_assert(GM ? (TL == 1 || TL == 2) : TL == 0, "TL: 1 or 2 with GM, else 0");
     extern __shared__ double ph_reg_lds[];
     const PhConst& c = a.c;
     PhReg<MC> S;
     if constexpr (GM) {
         S.g = a.work + (long)blockIdx.x * a.work_doubles;
         if constexpr (TL == 1) S.t = ph_reg_lds;
-        else if constexpr (TL == 2) S.t = S.g;
-        else S.t = a.tables_in_lds ? ph_reg_lds : S.g;
+        else S.t = S.g;
     } else {
         S.g = ph_reg_lds;
         S.t = ph_reg_lds;
@@ -986,10 +986,10 @@ __device__ __forceinline__ void ph_region_body(const PhRegionArgs& a)
 // one's latency (n_spatial 2, 3 bottoms: 21.4 k px/s with the blocks in LDS, 23.9 k with the
 // simplexes in device memory at two waves; six slots keep their registers unspilled).  Both
 // take the launch arguments as their only argument (ph_kernarg).
-template <int NBOT, int MC, bool GM>
+template <int NBOT, int MC, bool GM, int TL>
 __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
 {
-    ph_region_body<NBOT, MC, GM>(a);
+    ph_region_body<NBOT, MC, GM, TL>(a);
 }
 
 // TL: 1 arrays in LDS, 2 arrays in device memory after the simplex
@@ -1006,8 +1006,11 @@ static const void* ph_region_fn(const PhRegionArgs* a)
     if constexpr (GM && MC <= 4) {
         if (a->tables_in_lds) return (const void*)ph_region_kernel_dm2<NBOT, MC, 1>;
         return (const void*)ph_region_kernel_dm2<NBOT, MC, 2>;
+    } else if constexpr (GM) {
+        if (a->tables_in_lds) return (const void*)ph_region_kernel<NBOT, MC, true, 1>;
+        return (const void*)ph_region_kernel<NBOT, MC, true, 2>;
     } else {
-        return (const void*)ph_region_kernel<NBOT, MC, GM>;
+        return (const void*)ph_region_kernel<NBOT, MC, false, 0>;
     }
 }
 
@@ -1026,8 +1029,13 @@ static int ph_region_launch_t(const PhRegionArgs* a, int grid, void* stream)
             ph_region_kernel_dm2<NBOT, MC, 1><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
         else
             ph_region_kernel_dm2<NBOT, MC, 2><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+    } else if constexpr (GM) {
+        if (a->tables_in_lds)
+            ph_region_kernel<NBOT, MC, true, 1><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+        else
+            ph_region_kernel<NBOT, MC, true, 2><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
     } else {
-        ph_region_kernel<NBOT, MC, GM><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+        ph_region_kernel<NBOT, MC, false, 0><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
     }
     return (int)hipGetLastError();
 }
