@@ -643,7 +643,7 @@ def _region_flops(n_regions, n_evals, nd=1):
 
 def region_kernel_name(nbot, n_max):
     """The instance the library launches for a one-scene problem of n_max parameters: blocks in
-    LDS up to 64, simplexes in device memory at two waves per SIMD up to 256 (two or four
+    LDS up to 64, simplexes in device memory at three or two waves per SIMD up to 256 (two or four
     coordinate slots; a one-scene problem's other arrays fit in LDS, the last template
     argument 1), then six slots."""
     mc = (n_max + 63) // 64

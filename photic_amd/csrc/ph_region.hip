@@ -981,11 +981,13 @@ __device__ __forceinline__ void ph_region_body(const PhRegionArgs& a)
 }
 
 // The kernels: blocks in LDS (one wave per SIMD at two coordinate slots, where the simplexes
-// fill the LDS), and simplexes in device memory with up to four slots compiled for two waves
-// per SIMD -- the LDS then holds only the small arrays, and the second wave hides the first
-// one's latency (n_spatial 2, 3 bottoms: 21.4 k px/s with the blocks in LDS, 23.9 k with the
-// simplexes in device memory at two waves; six slots keep their registers unspilled).  Both
-// take the launch arguments as their only argument (ph_kernarg).
+// fill the LDS), and simplexes in device memory with up to four slots compiled for several
+// waves per SIMD -- the LDS then holds only the small arrays, and the other waves hide each
+// one's latency.  Two slots run at three waves (168 registers, 107 spilled: 25.9 k px/s at two
+// waves, 31.1 k at three, 27.1 k at four on a 512^2 scene with n_spatial 2 and 3 bottoms),
+// four slots at two (their spills at three cost more than the wave gains); six slots keep
+// their registers unspilled at one.  Both take the launch arguments as their only argument
+// (ph_kernarg).
 template <int NBOT, int MC, bool GM, int TL>
 __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
 {
@@ -993,8 +995,10 @@ __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
 }
 
 // TL: 1 arrays in LDS, 2 arrays in device memory after the simplex
+constexpr int ph_reg_dm_waves(int mc) { return mc <= 2 ? 3 : 2; }
 template <int NBOT, int MC, int TL>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(ph_reg_dm_waves(MC), ph_reg_dm_waves(MC))))
 ph_region_kernel_dm2(PhRegionArgs a)
 {
     ph_region_body<NBOT, MC, true, TL>(a);

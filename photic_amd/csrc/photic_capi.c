@@ -855,8 +855,8 @@ int ph_invert_regions_bands(const photic_cfg *cfg, int n_rows, int n_cols, int n
      * per lane; else its simplex is in device memory, one per wave of the grid, and the other
      * arrays stay in LDS when they fit there (their offsets then counted from the LDS base) or
      * follow the simplex.  Problems of two slots (65-128 parameters) fill the LDS with a
-     * single wave per SIMD in the first form and run faster in the second, at two waves per
-     * SIMD.  The results are the same in every form.  PHOTIC_REGION_DEVICE_BLOCKS (test
+     * single wave per SIMD in the first form and run faster in the second, at three waves
+     * per SIMD.  The results are the same in every form.  PHOTIC_REGION_DEVICE_BLOCKS (test
      * hooks): 0 keeps blocks of up to two slots in LDS where they fit, 1 forces the
      * device-memory simplex, 2 the whole block into device memory. */
     const char *force_dm = getenv("PHOTIC_REGION_DEVICE_BLOCKS");
