@@ -645,13 +645,18 @@ def region_kernel_name(nbot, n_max):
     """The instance the library launches for a one-scene problem of n_max parameters: blocks in
     LDS up to 64, simplexes in device memory at three or two waves per SIMD up to 256 (two or four
     coordinate slots; a one-scene problem's other arrays fit in LDS, the last template
-    argument 1), then six slots."""
+    argument 1), then six slots.  PHOTIC_REGION_DEVICE_BLOCKS, the library's placement switch,
+    is followed: 0 keeps blocks of up to 128 parameters in LDS, 1 puts every simplex in device
+    memory, 2 every whole block."""
+    force = os.environ.get("PHOTIC_REGION_DEVICE_BLOCKS", "")
+    force = int(force) if force.lstrip("-").isdigit() else -1
     mc = (n_max + 63) // 64
-    if mc == 1:
-        return "ph_region_kernel<%d, 1, false, 0>" % nbot
+    if mc <= 2 and (force == 0 or (force not in (1, 2) and mc == 1)):
+        return "ph_region_kernel<%d, %d, false, 0>" % (nbot, mc)
+    tl = 2 if force == 2 else 1
     if mc <= 4:
-        return "ph_region_kernel_dm2<%d, %d, 1>" % (nbot, 2 if mc == 2 else 4)
-    return "ph_region_kernel<%d, 6, true, 1>" % nbot
+        return "ph_region_kernel_dm2<%d, %d, %d>" % (nbot, 2 if mc <= 2 else 4, tl)
+    return "ph_region_kernel<%d, 6, true, %d>" % (nbot, tl)
 
 
 def bench_regions(args, P):
