@@ -651,8 +651,11 @@ def region_kernel_name(nbot, n_max):
     force = os.environ.get("PHOTIC_REGION_DEVICE_BLOCKS", "")
     force = int(force) if force.lstrip("-").isdigit() else -1
     mc = (n_max + 63) // 64
-    if mc <= 2 and (force == 0 or (force not in (1, 2) and mc == 1)):
-        return "ph_region_kernel<%d, %d, false, 0>" % (nbot, mc)
+    # one-slot blocks stay in LDS where 12 fit a CU: for one scene, up to about 40 parameters
+    if mc == 1 and (force == 0 or (force not in (1, 2) and n_max <= 40)):
+        return "ph_region_kernel_lds3<%d>" % nbot
+    if mc == 2 and force == 0:
+        return "ph_region_kernel<%d, 2, false, 0>" % nbot
     tl = 2 if force == 2 else 1
     if mc <= 4:
         return "ph_region_kernel_dm2<%d, %d, %d>" % (nbot, 2 if mc <= 2 else 4, tl)

@@ -980,18 +980,27 @@ __device__ __forceinline__ void ph_region_body(const PhRegionArgs& a)
     }
 }
 
-// The kernels: blocks in LDS (one wave per SIMD at two coordinate slots, where the simplexes
-// fill the LDS), and simplexes in device memory with up to four slots compiled for several
-// waves per SIMD -- the LDS then holds only the small arrays, and the other waves hide each
-// one's latency.  Two slots run at three waves (168 registers, 107 spilled: 25.9 k px/s at two
-// waves, 31.1 k at three, 27.1 k at four on a 512^2 scene with n_spatial 2 and 3 bottoms),
-// four slots at two (their spills at three cost more than the wave gains); six slots keep
-// their registers unspilled at one.  Both take the launch arguments as their only argument
-// (ph_kernarg).
+// The kernels.  Blocks in LDS: one coordinate slot compiled for three waves per SIMD, used
+// where the LDS holds three blocks per SIMD (30 parameters: 101 -> 128 k px/s), and two slots
+// at one wave per SIMD, where the simplexes fill the LDS.  Simplexes in device memory with up
+// to four slots, compiled for several waves per SIMD: the LDS then holds only the small
+// arrays, and the other waves hide each one's latency.  Two slots run at three waves (168
+// registers, 107 spilled: 25.9 k px/s at two waves, 31.1 k at three, 27.1 k at four on a
+// 512^2 scene with n_spatial 2 and 3 bottoms), four slots at two (their spills at three cost
+// more than the wave gains); six slots keep their registers unspilled at one.  All take the
+// launch arguments as their only argument (ph_kernarg).
 template <int NBOT, int MC, bool GM, int TL>
 __global__ void __launch_bounds__(64) ph_region_kernel(PhRegionArgs a)
 {
     ph_region_body<NBOT, MC, GM, TL>(a);
+}
+
+// one coordinate slot, blocks in LDS, compiled for three waves per SIMD
+template <int NBOT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3)))
+ph_region_kernel_lds3(PhRegionArgs a)
+{
+    ph_region_body<NBOT, 1, false, 0>(a);
 }
 
 // TL: 1 arrays in LDS, 2 arrays in device memory after the simplex
@@ -1013,6 +1022,8 @@ static const void* ph_region_fn(const PhRegionArgs* a)
     } else if constexpr (GM) {
         if (a->tables_in_lds) return (const void*)ph_region_kernel<NBOT, MC, true, 1>;
         return (const void*)ph_region_kernel<NBOT, MC, true, 2>;
+    } else if constexpr (MC == 1) {
+        return (const void*)ph_region_kernel_lds3<NBOT>;
     } else {
         return (const void*)ph_region_kernel<NBOT, MC, false, 0>;
     }
@@ -1038,6 +1049,8 @@ static int ph_region_launch_t(const PhRegionArgs* a, int grid, void* stream)
             ph_region_kernel<NBOT, MC, true, 1><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
         else
             ph_region_kernel<NBOT, MC, true, 2><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
+    } else if constexpr (MC == 1) {
+        ph_region_kernel_lds3<NBOT><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
     } else {
         ph_region_kernel<NBOT, MC, false, 0><<<dim3(grid), dim3(64), lds, (hipStream_t)stream>>>(*a);
     }

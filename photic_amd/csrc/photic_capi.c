@@ -856,13 +856,18 @@ int ph_invert_regions_bands(const photic_cfg *cfg, int n_rows, int n_cols, int n
      * arrays stay in LDS when they fit there (their offsets then counted from the LDS base) or
      * follow the simplex.  Problems of two slots (65-128 parameters) fill the LDS with a
      * single wave per SIMD in the first form and run faster in the second, at three waves
-     * per SIMD.  The results are the same in every form.  PHOTIC_REGION_DEVICE_BLOCKS (test
-     * hooks): 0 keeps blocks of up to two slots in LDS where they fit, 1 forces the
-     * device-memory simplex, 2 the whole block into device memory. */
+     * per SIMD.  One-slot problems stay in LDS when three blocks per SIMD fit there (their
+     * kernel is compiled for three waves); larger one-slot blocks, which the LDS would hold
+     * to fewer than two waves per SIMD, run faster in the device-memory form.  The results
+     * are the same in every form.  PHOTIC_REGION_DEVICE_BLOCKS (test hooks): 0 keeps blocks
+     * of up to two slots in LDS where they fit, 1 forces the device-memory simplex, 2 the
+     * whole block into device memory. */
     const char *force_dm = getenv("PHOTIC_REGION_DEVICE_BLOCKS");
     const int force = force_dm && *force_dm ? atoi(force_dm) : -1;
     const int fits_lds = (long)a.lds_doubles * 8 <= max_lds && n_max <= 128;
-    const int in_device = force == 1 || force == 2 || !fits_lds || (force != 0 && n_max > 64);
+    const int fits_lds3 = (long)a.lds_doubles * 8 * 12 <= max_lds; /* 3 waves x 4 SIMDs */
+    const int in_device = force == 1 || force == 2 || !fits_lds ||
+                          (force != 0 && (n_max > 64 || !fits_lds3));
     a.work_doubles = 0;
     a.tables_in_lds = 0;
     if (in_device) {
