@@ -190,9 +190,9 @@ int photic_invert_stack_h(const photic_cfg *cfg, long n_pix, int n_dates, const 
  * (0: not modelled, status -1).  PHOTIC_MODE_NATIVE with 1..3 bottoms, 4 bands, FP64, 1..32
  * scenes (the reference's MAX_SCENES), n_spatial <= 4 (R_max <= 64) and R_max (1 + 2
  * n_bottoms) + 3 n_scenes <= 384 parameters.  A pixel's simplex and working arrays live in
- * LDS for problems of at most 64 parameters; larger problems keep the simplex in device
- * memory allocated for the call (one per wavefront of the launch) and the working arrays in
- * LDS where they fit; the results are the same. */
+ * LDS for problems of at most 64 parameters whose blocks fit three per SIMD; other problems
+ * keep the simplex in device memory allocated for the call (one per wavefront of the launch)
+ * and the working arrays in LDS where they fit; the results are the same. */
 int photic_invert_regions(const photic_cfg *cfg, int n_rows, int n_cols, int n_scenes,
                           const float *rrs, const float *nodata, const double *sec_sun,
                           const double *sec_view, int n_spatial, long n, const int64_t *idx,
